@@ -1,0 +1,72 @@
+/*
+ * jxr_hip_encode.h — forward analysis into a coefficient arena.
+ *
+ * The encoder's transform and quantization stages without entropy coding:
+ * colour transform, chroma decimation, overlap prefilters, forward core
+ * transforms, quantization and high-pass prediction residuals, laid out as
+ * the reference's entropy decoder lays out an arena (component-major,
+ * tile-major, crates/jxr-native/src/tile_decode/output.rs:232-299). It
+ * stands in for the reference encoder (crates/jxr-native/src/encode,
+ * src/analyze.rs) so that seeded synthetic images become the arenas the
+ * reconstruction consumes; with all steps 1 and no chroma decimation the
+ * reconstruction returns the input exactly.
+ */
+#ifndef JXR_HIP_ENCODE_H
+#define JXR_HIP_ENCODE_H
+
+#include "jxr_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JXRHIP_INPUT_U8 0u
+#define JXRHIP_INPUT_U16 1u
+#define JXRHIP_INPUT_F32 2u
+
+typedef struct jxrhip_encode_params {
+  uint32_t width;          /* pixels                                          */
+  uint32_t height;
+  uint32_t input_kind;     /* JXRHIP_INPUT_*                                  */
+  uint32_t input_channels; /* 1 gray, 2 gray+alpha, 3 RGB, 4 RGBA (interleaved) */
+  uint32_t color_format;   /* JXRHIP_COLOR_LUMA or JXRHIP_COLOR_YUV420/422/444 */
+  uint32_t alpha_mode;     /* JXRHIP_ALPHA_* for inputs with alpha            */
+  uint32_t overlap;        /* 0, 1, 2                                         */
+  uint32_t hard_tiles;
+  uint32_t tile_column_count; /* 0 = one tile                                 */
+  const uint32_t *tile_column_widths; /* macroblocks                         */
+  uint32_t tile_row_count;
+  const uint32_t *tile_row_heights;
+  uint32_t scaled;         /* SCALED_FLAG                                     */
+  uint32_t qp_dc;          /* coded QP indices, 0 = step 1 (lossless)         */
+  uint32_t qp_lp;
+  uint32_t qp_hp;
+  uint32_t bands;          /* JXRHIP_BANDS_*                                  */
+  uint32_t shift_bits;     /* fixed-point inputs                              */
+  uint32_t mantissa_length; /* f32 inputs                                     */
+  int32_t exponent_bias;
+  uint32_t alpha_qp;       /* QP of the alpha plane (all bands)               */
+  uint32_t seed;           /* varies quantizer sets between macroblocks       */
+  uint32_t vary_quantizers; /* nonzero: per-tile and per-macroblock steps vary */
+} jxrhip_encode_params;
+
+typedef struct jxrhip_encoded jxrhip_encoded;
+
+jxrhip_status jxrhip_encode(const void *pixels, uint64_t pixel_bytes,
+                            const jxrhip_encode_params *params, jxrhip_encoded **out);
+void jxrhip_encoded_destroy(jxrhip_encoded *encoded);
+/* which = 0: primary arena; 1: separate alpha arena (NULL if none). The
+ * returned struct and everything it points to live as long as `encoded`. */
+const jxrhip_arena *jxrhip_encoded_arena(const jxrhip_encoded *encoded, uint32_t which);
+/* Quantization step of a coded QP (T.832 QP mapping,
+ * crates/jxr-native/src/tile_decode/quantizer.rs:309-331). */
+uint32_t jxrhip_quant_step(uint32_t qp, uint32_t scaled, uint32_t scaled_shift);
+/* The reconstructed sample whose f32 output is nearest `bits`
+ * (crates/jxr-native/src/output_format/scaling.rs:177-229). */
+jxrhip_status jxrhip_float32_sample(uint32_t bits, uint32_t scaled, uint32_t mantissa_length,
+                                    int32_t exponent_bias, int32_t *sample);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,119 @@
+// recon_abi.h — device-side descriptors shared by the host runtime and the
+// gfx950 reconstruction kernels. The host validates a plan and builds these
+// once; the kernels only read them.
+#pragma once
+#include <stdint.h>
+
+namespace jxr {
+
+constexpr uint32_t kMaxPlanes = 5;     // Y, U, V (or CMYK/K), alpha
+constexpr uint32_t kThreads = 256;     // threads per workgroup (4 wave64)
+
+// Raster-ordered macroblock record (the reference's JxrMacroblockAbi words,
+// crates/jxr-math/data/reconstruction.abi:3, without coded_x/y because the
+// record's index is its position).
+struct alignas(32) MbRec {
+  uint32_t coef_off;  // i32 index into the plane's arena coefficients
+  uint32_t qdc, qlp, qhp;
+  uint32_t bands;     // JXRHIP_BANDS_*
+  uint32_t pred;      // JXRHIP_PRED_* (NONE when the high-pass band is absent)
+  uint32_t pad0, pad1;
+};
+
+struct DevPlane {
+  const int32_t *coef;     // arena coefficients
+  const MbRec *mbs;        // [mby][mbx]
+  const uint32_t *colmask; // bit m: hard tile boundary at MB column m (or null)
+  const uint32_t *rowmask;
+  uint32_t bc, br;         // blocks across / down one macroblock
+  uint32_t scale_first;    // x2 after the first-level transform
+  uint32_t overlap;        // 0, 1, 2
+  uint32_t alpha;
+  uint32_t pad[3];
+};
+
+// Bias, rounding, shift, postscale of one channel (scaling.rs:42-124).
+struct ChanScale {
+  int32_t shifted_bias;
+  int32_t rounding;
+  uint32_t shift;
+  uint32_t post_shift;
+  uint32_t flags;  // 1 invalid, 2 padding, 4 post-shift present
+  uint32_t pad[3];
+};
+
+enum Conversion : uint32_t {
+  kConvLuma = 0,      // Luma -> Luma
+  kConvLumaRgb = 1,   // Luma -> RGB (replicated)
+  kConvYuvRgb = 2,    // YUV -> RGB/RGBE
+  kConvYuvkCmyk = 3,  // YUVK -> CMYK
+  kConvYuvkDirect = 4,// YUVK -> CMYK direct
+  kConvDirect = 5,    // N-component direct
+};
+
+struct alignas(64) DevImage {
+  DevPlane planes[kMaxPlanes];
+  uint32_t plane_count;
+  uint32_t colors;         // colour planes
+  int32_t alpha_plane;     // plane index of alpha, -1 none
+  uint32_t mbx, mby;       // macroblock window
+  uint32_t upsample;       // 0 none, 1 = 4:2:0, 2 = 4:2:2
+  uint32_t crop_x, crop_y; // crop, relative to the window's sample origin
+  uint32_t crop_w, crop_h;
+  uint32_t conv;           // Conversion
+  uint32_t primary_count;  // primaries of the layout
+  uint32_t channels;       // stored samples per pixel
+  uint32_t has_alpha;      // layout stores alpha as its last channel
+  uint32_t reverse;        // BGR order
+  uint32_t storage;        // JXRHIP_STORE_*
+  uint32_t bytes;          // bytes per stored sample
+  uint32_t planar;         // channel c -> surface plane c
+  int32_t u16_max;
+  uint32_t premultiply;
+  uint32_t centering_x, centering_y;
+  uint32_t f_scaled, f_alpha_scaled;       // F16/F32 scaled flags
+  uint32_t f_mant, f_alpha_mant;           // F32 mantissa lengths
+  int32_t f_bias, f_alpha_bias;            // F32 exponent biases
+  uint32_t pad0, pad1;
+  uint64_t plane_offset[kMaxPlanes];       // output surface planes
+  uint64_t row_stride[kMaxPlanes];
+  ChanScale color_scale[8];
+  ChanScale alpha_scale;
+};
+
+// One image of a launch: descriptor, output and status word.
+struct alignas(32) Slot {
+  const DevImage *img;
+  uint8_t *out;
+  uint32_t *status;
+  uint32_t tiles_x;     // tiles across the window for this launch's tile shape
+  uint32_t tile_start;  // first work item of this image in a persistent launch
+};
+
+struct KernelArgs {
+  Slot inline_slot;      // per-image launch: the one image
+  const Slot *slots;     // persistent: images, ordered by tile_start
+  uint32_t slot_count;
+  uint32_t item_count;   // persistent: total tiles
+  uint32_t *counter;     // persistent: next work item
+  uint32_t persistent;
+  uint32_t tile_x, tile_y;  // macroblocks per tile
+  uint32_t lds_words;    // dynamic LDS in 32-bit words
+};
+
+// Halo of the low-pass region in macroblocks: the subsampled first-level
+// overlap reads a corner's neighbour one pair beyond the block halo.
+__host__ __device__ inline uint32_t low_halo(uint32_t bc) { return bc == 2 ? 2u : 1u; }
+
+// Dynamic LDS words one workgroup needs for a tile shape and plane set.
+__host__ __device__ inline uint32_t sample_region_words(uint32_t bc, uint32_t br, uint32_t tx,
+                                                        uint32_t ty) {
+  return (tx * bc * 4 + 8) * (ty * br * 4 + 8);
+}
+__host__ __device__ inline uint32_t low_region_words(uint32_t bc, uint32_t br, uint32_t tx,
+                                                     uint32_t ty) {
+  uint32_t h = low_halo(bc);
+  return (tx + 2 * h) * bc * (ty + 2 * h) * br;
+}
+
+}  // namespace jxr

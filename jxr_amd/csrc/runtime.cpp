@@ -1,0 +1,1258 @@
+// runtime.cpp — host runtime of the gfx950 reconstruction backend: plans,
+// sessions, uploads, submissions and batches behind the C ABI of
+// include/jxr_hip.h.
+//
+// Reference behaviour kept (crates/jxr-cuda/src):
+//   plan validation before any device resource exists   plan.rs:91-354
+//   immutable coefficient uploads cached by identity      upload_cache.rs:15-153
+//   exact-size scratch pool                               buffer_pool.rs:9-129
+//   status word per submission, first failing stage       submission.rs:141-154
+//   several streams, independent images round-robin       session.rs:112-123
+//   route threshold of 16,384 coefficients                route.rs:8-50
+// New here: one persistent launch per shape bucket over a whole batch, and
+// 64-bit sizes throughout.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <list>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/jxr_hip.h"
+#include "recon_abi.h"
+
+namespace jxr {
+hipError_t launch_tiles(const KernelArgs &args, uint32_t grid, hipStream_t stream);
+hipError_t launch_reset(uint32_t *counter, hipStream_t stream);
+hipError_t set_kernel_lds_limit(size_t bytes);
+}  // namespace jxr
+
+using namespace jxr;
+
+// ---- errors -----------------------------------------------------------------
+static thread_local std::string g_error;
+extern "C" void jxrhip_set_error(const char *message) { g_error = message ? message : ""; }
+extern "C" const char *jxrhip_last_error(void) { return g_error.c_str(); }
+extern "C" uint32_t jxrhip_abi_version(void) { return JXRHIP_ABI_VERSION; }
+
+static jxrhip_status fail(jxrhip_status code, const std::string &message) {
+  g_error = message;
+  return code;
+}
+static jxrhip_status hip_fail(hipError_t e, const char *what) {
+  g_error = std::string(what) + ": " + hipGetErrorString(e);
+  return JXRHIP_ERR_DEVICE;
+}
+#define HIP_TRY(expr, what)                   \
+  do {                                        \
+    hipError_t e_ = (expr);                   \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+// ---- routing (route.rs:8-50) ------------------------------------------------
+static constexpr uint64_t kAutoThreshold = 16384;
+extern "C" uint64_t jxrhip_route_threshold(void) { return kAutoThreshold; }
+extern "C" jxrhip_status jxrhip_route(uint64_t coefficients, int32_t strict, int32_t available,
+                                      int32_t *use_device) {
+  if (!use_device) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null output");
+  if (strict) {
+    if (!available) return fail(JXRHIP_ERR_UNAVAILABLE, "strict device route without a usable device");
+    *use_device = 1;
+    return JXRHIP_OK;
+  }
+  *use_device = available && coefficients >= kAutoThreshold ? 1 : 0;
+  return JXRHIP_OK;
+}
+
+extern "C" int32_t jxrhip_is_available(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  for (int d = 0; d < n; d++) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, d) == hipSuccess && std::strncmp(prop.gcnArchName, "gfx950", 6) == 0)
+      return 1;
+  }
+  return 0;
+}
+
+// ---- plans ---------------------------------------------------------------
+struct PlanPlane {
+  uint32_t arena;  // 0 primary, 1 separate alpha
+  uint32_t bc, br;
+  bool scale_first = false, alpha = false;
+  uint32_t overlap = 0;
+  std::vector<MbRec> recs;                 // raster order
+  std::vector<uint32_t> colmask, rowmask;  // empty for soft tiles
+};
+
+static std::atomic<uint64_t> g_plan_ids{1};
+
+struct jxrhip_plan {
+  uint64_t id = 0;
+  const jxrhip_arena *arenas[2] = {nullptr, nullptr};
+  uint32_t arena_count = 0;
+  std::vector<PlanPlane> planes;
+  DevImage dev{};  // device pointers filled per session at upload
+  uint32_t mbx = 0, mby = 0;
+  uint64_t output_bytes = 0, recon_coefs = 0, pixels = 0, alg_bytes = 0;
+  uint64_t descriptor_bytes = 0;
+  uint64_t coefficient_bytes = 0;
+  uint64_t geometry_key = 0;  // bucket key: plane geometries
+};
+
+static int32_t bias_of(uint32_t depth, uint32_t shift_bits, bool *bad) {
+  int32_t base = 0;
+  uint32_t shift = 0;
+  switch (depth) {
+    case JXRHIP_DEPTH_RGB555: base = 16; break;
+    case JXRHIP_DEPTH_RGB565: base = 32; break;
+    case JXRHIP_DEPTH_U8: base = 128; break;
+    case JXRHIP_DEPTH_U10:
+    case JXRHIP_DEPTH_RGB101010: base = 512; break;
+    case JXRHIP_DEPTH_U16: base = 32768; shift = shift_bits; break;
+    case JXRHIP_DEPTH_I16:
+    case JXRHIP_DEPTH_I32: shift = shift_bits; break;
+    default: break;
+  }
+  if (shift >= 32) {
+    *bad = true;
+    return 0;
+  }
+  return base >> shift;
+}
+
+// ChannelScale::resolve (crates/jxr-native/src/output_format/scaling.rs:42-97)
+static ChanScale make_scale(uint32_t component, uint32_t output_color, uint32_t depth,
+                            uint32_t shift_bits, uint32_t scaled) {
+  ChanScale s{};
+  bool bad = false;
+  int32_t bias = bias_of(depth, shift_bits, &bad);
+  if (bad) {
+    s.flags = 1;
+    return s;
+  }
+  int32_t cb = bias;
+  if (output_color == JXRHIP_COLOR_CMYK) cb = component < 3 ? (bias >> 1) : -(bias >> 1);
+  else if (output_color == JXRHIP_COLOR_RGBE) cb = 0;
+  uint32_t scale = scaled ? 3 : 0;
+  int64_t sh = (int64_t)cb * ((int64_t)1 << scale);
+  if (sh > INT32_MAX || sh < INT32_MIN) {
+    s.flags = 1;
+    return s;
+  }
+  s.shifted_bias = (int32_t)sh;
+  s.rounding = scaled ? ((depth == JXRHIP_DEPTH_BIT1_WHITE || depth == JXRHIP_DEPTH_BIT1_BLACK ||
+                          depth == JXRHIP_DEPTH_U16)
+                             ? 4
+                             : 3)
+                      : 0;
+  s.shift = (depth == JXRHIP_DEPTH_RGB565 && component != 1) ? scale + 1 : scale;
+  if (depth == JXRHIP_DEPTH_U16 || depth == JXRHIP_DEPTH_I16 || depth == JXRHIP_DEPTH_I32) {
+    s.flags |= 4;
+    s.post_shift = shift_bits;
+  }
+  return s;
+}
+
+static bool layout_has_alpha(uint32_t l) {
+  return l == JXRHIP_LAYOUT_LUMA_ALPHA || l == JXRHIP_LAYOUT_YUVA || l == JXRHIP_LAYOUT_RGBA ||
+         l == JXRHIP_LAYOUT_BGRA || l == JXRHIP_LAYOUT_CMYKA || l == JXRHIP_LAYOUT_NCOMPONENT_ALPHA;
+}
+
+// Clipped tile boundaries of one axis (crates/jxr-core/src/device_plan/
+// overlap.rs:102-123) as a bitmask over macroblock positions.
+static bool tile_mask(const uint32_t *sizes, uint32_t n, uint32_t origin, uint32_t extent,
+                      std::vector<uint32_t> &mask) {
+  mask.assign(extent / 32 + 2, 0u);
+  uint64_t end = (uint64_t)origin + extent, pos = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    pos += sizes[i];
+    if (pos > origin && pos < end) {
+      uint64_t b = pos - origin;
+      mask[b >> 5] |= 1u << (b & 31);
+    }
+  }
+  return pos >= end;
+}
+
+static jxrhip_status build_plane(jxrhip_plan *plan, const jxrhip_arena *a, uint32_t arena_index,
+                                 uint32_t index, bool scale, bool alpha, uint32_t *ox, uint32_t *oy) {
+  if (index >= a->plane_count) return fail(JXRHIP_ERR_INVALID_PLAN, "coefficient plane index is absent");
+  const jxrhip_coefficient_plane &cp = a->planes[index];
+  PlanPlane pp;
+  pp.arena = arena_index;
+  pp.bc = cp.block_columns;
+  pp.br = cp.block_rows;
+  pp.scale_first = scale;
+  pp.alpha = alpha;
+  pp.overlap = a->overlap;
+  if (!((pp.bc == 4 && pp.br == 4) || (pp.bc == 2 && pp.br == 4) || (pp.bc == 2 && pp.br == 2)))
+    return fail(JXRHIP_ERR_INVALID_PLAN, "unsupported component macroblock geometry");
+  uint64_t mo = cp.macroblock_offset, mc = cp.macroblock_count;
+  if (mc == 0 || mo + mc > a->macroblock_count || mo + mc < mo)
+    return fail(JXRHIP_ERR_INVALID_PLAN, "macroblock plane range");
+  if (cp.coefficient_offset + cp.coefficient_count > a->coefficient_count)
+    return fail(JXRHIP_ERR_INVALID_PLAN, "coefficient plane range");
+  uint32_t x0 = UINT32_MAX, y0 = UINT32_MAX, x1 = 0, y1 = 0;
+  for (uint64_t i = mo; i < mo + mc; i++) {
+    x0 = std::min(x0, a->coded_x[i]);
+    y0 = std::min(y0, a->coded_y[i]);
+    x1 = std::max(x1, a->coded_x[i]);
+    y1 = std::max(y1, a->coded_y[i]);
+  }
+  const uint32_t mbx = x1 - x0 + 1, mby = y1 - y0 + 1;
+  if ((uint64_t)mbx * mby != mc)
+    return fail(JXRHIP_ERR_INVALID_PLAN, "coefficient plane reconstruction window is not rectangular");
+  if (plan->mbx == 0) {
+    plan->mbx = mbx;
+    plan->mby = mby;
+    *ox = x0;
+    *oy = y0;
+  } else if (plan->mbx != mbx || plan->mby != mby || *ox != x0 || *oy != y0) {
+    return fail(JXRHIP_ERR_INVALID_PLAN, "planes do not share one macroblock window");
+  }
+  if ((uint64_t)mbx * 16 > INT32_MAX / 2 || (uint64_t)mby * 16 > INT32_MAX / 2)
+    return fail(JXRHIP_ERR_INVALID_PLAN, "sample coordinates exceed signed indexing");
+  const uint32_t blocks = pp.bc * pp.br;
+  pp.recs.assign(mc, MbRec{UINT32_MAX, 0, 0, 0, 0, 0, 0, 0});
+  std::vector<uint8_t> seen(mc, 0);
+  for (uint64_t i = mo; i < mo + mc; i++) {
+    uint64_t r = (uint64_t)(a->coded_y[i] - y0) * mbx + (a->coded_x[i] - x0);
+    if (seen[r]) return fail(JXRHIP_ERR_INVALID_PLAN, "coefficient plane repeats a macroblock coordinate");
+    seen[r] = 1;
+    uint32_t bands = a->bands[i];
+    if (bands > 3) return fail(JXRHIP_ERR_INVALID_PLAN, "band presence code");
+    uint64_t need = bands == 0 ? 1 : bands == 1 ? blocks : (uint64_t)blocks * 16;
+    if ((uint64_t)a->coefficient_offsets[i] + need > a->coefficient_count)
+      return fail(JXRHIP_ERR_INVALID_PLAN, "macroblock coefficient span exceeds the uploaded arena");
+    uint32_t pred = a->hp_predictions[i];
+    if (pred > JXRHIP_PRED_FROM_TOP) return fail(JXRHIP_ERR_INVALID_PLAN, "top-left high-pass prediction is invalid");
+    const uint32_t *q = a->quantizers + 3 * i;
+    if (q[0] == 0 || q[1] == 0 || q[2] == 0) return fail(JXRHIP_ERR_INVALID_PLAN, "zero quantizer step");
+    MbRec &rec = pp.recs[r];
+    rec.coef_off = a->coefficient_offsets[i];
+    rec.qdc = q[0];
+    rec.qlp = q[1];
+    rec.qhp = q[2];
+    rec.bands = bands;
+    rec.pred = bands >= JXRHIP_BANDS_NO_FLEXBITS ? pred : JXRHIP_PRED_NONE;
+  }
+  if (a->hard_tiles) {
+    if (!tile_mask(a->tile_column_widths, a->tile_column_count, x0, mbx, pp.colmask) ||
+        !tile_mask(a->tile_row_heights, a->tile_row_count, y0, mby, pp.rowmask))
+      return fail(JXRHIP_ERR_INVALID_PLAN, "tile partition does not cover reconstruction window");
+  }
+  if (a->overlap > 2) return fail(JXRHIP_ERR_INVALID_PLAN, "overlap mode");
+  plan->planes.push_back(std::move(pp));
+  return JXRHIP_OK;
+}
+
+static jxrhip_status validate_arena(const jxrhip_arena *a) {
+  if (!a || !a->coefficients || !a->coefficient_offsets || !a->quantizers || !a->bands ||
+      !a->hp_predictions || !a->coded_x || !a->coded_y || !a->planes)
+    return fail(JXRHIP_ERR_INVALID_PLAN, "coefficient arena has a null array");
+  if (a->coefficient_count > UINT32_MAX)
+    return fail(JXRHIP_ERR_INVALID_PLAN, "coefficient arena exceeds 2^32 coefficients");
+  if (a->hard_tiles && (!a->tile_column_widths || !a->tile_row_heights))
+    return fail(JXRHIP_ERR_INVALID_PLAN, "hard tiles without a tile grid");
+  return JXRHIP_OK;
+}
+
+extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_plan **out) {
+  if (!d || !out) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (d->abi_version != JXRHIP_ABI_VERSION) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "ABI version mismatch");
+  std::unique_ptr<jxrhip_plan> plan(new (std::nothrow) jxrhip_plan());
+  if (!plan) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "out of host memory");
+  plan->id = g_plan_ids.fetch_add(1);
+  jxrhip_status st = validate_arena(d->primary);
+  if (st) return st;
+  const uint32_t cf = d->color_format;
+  const uint32_t colors = d->component_count;
+  const uint32_t expect = cf == JXRHIP_COLOR_LUMA ? 1
+                          : (cf == JXRHIP_COLOR_YUV420 || cf == JXRHIP_COLOR_YUV422 ||
+                             cf == JXRHIP_COLOR_YUV444 || cf == JXRHIP_COLOR_RGB || cf == JXRHIP_COLOR_RGBE)
+                              ? 3
+                          : (cf == JXRHIP_COLOR_CMYK || cf == JXRHIP_COLOR_CMYK_DIRECT || cf == JXRHIP_COLOR_YUVK)
+                              ? 4
+                              : colors;
+  if (colors == 0 || colors > 4 || colors != expect)
+    return fail(JXRHIP_ERR_INVALID_PLAN, "primary component count does not match the colour format");
+  const bool integrated = d->alpha_mode == JXRHIP_ALPHA_INTEGRATED;
+  if (d->primary->plane_count != colors + (integrated ? 1 : 0))
+    return fail(JXRHIP_ERR_INVALID_PLAN, "primary coefficient arena component count does not match the image");
+  plan->arenas[0] = d->primary;
+  plan->arena_count = 1;
+  uint32_t ox = 0, oy = 0;
+  for (uint32_t i = 0; i < colors + (integrated ? 1 : 0); i++) {
+    const bool alpha = integrated && i == colors;
+    st = build_plane(plan.get(), d->primary, 0, i, !alpha && d->scaled && i != 0, alpha, &ox, &oy);
+    if (st) return st;
+  }
+  int32_t alpha_plane = integrated ? (int32_t)colors : -1;
+  if (d->alpha_mode == JXRHIP_ALPHA_SEPARATE) {
+    st = validate_arena(d->separate_alpha);
+    if (st) return st;
+    if (d->separate_alpha->plane_count != 1)
+      return fail(JXRHIP_ERR_INVALID_PLAN, "separate alpha arena is not one independent luma plane");
+    plan->arenas[1] = d->separate_alpha;
+    plan->arena_count = 2;
+    st = build_plane(plan.get(), d->separate_alpha, 1, 0, false, true, &ox, &oy);
+    if (st) return st;
+    alpha_plane = (int32_t)plan->planes.size() - 1;
+  } else if (d->alpha_mode > JXRHIP_ALPHA_SEPARATE) {
+    return fail(JXRHIP_ERR_INVALID_PLAN, "alpha mode");
+  }
+  for (uint32_t i = 0; i < colors; i++) {
+    const PlanPlane &p = plan->planes[i];
+    const bool chroma = i == 1 || i == 2;
+    const bool ok = (cf == JXRHIP_COLOR_YUV420 && chroma)   ? (p.bc == 2 && p.br == 2)
+                    : (cf == JXRHIP_COLOR_YUV422 && chroma) ? (p.bc == 2 && p.br == 4)
+                                                           : (p.bc == 4 && p.br == 4);
+    if (!ok) return fail(JXRHIP_ERR_INVALID_PLAN, "component macroblock geometry does not match the colour format");
+  }
+  if (plan->planes.size() > kMaxPlanes) return fail(JXRHIP_ERR_INVALID_PLAN, "too many planes");
+
+  // ---- output policy (device_plan/output.rs:102-295) ----
+  const jxrhip_output_format &f = d->output;
+  DevImage &dv = plan->dev;
+  std::memset(&dv, 0, sizeof dv);
+  dv.plane_count = (uint32_t)plan->planes.size();
+  dv.colors = colors;
+  dv.alpha_plane = alpha_plane;
+  dv.mbx = plan->mbx;
+  dv.mby = plan->mby;
+  dv.upsample = cf == JXRHIP_COLOR_YUV420 ? 1 : cf == JXRHIP_COLOR_YUV422 ? 2 : 0;
+  const uint32_t l = f.channel_layout;
+  uint32_t primary = 0;
+  switch (l) {
+    case JXRHIP_LAYOUT_LUMA:
+    case JXRHIP_LAYOUT_LUMA_ALPHA: primary = 1; break;
+    case JXRHIP_LAYOUT_RGB:
+    case JXRHIP_LAYOUT_BGR:
+    case JXRHIP_LAYOUT_RGBA:
+    case JXRHIP_LAYOUT_BGRA: primary = 3; break;
+    case JXRHIP_LAYOUT_RGBX:
+    case JXRHIP_LAYOUT_BGRX:
+    case JXRHIP_LAYOUT_CMYK:
+    case JXRHIP_LAYOUT_CMYKA: primary = 4; break;
+    case JXRHIP_LAYOUT_NCOMPONENT:
+    case JXRHIP_LAYOUT_NCOMPONENT_ALPHA: primary = colors; break;
+    default: return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "channel layout not supported");
+  }
+  dv.primary_count = primary;
+  dv.has_alpha = layout_has_alpha(l) ? 1 : 0;
+  dv.reverse = (l == JXRHIP_LAYOUT_BGR || l == JXRHIP_LAYOUT_BGRX || l == JXRHIP_LAYOUT_BGRA) ? 1 : 0;
+  dv.channels = primary + dv.has_alpha;
+  if (dv.channels != f.channel_count) return fail(JXRHIP_ERR_INVALID_PLAN, "channel count differs from the layout");
+  if (dv.has_alpha && alpha_plane < 0)
+    return fail(JXRHIP_ERR_INVALID_PLAN, "alpha output layout has no prepared alpha plane");
+  const uint32_t dep = f.bit_depth;
+  const uint32_t adep = f.has_alpha_format ? f.alpha_bit_depth : dep;
+  {
+    bool depth_ok = true;
+    switch (f.storage) {
+      case JXRHIP_STORE_U8: dv.bytes = 1; depth_ok = dep == JXRHIP_DEPTH_U8; break;
+      case JXRHIP_STORE_U16: dv.bytes = 2; depth_ok = dep == JXRHIP_DEPTH_U16 || dep == JXRHIP_DEPTH_U10; break;
+      case JXRHIP_STORE_I16: dv.bytes = 2; depth_ok = dep == JXRHIP_DEPTH_I16; break;
+      case JXRHIP_STORE_I32: dv.bytes = 4; depth_ok = dep == JXRHIP_DEPTH_I32; break;
+      case JXRHIP_STORE_F16: dv.bytes = 2; depth_ok = dep == JXRHIP_DEPTH_F16; break;
+      case JXRHIP_STORE_F32:
+        dv.bytes = 4;
+        depth_ok = dep == JXRHIP_DEPTH_F32 && (!dv.has_alpha || adep == JXRHIP_DEPTH_F32);
+        if (depth_ok && (f.mantissa_length > 23 || (dv.has_alpha && f.has_alpha_format && f.alpha_mantissa_length > 23)))
+          return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "BD32F mantissa length exceeds 23 bits");
+        break;
+      default: return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "storage kind not supported");
+    }
+    if (!depth_ok) return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "bit depth does not match the storage kind");
+  }
+  dv.storage = f.storage;
+  dv.u16_max = dep == JXRHIP_DEPTH_U10 ? 1023 : 65535;
+  {
+    const uint32_t oc = f.output_color;
+    const bool yuv = cf == JXRHIP_COLOR_YUV420 || cf == JXRHIP_COLOR_YUV422 || cf == JXRHIP_COLOR_YUV444;
+    if (oc == JXRHIP_COLOR_NCOMPONENT && cf == JXRHIP_COLOR_NCOMPONENT) dv.conv = kConvDirect;
+    else if (cf == JXRHIP_COLOR_LUMA && oc == JXRHIP_COLOR_LUMA) dv.conv = kConvLuma;
+    else if (cf == JXRHIP_COLOR_LUMA && oc == JXRHIP_COLOR_RGB) dv.conv = kConvLumaRgb;
+    else if (yuv && (oc == JXRHIP_COLOR_RGB || oc == JXRHIP_COLOR_RGBE)) dv.conv = kConvYuvRgb;
+    else if (cf == JXRHIP_COLOR_YUVK && oc == JXRHIP_COLOR_CMYK) dv.conv = kConvYuvkCmyk;
+    else if (cf == JXRHIP_COLOR_YUVK && oc == JXRHIP_COLOR_CMYK_DIRECT) dv.conv = kConvYuvkDirect;
+    else return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "internal-to-output colour conversion");
+    if (dv.upsample && (f.chroma_centering_x > 4 || (dv.upsample == 1 && f.chroma_centering_y > 4)))
+      return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "unknown chroma centering has no exact reconstruction filter");
+    for (uint32_t c = 0; c < 8; c++) {
+      dv.color_scale[c] = make_scale(c, oc, dep, f.shift_bits, f.scaled);
+      if ((l == JXRHIP_LAYOUT_RGBX || l == JXRHIP_LAYOUT_BGRX) && c == 3) dv.color_scale[c].flags = 2;
+    }
+    dv.alpha_scale = make_scale(0, JXRHIP_COLOR_LUMA, adep, f.has_alpha_format ? f.alpha_shift_bits : f.shift_bits,
+                                f.has_alpha_format ? f.alpha_scaled : f.scaled);
+  }
+  dv.premultiply = f.premultiply_alpha ? 1 : 0;
+  dv.centering_x = f.chroma_centering_x;
+  dv.centering_y = f.chroma_centering_y;
+  dv.f_scaled = f.scaled ? 1 : 0;
+  dv.f_alpha_scaled = (f.has_alpha_format ? f.alpha_scaled : f.scaled) ? 1 : 0;
+  dv.f_mant = f.mantissa_length;
+  dv.f_bias = f.exponent_bias;
+  dv.f_alpha_mant = f.has_alpha_format ? f.alpha_mantissa_length : f.mantissa_length;
+  dv.f_alpha_bias = f.has_alpha_format ? f.alpha_exponent_bias : f.exponent_bias;
+  {
+    // crop in the coded plane -> relative to the window's sample origin
+    const uint64_t wx = (uint64_t)ox * 16, wy = (uint64_t)oy * 16;
+    const jxrhip_surface_layout &s = d->surface;
+    if (f.crop_width == 0 || f.crop_height == 0 || s.width != f.crop_width || s.height != f.crop_height)
+      return fail(JXRHIP_ERR_INVALID_PLAN, "output layout and prepared request differ");
+    if (f.crop_x < wx || f.crop_y < wy || (uint64_t)f.crop_x + f.crop_width > wx + (uint64_t)plan->mbx * 16 ||
+        (uint64_t)f.crop_y + f.crop_height > wy + (uint64_t)plan->mby * 16)
+      return fail(JXRHIP_ERR_INVALID_PLAN, "output crop is outside a sample plane");
+    dv.crop_x = (uint32_t)(f.crop_x - wx);
+    dv.crop_y = (uint32_t)(f.crop_y - wy);
+    dv.crop_w = f.crop_width;
+    dv.crop_h = f.crop_height;
+    const uint32_t need = s.planar ? dv.channels : 1;
+    if (s.plane_count != need || need > kMaxPlanes) return fail(JXRHIP_ERR_INVALID_PLAN, "surface plane count");
+    dv.planar = s.planar ? 1 : 0;
+    for (uint32_t i = 0; i < need; i++) {
+      const jxrhip_surface_plane &sp = s.planes[i];
+      const uint32_t ch = s.planar ? 1 : dv.channels;
+      if (sp.width != s.width || sp.height != s.height || sp.channels != ch)
+        return fail(JXRHIP_ERR_INVALID_PLAN, "surface plane geometry");
+      const uint64_t row = (uint64_t)sp.width * ch * dv.bytes;
+      if (sp.row_stride_bytes < row) return fail(JXRHIP_ERR_INVALID_PLAN, "surface row stride");
+      if (sp.byte_offset + (uint64_t)(sp.height - 1) * sp.row_stride_bytes + row > s.byte_len)
+        return fail(JXRHIP_ERR_INVALID_PLAN, "surface plane exceeds its byte length");
+      if (sp.byte_offset % dv.bytes || sp.row_stride_bytes % dv.bytes)
+        return fail(JXRHIP_ERR_INVALID_PLAN, "surface plane is not aligned to its sample size");
+      dv.plane_offset[i] = sp.byte_offset;
+      dv.row_stride[i] = sp.row_stride_bytes;
+    }
+    plan->output_bytes = s.byte_len;
+  }
+  // ---- work and footprint figures ----
+  plan->pixels = (uint64_t)f.crop_width * f.crop_height;
+  for (const PlanPlane &p : plan->planes) {
+    plan->recon_coefs += (uint64_t)p.recs.size() * p.bc * p.br * 16;
+    plan->descriptor_bytes += p.recs.size() * sizeof(MbRec) + (p.colmask.size() + p.rowmask.size()) * 4;
+  }
+  for (uint32_t a = 0; a < plan->arena_count; a++) plan->coefficient_bytes += plan->arenas[a]->coefficient_count * 4;
+  {
+    // algorithmic bytes: 4 B per stored coefficient read, 32 B per
+    // macroblock record per plane, output bytes written
+    uint64_t stored = 0, mbs = 0;
+    for (const PlanPlane &p : plan->planes) {
+      mbs += p.recs.size();
+      for (const MbRec &r : p.recs)
+        stored += r.bands == 0 ? 1 : r.bands == 1 ? p.bc * p.br : (uint64_t)p.bc * p.br * 16;
+    }
+    const uint64_t out_px_bytes = plan->pixels * dv.channels * dv.bytes;
+    plan->alg_bytes = stored * 4 + mbs * sizeof(MbRec) + out_px_bytes;
+  }
+  plan->descriptor_bytes += sizeof(DevImage);
+  uint64_t key = plan->planes.size();
+  for (const PlanPlane &p : plan->planes) key = key * 16 + p.bc * 4 + p.br;
+  plan->geometry_key = key;
+  *out = plan.release();
+  return JXRHIP_OK;
+}
+
+extern "C" void jxrhip_plan_destroy(jxrhip_plan *plan) { delete plan; }
+extern "C" uint64_t jxrhip_plan_output_bytes(const jxrhip_plan *p) { return p ? p->output_bytes : 0; }
+extern "C" uint64_t jxrhip_plan_reconstructed_coefficients(const jxrhip_plan *p) { return p ? p->recon_coefs : 0; }
+extern "C" uint64_t jxrhip_plan_output_pixels(const jxrhip_plan *p) { return p ? p->pixels : 0; }
+extern "C" uint64_t jxrhip_plan_device_footprint(const jxrhip_plan *p) {
+  return p ? p->coefficient_bytes + p->descriptor_bytes + p->output_bytes : 0;
+}
+extern "C" uint64_t jxrhip_plan_algorithmic_bytes(const jxrhip_plan *p) { return p ? p->alg_bytes : 0; }
+
+// ---- device memory pool (exact sizes, buffer_pool.rs:9-129) -----------
+struct DeviceBuffer {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+class Pool {
+ public:
+  ~Pool() { trim(0); }
+  hipError_t get(size_t bytes, DeviceBuffer &out, uint64_t &hits, uint64_t &misses) {
+    bytes = (bytes + 255) & ~size_t(255);
+    if (bytes == 0) bytes = 256;
+    auto it = free_.find(bytes);
+    if (it != free_.end() && !it->second.empty()) {
+      out.ptr = it->second.back();
+      it->second.pop_back();
+      out.bytes = bytes;
+      retained_ -= bytes;
+      hits++;
+      return hipSuccess;
+    }
+    misses++;
+    hipError_t e = hipMalloc(&out.ptr, bytes);
+    if (e != hipSuccess) {
+      trim(0);
+      e = hipMalloc(&out.ptr, bytes);
+      if (e != hipSuccess) return e;
+    }
+    out.bytes = bytes;
+    live_ += bytes;
+    high_water_ = std::max(high_water_, live_ + retained_);
+    return hipSuccess;
+  }
+  void put(DeviceBuffer &b) {
+    if (!b.ptr) return;
+    live_ -= b.bytes;
+    if (retained_ + b.bytes > limit_) {
+      (void)hipFree(b.ptr);
+    } else {
+      free_[b.bytes].push_back(b.ptr);
+      retained_ += b.bytes;
+    }
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  void trim(size_t keep) {
+    for (auto &kv : free_) {
+      while (!kv.second.empty() && retained_ > keep) {
+        (void)hipFree(kv.second.back());
+        kv.second.pop_back();
+        retained_ -= kv.first;
+      }
+    }
+  }
+  uint64_t high_water() const { return high_water_; }
+
+ private:
+  std::map<size_t, std::vector<void *>> free_;
+  size_t retained_ = 0, live_ = 0, high_water_ = 0;
+  size_t limit_ = size_t(8) << 30;
+};
+
+// ---- sessions -----------------------------------------------------------
+static constexpr uint32_t kStreams = 4;  // runtime.rs:17
+
+struct CacheEntry {
+  DeviceBuffer buf;
+  uint64_t last_use = 0;
+  const void *deps[2] = {nullptr, nullptr};  // coefficient buffers a descriptor points at
+};
+
+struct jxrhip_session {
+  int device = 0;
+  uint32_t cus = 0;
+  size_t max_lds = 65536;
+  hipStream_t streams[kStreams] = {};
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  std::mutex lock;  // one submitter at a time (encode.rs:83-88)
+  Pool pool;
+  // coefficient uploads by arena identity; descriptor uploads by plan id
+  std::unordered_map<uint64_t, CacheEntry> coef_cache;
+  std::unordered_map<uint64_t, CacheEntry> desc_cache;
+  uint64_t cache_bytes = 0, cache_budget = uint64_t(64) << 30, tick = 0;
+  uint32_t next_stream = 0;
+  uint32_t tile_x = 0, tile_y = 0, groups_per_cu = 0;
+  DeviceBuffer counter;
+  double last_ms = 0;
+  uint32_t last_launches = 0;
+  jxrhip_diagnostics diag{};
+};
+
+struct jxrhip_image {
+  jxrhip_session *session = nullptr;
+  DeviceBuffer buf;
+  uint64_t bytes = 0;
+};
+
+struct jxrhip_submission {
+  jxrhip_session *session = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  DeviceBuffer status;    // device status word
+  uint32_t *host_status = nullptr;  // pinned
+  DeviceBuffer output;    // owned output (submit), empty for submit_into
+  uint64_t output_bytes = 0;
+  std::vector<DeviceBuffer> transient;
+  bool waited = false;
+};
+
+static void make_current(jxrhip_session *s) { (void)hipSetDevice(s->device); }
+
+extern "C" jxrhip_status jxrhip_session_create(int32_t ordinal, jxrhip_session **out) {
+  if (!out) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null output");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return fail(JXRHIP_ERR_UNAVAILABLE, "no HIP device is available");
+  }
+  if (ordinal < 0 || ordinal >= n) return fail(JXRHIP_ERR_UNAVAILABLE, "device ordinal out of range");
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, ordinal), "hipGetDeviceProperties");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(JXRHIP_ERR_UNAVAILABLE, std::string("device is ") + prop.gcnArchName + ", kernels are built for gfx950");
+  std::unique_ptr<jxrhip_session> s(new (std::nothrow) jxrhip_session());
+  if (!s) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "out of host memory");
+  s->device = ordinal;
+  s->cus = (uint32_t)prop.multiProcessorCount;
+  HIP_TRY(hipSetDevice(ordinal), "hipSetDevice");
+  for (uint32_t i = 0; i < kStreams; i++)
+    HIP_TRY(hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking), "hipStreamCreate");
+  HIP_TRY(hipEventCreate(&s->t0), "hipEventCreate");
+  HIP_TRY(hipEventCreate(&s->t1), "hipEventCreate");
+  size_t lds_limit = prop.sharedMemPerBlockOptin ? prop.sharedMemPerBlockOptin : prop.sharedMemPerBlock;
+  s->max_lds = std::min<size_t>(lds_limit, 160 * 1024);
+  if (s->max_lds > 65536 && set_kernel_lds_limit(s->max_lds) != hipSuccess) {
+    (void)hipGetLastError();
+    s->max_lds = 65536;
+  }
+  HIP_TRY(s->pool.get(256, s->counter, s->diag.scratch_hits, s->diag.scratch_misses), "counter");
+  s->diag.device_ordinal = ordinal;
+  s->diag.stream_count = kStreams;
+  s->diag.compute_units = s->cus;
+  *out = s.release();
+  return JXRHIP_OK;
+}
+
+extern "C" void jxrhip_session_destroy(jxrhip_session *s) {
+  if (!s) return;
+  make_current(s);
+  for (uint32_t i = 0; i < kStreams; i++)
+    if (s->streams[i]) (void)hipStreamSynchronize(s->streams[i]);
+  for (auto &kv : s->coef_cache) s->pool.put(kv.second.buf);
+  for (auto &kv : s->desc_cache) s->pool.put(kv.second.buf);
+  s->pool.put(s->counter);
+  for (uint32_t i = 0; i < kStreams; i++)
+    if (s->streams[i]) (void)hipStreamDestroy(s->streams[i]);
+  if (s->t0) (void)hipEventDestroy(s->t0);
+  if (s->t1) (void)hipEventDestroy(s->t1);
+  delete s;
+}
+
+extern "C" jxrhip_status jxrhip_session_diagnostics(jxrhip_session *s, jxrhip_diagnostics *out) {
+  if (!s || !out) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> g(s->lock);
+  s->diag.upload_resident_bytes = s->cache_bytes;
+  s->diag.scratch_high_water_bytes = s->pool.high_water();
+  *out = s->diag;
+  return JXRHIP_OK;
+}
+
+extern "C" jxrhip_status jxrhip_session_set_upload_budget(jxrhip_session *s, uint64_t bytes) {
+  if (!s) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null session");
+  std::lock_guard<std::mutex> g(s->lock);
+  s->cache_budget = bytes;
+  return JXRHIP_OK;
+}
+
+extern "C" jxrhip_status jxrhip_session_set_tile(jxrhip_session *s, uint32_t tx, uint32_t ty) {
+  if (!s) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null session");
+  if ((tx == 0) != (ty == 0) || tx > 16 || ty > 16) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "tile shape");
+  std::lock_guard<std::mutex> g(s->lock);
+  s->tile_x = tx;
+  s->tile_y = ty;
+  return JXRHIP_OK;
+}
+
+extern "C" jxrhip_status jxrhip_session_set_occupancy(jxrhip_session *s, uint32_t groups) {
+  if (!s) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null session");
+  std::lock_guard<std::mutex> g(s->lock);
+  s->groups_per_cu = groups;
+  return JXRHIP_OK;
+}
+
+static void evict_until(jxrhip_session *s, uint64_t need) {
+  while (s->cache_bytes + need > s->cache_budget) {
+    // least recently used of both caches
+    std::unordered_map<uint64_t, CacheEntry> *victim_map = nullptr;
+    uint64_t victim = 0, oldest = UINT64_MAX;
+    for (auto *m : {&s->coef_cache, &s->desc_cache})
+      for (auto &kv : *m)
+        if (kv.second.last_use < oldest) {
+          oldest = kv.second.last_use;
+          victim = kv.first;
+          victim_map = m;
+        }
+    if (!victim_map) return;
+    auto it = victim_map->find(victim);
+    // a buffer still read by queued work must not be recycled early
+    for (uint32_t i = 0; i < kStreams; i++) (void)hipStreamSynchronize(s->streams[i]);
+    s->cache_bytes -= it->second.buf.bytes;
+    s->pool.put(it->second.buf);
+    victim_map->erase(it);
+    s->diag.upload_evictions++;
+  }
+}
+
+// Device-resident coefficients of arena `a` (cached by identity when set).
+static jxrhip_status device_coefficients(jxrhip_session *s, const jxrhip_arena *a, hipStream_t stream,
+                                         const int32_t **out, jxrhip_submission *transient_owner) {
+  const size_t bytes = (size_t)a->coefficient_count * 4;
+  if (a->identity) {
+    auto it = s->coef_cache.find(a->identity);
+    if (it != s->coef_cache.end()) {
+      it->second.last_use = ++s->tick;
+      s->diag.upload_hits++;
+      *out = (const int32_t *)it->second.buf.ptr;
+      return JXRHIP_OK;
+    }
+  }
+  s->diag.upload_misses++;
+  DeviceBuffer b;
+  if (a->identity) evict_until(s, bytes);
+  HIP_TRY(s->pool.get(bytes + 64, b, s->diag.scratch_hits, s->diag.scratch_misses), "coefficient buffer");
+  HIP_TRY(hipMemcpyAsync(b.ptr, a->coefficients, bytes, hipMemcpyHostToDevice, stream), "coefficient upload");
+  if (a->identity) {
+    s->cache_bytes += b.bytes;
+    CacheEntry ce{b, ++s->tick, {nullptr, nullptr}};
+    s->coef_cache[a->identity] = ce;
+  } else if (transient_owner) {
+    transient_owner->transient.push_back(b);
+  } else {
+    HIP_TRY(hipStreamSynchronize(stream), "upload");
+    s->pool.put(b);
+    return fail(JXRHIP_ERR_INVALID_ARGUMENT, "an arena without identity cannot be uploaded ahead of a submission");
+  }
+  *out = (const int32_t *)b.ptr;
+  return JXRHIP_OK;
+}
+
+// Device DevImage for a plan (descriptors cached per plan id). The
+// coefficient pointers inside depend on the cached uploads, so the
+// descriptor is rebuilt whenever an upload changes.
+static jxrhip_status device_image(jxrhip_session *s, const jxrhip_plan *p, hipStream_t stream,
+                                  const DevImage **out, jxrhip_submission *owner) {
+  const int32_t *coef[2] = {nullptr, nullptr};
+  for (uint32_t a = 0; a < p->arena_count; a++) {
+    jxrhip_status st = device_coefficients(s, p->arenas[a], stream, &coef[a], owner);
+    if (st) return st;
+  }
+  const bool cacheable = p->arenas[0]->identity && (p->arena_count < 2 || p->arenas[1]->identity);
+  auto it = s->desc_cache.find(p->id);
+  if (cacheable && it != s->desc_cache.end()) {
+    // valid while the coefficient buffers it points at are the same
+    if (it->second.deps[0] == coef[0] && it->second.deps[1] == coef[1]) {
+      it->second.last_use = ++s->tick;
+      *out = (const DevImage *)it->second.buf.ptr;
+      return JXRHIP_OK;
+    }
+  }
+  // layout: DevImage | recs of each plane | masks
+  size_t bytes = sizeof(DevImage);
+  for (const PlanPlane &pp : p->planes)
+    bytes += pp.recs.size() * sizeof(MbRec) + (pp.colmask.size() + pp.rowmask.size()) * 4 + 64;
+  std::vector<uint8_t> host(bytes, 0);
+  DeviceBuffer b;
+  if (cacheable) evict_until(s, bytes);
+  HIP_TRY(s->pool.get(bytes, b, s->diag.scratch_hits, s->diag.scratch_misses), "descriptor buffer");
+  DevImage dv = p->dev;
+  size_t off = sizeof(DevImage);
+  for (size_t i = 0; i < p->planes.size(); i++) {
+    const PlanPlane &pp = p->planes[i];
+    DevPlane &d = dv.planes[i];
+    d.coef = coef[pp.arena];
+    off = (off + 63) & ~size_t(63);
+    std::memcpy(host.data() + off, pp.recs.data(), pp.recs.size() * sizeof(MbRec));
+    d.mbs = (const MbRec *)((uint8_t *)b.ptr + off);
+    off += pp.recs.size() * sizeof(MbRec);
+    if (!pp.colmask.empty()) {
+      std::memcpy(host.data() + off, pp.colmask.data(), pp.colmask.size() * 4);
+      d.colmask = (const uint32_t *)((uint8_t *)b.ptr + off);
+      off += pp.colmask.size() * 4;
+      std::memcpy(host.data() + off, pp.rowmask.data(), pp.rowmask.size() * 4);
+      d.rowmask = (const uint32_t *)((uint8_t *)b.ptr + off);
+      off += pp.rowmask.size() * 4;
+    } else {
+      d.colmask = d.rowmask = nullptr;
+    }
+    d.bc = pp.bc;
+    d.br = pp.br;
+    d.scale_first = pp.scale_first ? 1 : 0;
+    d.overlap = pp.overlap;
+    d.alpha = pp.alpha ? 1 : 0;
+  }
+  std::memcpy(host.data(), &dv, sizeof dv);
+  HIP_TRY(hipMemcpyAsync(b.ptr, host.data(), bytes, hipMemcpyHostToDevice, stream), "descriptor upload");
+  // pageable copies are staged before the call returns, so `host` may go
+  HIP_TRY(hipStreamSynchronize(stream), "descriptor upload");
+  if (cacheable) {
+    if (it != s->desc_cache.end()) {
+      s->cache_bytes -= it->second.buf.bytes;
+      s->pool.put(it->second.buf);
+    }
+    s->cache_bytes += b.bytes;
+    CacheEntry ce{b, ++s->tick, {coef[0], coef[1]}};
+    s->desc_cache[p->id] = ce;
+  } else {
+    owner->transient.push_back(b);
+  }
+  *out = (const DevImage *)b.ptr;
+  return JXRHIP_OK;
+}
+
+extern "C" jxrhip_status jxrhip_session_upload(jxrhip_session *s, const jxrhip_plan *p) {
+  if (!s || !p) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> g(s->lock);
+  make_current(s);
+  for (uint32_t a = 0; a < p->arena_count; a++)
+    if (!p->arenas[a]->identity)
+      return fail(JXRHIP_ERR_INVALID_ARGUMENT, "an arena without identity cannot be uploaded ahead of a submission");
+  const DevImage *dimg;
+  jxrhip_status st = device_image(s, p, s->streams[0], &dimg, nullptr);
+  if (st) return st;
+  HIP_TRY(hipStreamSynchronize(s->streams[0]), "upload");
+  return JXRHIP_OK;
+}
+
+extern "C" jxrhip_status jxrhip_session_clear_cache(jxrhip_session *s) {
+  if (!s) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null session");
+  std::lock_guard<std::mutex> g(s->lock);
+  make_current(s);
+  for (uint32_t i = 0; i < kStreams; i++) (void)hipStreamSynchronize(s->streams[i]);
+  for (auto &kv : s->coef_cache) s->pool.put(kv.second.buf);
+  for (auto &kv : s->desc_cache) s->pool.put(kv.second.buf);
+  s->coef_cache.clear();
+  s->desc_cache.clear();
+  s->cache_bytes = 0;
+  s->pool.trim(0);
+  return JXRHIP_OK;
+}
+
+// Tile shape of a plan set: the largest candidate whose LDS fits the budget.
+static void choose_tile(const jxrhip_session *s, const jxrhip_plan *p, uint32_t &tx, uint32_t &ty,
+                        uint32_t &words) {
+  auto need = [&](uint32_t x, uint32_t y) {
+    uint32_t w = 0, lowmax = 0;
+    for (const PlanPlane &pp : p->planes) {
+      w += sample_region_words(pp.bc, pp.br, x, y);
+      lowmax = std::max(lowmax, low_region_words(pp.bc, pp.br, x, y));
+    }
+    return w + lowmax;
+  };
+  const uint32_t budget = 48 * 1024 / 4;
+  if (s->tile_x) {
+    tx = s->tile_x;
+    ty = s->tile_y;
+    while (need(tx, ty) > budget && (tx > 1 || ty > 1)) {
+      if (tx >= ty) tx = (tx + 1) / 2;
+      else ty = (ty + 1) / 2;
+    }
+    words = need(tx, ty);
+    return;
+  }
+  static const uint32_t cand[][2] = {{4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
+  for (auto &c : cand) {
+    if (need(c[0], c[1]) <= budget) {
+      tx = c[0];
+      ty = c[1];
+      words = need(tx, ty);
+      return;
+    }
+  }
+  tx = ty = 1;
+  words = need(1, 1);
+}
+
+static jxrhip_status submit_impl(jxrhip_session *s, const jxrhip_plan *p, void *dst, uint64_t dst_bytes,
+                                 jxrhip_submission **out) {
+  if (!s || !p || !out) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (dst && dst_bytes < p->output_bytes) return fail(JXRHIP_ERR_BUFFER_TOO_SMALL, "destination smaller than the plan's output");
+  std::lock_guard<std::mutex> g(s->lock);
+  make_current(s);
+  std::unique_ptr<jxrhip_submission> sub(new (std::nothrow) jxrhip_submission());
+  if (!sub) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "out of host memory");
+  sub->session = s;
+  sub->stream = s->streams[s->next_stream++ % kStreams];
+  HIP_TRY(hipEventCreateWithFlags(&sub->done, hipEventDisableTiming), "hipEventCreate");
+  HIP_TRY(hipHostMalloc((void **)&sub->host_status, sizeof(uint32_t), hipHostMallocDefault), "hipHostMalloc");
+  *sub->host_status = 0;
+  HIP_TRY(s->pool.get(256, sub->status, s->diag.scratch_hits, s->diag.scratch_misses), "status word");
+  if (!dst) {
+    HIP_TRY(s->pool.get(p->output_bytes, sub->output, s->diag.scratch_hits, s->diag.scratch_misses), "output");
+    dst = sub->output.ptr;
+    sub->output_bytes = p->output_bytes;
+  }
+  const DevImage *dimg;
+  jxrhip_status st = device_image(s, p, sub->stream, &dimg, sub.get());
+  if (st) return st;
+  uint32_t tx, ty, words;
+  choose_tile(s, p, tx, ty, words);
+  if ((size_t)words * 4 > s->max_lds) return fail(JXRHIP_ERR_INVALID_PLAN, "tile does not fit in LDS");
+  KernelArgs args{};
+  args.inline_slot.img = dimg;
+  args.inline_slot.out = (uint8_t *)dst;
+  args.inline_slot.status = (uint32_t *)sub->status.ptr;
+  args.inline_slot.tiles_x = (p->mbx + tx - 1) / tx;
+  args.inline_slot.tile_start = 0;
+  args.persistent = 0;
+  args.tile_x = tx;
+  args.tile_y = ty;
+  args.lds_words = words;
+  const uint32_t tiles = args.inline_slot.tiles_x * ((p->mby + ty - 1) / ty);
+  HIP_TRY(hipMemsetAsync(sub->status.ptr, 0, 4, sub->stream), "status reset");
+  HIP_TRY(launch_tiles(args, tiles, sub->stream), "reconstruction launch");
+  HIP_TRY(hipMemcpyAsync(sub->host_status, sub->status.ptr, 4, hipMemcpyDeviceToHost, sub->stream), "status readback");
+  HIP_TRY(hipEventRecord(sub->done, sub->stream), "hipEventRecord");
+  s->diag.submissions++;
+  s->diag.kernel_launches++;
+  *out = sub.release();
+  return JXRHIP_OK;
+}
+
+extern "C" jxrhip_status jxrhip_submit(jxrhip_session *s, const jxrhip_plan *p, jxrhip_submission **out) {
+  return submit_impl(s, p, nullptr, 0, out);
+}
+extern "C" jxrhip_status jxrhip_submit_into(jxrhip_session *s, const jxrhip_plan *p, void *dst, uint64_t bytes,
+                                            jxrhip_submission **out) {
+  if (!dst) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null destination");
+  return submit_impl(s, p, dst, bytes, out);
+}
+
+extern "C" int32_t jxrhip_submission_is_complete(const jxrhip_submission *sub) {
+  if (!sub) return 1;
+  return hipEventQuery(sub->done) == hipSuccess ? 1 : 0;
+}
+
+static void release_submission(jxrhip_submission *sub) {
+  jxrhip_session *s = sub->session;
+  std::lock_guard<std::mutex> g(s->lock);
+  make_current(s);
+  s->pool.put(sub->status);
+  for (auto &b : sub->transient) s->pool.put(b);
+  sub->transient.clear();
+  if (sub->output.ptr) s->pool.put(sub->output);
+  if (sub->host_status) (void)hipHostFree(sub->host_status);
+  if (sub->done) (void)hipEventDestroy(sub->done);
+  delete sub;
+}
+
+extern "C" jxrhip_status jxrhip_submission_wait(jxrhip_submission *sub, jxrhip_image **image, uint32_t *stage) {
+  if (!sub) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null submission");
+  if (image) *image = nullptr;
+  if (stage) *stage = 0;
+  make_current(sub->session);
+  hipError_t e = hipEventSynchronize(sub->done);
+  if (e != hipSuccess) {
+    release_submission(sub);
+    return hip_fail(e, "hipEventSynchronize");
+  }
+  const uint32_t status = *sub->host_status;
+  if (status != 0) {
+    if (stage) *stage = status;
+    release_submission(sub);
+    return fail(JXRHIP_ERR_KERNEL_ARITHMETIC, "reconstruction arithmetic overflow (stage " + std::to_string(status) + ")");
+  }
+  if (image && sub->output.ptr) {
+    jxrhip_image *im = new (std::nothrow) jxrhip_image();
+    if (!im) {
+      release_submission(sub);
+      return fail(JXRHIP_ERR_INVALID_ARGUMENT, "out of host memory");
+    }
+    im->session = sub->session;
+    im->buf = sub->output;
+    im->bytes = sub->output_bytes;
+    sub->output = DeviceBuffer{};
+    *image = im;
+  }
+  release_submission(sub);
+  return JXRHIP_OK;
+}
+
+extern "C" void jxrhip_submission_destroy(jxrhip_submission *sub) {
+  if (!sub) return;
+  make_current(sub->session);
+  (void)hipEventSynchronize(sub->done);
+  release_submission(sub);
+}
+
+extern "C" jxrhip_status jxrhip_allocate_destination(jxrhip_session *s, uint64_t bytes, void **ptr) {
+  if (!s || !ptr) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  *ptr = nullptr;
+  make_current(s);
+  HIP_TRY(hipMalloc(ptr, bytes ? bytes : 1), "hipMalloc");
+  return JXRHIP_OK;
+}
+extern "C" jxrhip_status jxrhip_free_destination(jxrhip_session *s, void *ptr) {
+  if (!s) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null session");
+  if (!ptr) return JXRHIP_OK;
+  make_current(s);
+  for (uint32_t i = 0; i < kStreams; i++) HIP_TRY(hipStreamSynchronize(s->streams[i]), "hipStreamSynchronize");
+  HIP_TRY(hipFree(ptr), "hipFree");
+  return JXRHIP_OK;
+}
+extern "C" jxrhip_status jxrhip_fill_destination(jxrhip_session *s, void *ptr, uint8_t value, uint64_t bytes) {
+  if (!s || !ptr) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  make_current(s);
+  HIP_TRY(hipMemsetAsync(ptr, value, bytes, s->streams[0]), "hipMemsetAsync");
+  HIP_TRY(hipStreamSynchronize(s->streams[0]), "hipStreamSynchronize");
+  return JXRHIP_OK;
+}
+extern "C" jxrhip_status jxrhip_copy_to_host(jxrhip_session *s, const void *ptr, void *host, uint64_t bytes) {
+  if (!s || !ptr || !host) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  make_current(s);
+  for (uint32_t i = 0; i < kStreams; i++) HIP_TRY(hipStreamSynchronize(s->streams[i]), "hipStreamSynchronize");
+  HIP_TRY(hipMemcpy(host, ptr, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+  return JXRHIP_OK;
+}
+extern "C" jxrhip_status jxrhip_session_synchronize(jxrhip_session *s) {
+  if (!s) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null session");
+  make_current(s);
+  for (uint32_t i = 0; i < kStreams; i++) HIP_TRY(hipStreamSynchronize(s->streams[i]), "hipStreamSynchronize");
+  return JXRHIP_OK;
+}
+
+extern "C" void *jxrhip_image_device_ptr(const jxrhip_image *im) { return im ? im->buf.ptr : nullptr; }
+extern "C" uint64_t jxrhip_image_bytes(const jxrhip_image *im) { return im ? im->bytes : 0; }
+extern "C" void jxrhip_image_destroy(jxrhip_image *im) {
+  if (!im) return;
+  jxrhip_session *s = im->session;
+  {
+    std::lock_guard<std::mutex> g(s->lock);
+    s->pool.put(im->buf);
+  }
+  delete im;
+}
+
+extern "C" jxrhip_status jxrhip_readback(jxrhip_session *s, const jxrhip_image *im, void *host, uint64_t bytes) {
+  if (!s || !im || !host) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  if (bytes < im->bytes) return fail(JXRHIP_ERR_BUFFER_TOO_SMALL, "host buffer smaller than the image");
+  make_current(s);
+  HIP_TRY(hipMemcpy(host, im->buf.ptr, im->bytes, hipMemcpyDeviceToHost), "readback");
+  return JXRHIP_OK;
+}
+
+extern "C" jxrhip_status jxrhip_decode_to_host(jxrhip_session *s, const jxrhip_plan *p, void *host, uint64_t bytes,
+                                               uint32_t *stage) {
+  if (stage) *stage = 0;
+  if (!p || bytes < p->output_bytes) return fail(JXRHIP_ERR_BUFFER_TOO_SMALL, "host buffer smaller than the output");
+  jxrhip_submission *sub;
+  jxrhip_status st = jxrhip_submit(s, p, &sub);
+  if (st) return st;
+  jxrhip_image *im = nullptr;
+  st = jxrhip_submission_wait(sub, &im, stage);
+  if (st) return st;
+  st = jxrhip_readback(s, im, host, bytes);
+  jxrhip_image_destroy(im);
+  return st;
+}
+
+// ---- batches ------------------------------------------------------------
+struct jxrhip_batch {
+  jxrhip_session *session = nullptr;
+  uint32_t count = 0;
+  std::vector<jxrhip_submission *> subs;  // per-image mode
+  DeviceBuffer statuses;                  // persistent mode: one word per image
+  uint32_t *host_statuses = nullptr;
+  std::vector<DeviceBuffer> transient;
+  hipEvent_t done = nullptr;
+  bool timed = false;
+};
+
+extern "C" jxrhip_status jxrhip_submit_batch(jxrhip_session *s, const jxrhip_plan *const *plans, uint32_t count,
+                                             void *const *dsts, uint32_t mode, jxrhip_batch **out) {
+  if (!s || !plans || !dsts || !out) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  for (uint32_t i = 0; i < count; i++)
+    if (!plans[i] || !dsts[i]) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null plan or destination");
+  std::unique_ptr<jxrhip_batch> b(new (std::nothrow) jxrhip_batch());
+  if (!b) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "out of host memory");
+  b->session = s;
+  b->count = count;
+  make_current(s);
+  if (mode == JXRHIP_BATCH_PER_IMAGE) {
+    // Upload everything first, so that the timed window holds launches only.
+    {
+      std::lock_guard<std::mutex> g(s->lock);
+      HIP_TRY(hipEventRecord(s->t0, s->streams[0]), "hipEventRecord");
+      for (uint32_t i = 1; i < kStreams; i++) HIP_TRY(hipStreamWaitEvent(s->streams[i], s->t0, 0), "hipStreamWaitEvent");
+      s->next_stream = 0;
+    }
+    for (uint32_t i = 0; i < count; i++) {
+      jxrhip_submission *sub = nullptr;
+      jxrhip_status st = submit_impl(s, plans[i], dsts[i], plans[i]->output_bytes, &sub);
+      if (st) {
+        for (auto *x : b->subs) jxrhip_submission_destroy(x);
+        return st;
+      }
+      b->subs.push_back(sub);
+    }
+    std::lock_guard<std::mutex> g(s->lock);
+    for (uint32_t i = 1; i < kStreams; i++) {
+      hipEvent_t e;
+      HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+      HIP_TRY(hipEventRecord(e, s->streams[i]), "hipEventRecord");
+      HIP_TRY(hipStreamWaitEvent(s->streams[0], e, 0), "hipStreamWaitEvent");
+      (void)hipEventDestroy(e);
+    }
+    HIP_TRY(hipEventRecord(s->t1, s->streams[0]), "hipEventRecord");
+    b->timed = true;
+    s->last_launches = count;
+    *out = b.release();
+    return JXRHIP_OK;
+  }
+  if (mode != JXRHIP_BATCH_PERSISTENT) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "batch mode");
+  std::lock_guard<std::mutex> g(s->lock);
+  hipStream_t stream = s->streams[0];
+  HIP_TRY(hipHostMalloc((void **)&b->host_statuses, sizeof(uint32_t) * std::max<uint32_t>(count, 1), hipHostMallocDefault),
+          "hipHostMalloc");
+  HIP_TRY(s->pool.get(sizeof(uint32_t) * std::max<uint32_t>(count, 1), b->statuses, s->diag.scratch_hits,
+                      s->diag.scratch_misses),
+          "status words");
+  // shape buckets: plans with the same plane geometry share a tile shape
+  std::map<uint64_t, std::vector<uint32_t>> buckets;
+  for (uint32_t i = 0; i < count; i++) buckets[plans[i]->geometry_key].push_back(i);
+  std::vector<const DevImage *> dimgs(count);
+  for (uint32_t i = 0; i < count; i++) {
+    jxrhip_submission tmp;
+    tmp.session = s;
+    jxrhip_status st = device_image(s, plans[i], stream, &dimgs[i], &tmp);
+    for (auto &t : tmp.transient) b->transient.push_back(t);
+    if (st) return st;
+  }
+  HIP_TRY(hipMemsetAsync(b->statuses.ptr, 0, sizeof(uint32_t) * std::max<uint32_t>(count, 1), stream), "status reset");
+  // per-bucket slot arrays go into one device buffer
+  std::vector<Slot> slots;
+  struct Launch {
+    size_t first;
+    uint32_t n, items, tx, ty, words;
+  };
+  std::vector<Launch> launches;
+  for (auto &kv : buckets) {
+    uint32_t tx, ty, words;
+    choose_tile(s, plans[kv.second[0]], tx, ty, words);
+    for (uint32_t i : kv.second) {
+      uint32_t w2;
+      uint32_t ttx, tty;
+      choose_tile(s, plans[i], ttx, tty, w2);
+      words = std::max(words, w2);
+    }
+    if ((size_t)words * 4 > s->max_lds) return fail(JXRHIP_ERR_INVALID_PLAN, "tile does not fit in LDS");
+    Launch L{slots.size(), (uint32_t)kv.second.size(), 0, tx, ty, words};
+    for (uint32_t i : kv.second) {
+      Slot sl{};
+      sl.img = dimgs[i];
+      sl.out = (uint8_t *)dsts[i];
+      sl.status = (uint32_t *)b->statuses.ptr + i;
+      sl.tiles_x = (plans[i]->mbx + tx - 1) / tx;
+      sl.tile_start = L.items;
+      L.items += sl.tiles_x * ((plans[i]->mby + ty - 1) / ty);
+      slots.push_back(sl);
+    }
+    launches.push_back(L);
+  }
+  DeviceBuffer slot_buf;
+  HIP_TRY(s->pool.get(std::max<size_t>(slots.size(), 1) * sizeof(Slot), slot_buf, s->diag.scratch_hits,
+                      s->diag.scratch_misses),
+          "slot buffer");
+  b->transient.push_back(slot_buf);
+  HIP_TRY(hipMemcpyAsync(slot_buf.ptr, slots.data(), slots.size() * sizeof(Slot), hipMemcpyHostToDevice, stream),
+          "slot upload");
+  HIP_TRY(hipStreamSynchronize(stream), "slot upload");
+  // counters: one per bucket
+  DeviceBuffer counters;
+  HIP_TRY(s->pool.get(std::max<size_t>(launches.size(), 1) * 64, counters, s->diag.scratch_hits, s->diag.scratch_misses),
+          "counters");
+  b->transient.push_back(counters);
+  HIP_TRY(hipEventRecord(s->t0, stream), "hipEventRecord");
+  for (size_t li = 0; li < launches.size(); li++) {
+    const Launch &L = launches[li];
+    KernelArgs args{};
+    args.slots = (const Slot *)slot_buf.ptr + L.first;
+    args.slot_count = L.n;
+    args.item_count = L.items;
+    args.counter = (uint32_t *)((uint8_t *)counters.ptr + li * 64);
+    args.persistent = 1;
+    args.tile_x = L.tx;
+    args.tile_y = L.ty;
+    args.lds_words = L.words;
+    // workgroups a compute unit holds: 160 KiB of LDS, at most 8 groups
+    const uint32_t per_cu = s->groups_per_cu ? s->groups_per_cu
+                                              : std::max<uint32_t>(1, std::min<uint32_t>(8, (uint32_t)((160u * 1024u) / std::max<uint32_t>(L.words * 4, 1))));
+    uint32_t grid = std::min<uint32_t>(L.items, s->cus * per_cu);
+    grid = std::max<uint32_t>(grid, 1);
+    HIP_TRY(launch_reset(args.counter, stream), "counter reset");
+    HIP_TRY(launch_tiles(args, grid, stream), "persistent launch");
+    s->diag.kernel_launches++;
+  }
+  HIP_TRY(hipEventRecord(s->t1, stream), "hipEventRecord");
+  HIP_TRY(hipMemcpyAsync(b->host_statuses, b->statuses.ptr, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, stream),
+          "status readback");
+  HIP_TRY(hipEventCreateWithFlags(&b->done, hipEventDisableTiming), "hipEventCreate");
+  HIP_TRY(hipEventRecord(b->done, stream), "hipEventRecord");
+  b->timed = true;
+  s->last_launches = (uint32_t)launches.size();
+  s->diag.submissions += count;
+  *out = b.release();
+  return JXRHIP_OK;
+}
+
+extern "C" int32_t jxrhip_batch_is_complete(const jxrhip_batch *b) {
+  if (!b) return 1;
+  if (b->done) return hipEventQuery(b->done) == hipSuccess ? 1 : 0;
+  for (auto *sub : b->subs)
+    if (!jxrhip_submission_is_complete(sub)) return 0;
+  return 1;
+}
+
+extern "C" jxrhip_status jxrhip_batch_wait(jxrhip_batch *b, uint32_t *stages) {
+  if (!b) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null batch");
+  jxrhip_session *s = b->session;
+  make_current(s);
+  jxrhip_status result = JXRHIP_OK;
+  std::string first_error;
+  if (b->done) {
+    hipError_t e = hipEventSynchronize(b->done);
+    if (e != hipSuccess) result = hip_fail(e, "hipEventSynchronize");
+    for (uint32_t i = 0; i < b->count; i++) {
+      const uint32_t st = result == JXRHIP_OK ? b->host_statuses[i] : 0;
+      if (stages) stages[i] = st;
+      if (st && result == JXRHIP_OK) {
+        result = JXRHIP_ERR_KERNEL_ARITHMETIC;
+        first_error = "image " + std::to_string(i) + ": reconstruction arithmetic overflow (stage " + std::to_string(st) + ")";
+      }
+    }
+  } else {
+    for (uint32_t i = 0; i < b->subs.size(); i++) {
+      uint32_t st = 0;
+      jxrhip_status r = jxrhip_submission_wait(b->subs[i], nullptr, &st);
+      if (stages) stages[i] = st;
+      if (r && result == JXRHIP_OK) {
+        result = r;
+        first_error = "image " + std::to_string(i) + ": " + g_error;
+      }
+    }
+    b->subs.clear();
+  }
+  if (b->timed) {
+    float ms = 0;
+    (void)hipEventSynchronize(s->t1);
+    if (hipEventElapsedTime(&ms, s->t0, s->t1) == hipSuccess) s->last_ms = ms;
+  }
+  {
+    std::lock_guard<std::mutex> g(s->lock);
+    for (auto &t : b->transient) s->pool.put(t);
+    s->pool.put(b->statuses);
+  }
+  if (b->host_statuses) (void)hipHostFree(b->host_statuses);
+  if (b->done) (void)hipEventDestroy(b->done);
+  delete b;
+  if (result) g_error = first_error.empty() ? g_error : first_error;
+  return result;
+}
+
+extern "C" jxrhip_status jxrhip_last_batch_timing(jxrhip_session *s, double *ms, uint32_t *launches) {
+  if (!s) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null session");
+  if (ms) *ms = s->last_ms;
+  if (launches) *launches = s->last_launches;
+  return JXRHIP_OK;
+}

@@ -1,0 +1,480 @@
+"""Host-side API over the C ABI, named after the reference's CUDA backend.
+
+  HipDecodePlan      CudaDecodePlan      (crates/jxr-cuda/src/plan.rs:17-148)
+  HipDecoderSession  CudaDecoderSession  (crates/jxr-cuda/src/session.rs:16-268)
+  HipSubmission      CudaSubmission      (crates/jxr-cuda/src/submission.rs:11-118)
+  ResidentHipImage   ResidentCudaImage   (crates/jxr-cuda/src/resident.rs:8-58)
+  plan_hip_route     plan_cuda_route     (crates/jxr-cuda/src/route.rs:23-50)
+
+Errors raise JxrHipError with the C status code; validation errors happen
+before any device resource exists, and an arithmetic failure after
+submission carries the first failing stage (no CPU retry).
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import abi
+from .abi import JxrHipError, check, load
+
+
+# ---------------------------------------------------------------------------
+# encoded images (forward analysis into arenas)
+# ---------------------------------------------------------------------------
+def quality_qp(quality: int) -> int:
+    """Coded QP of a 0..100 quality: 100 is lossless (step 1)."""
+    if quality >= 100:
+        return 0
+    return int(round(4 + (100 - quality) * 0.8))
+
+
+class EncodedImage:
+    """Arenas produced by jxrhip_encode for one seeded image."""
+
+    def __init__(self, pixels: np.ndarray, *, color: int, overlap: int, tiles=None,
+                 hard_tiles: bool = False, scaled: bool = False, qp=(0, 0, 0),
+                 bands: int = abi.BANDS_ALL, alpha_mode: int = abi.ALPHA_NONE,
+                 shift_bits: int = 0, mantissa_length: int = 13, exponent_bias: int = 4,
+                 alpha_qp: int = 0, seed: int = 0, vary_quantizers: bool = False):
+        lib = load()
+        pixels = np.ascontiguousarray(pixels)
+        if pixels.ndim == 2:
+            pixels = pixels[:, :, None]
+        h, w, ch = pixels.shape
+        kind = {np.dtype(np.uint8): abi.INPUT_U8, np.dtype(np.uint16): abi.INPUT_U16,
+                np.dtype(np.float32): abi.INPUT_F32}[pixels.dtype]
+        p = abi.EncodeParams()
+        p.width, p.height, p.input_kind, p.input_channels = w, h, kind, ch
+        p.color_format, p.alpha_mode, p.overlap = color, alpha_mode, overlap
+        p.hard_tiles = 1 if hard_tiles else 0
+        self._tiles = None
+        if tiles is not None:
+            cols, rows = tiles
+            self._tiles = (np.asarray(cols, np.uint32), np.asarray(rows, np.uint32))
+            p.tile_column_count = len(cols)
+            p.tile_column_widths = self._tiles[0].ctypes.data_as(abi.u32p)
+            p.tile_row_count = len(rows)
+            p.tile_row_heights = self._tiles[1].ctypes.data_as(abi.u32p)
+        p.scaled = 1 if scaled else 0
+        p.qp_dc, p.qp_lp, p.qp_hp = qp
+        p.bands = bands
+        p.shift_bits, p.mantissa_length, p.exponent_bias = shift_bits, mantissa_length, exponent_bias
+        p.alpha_qp, p.seed, p.vary_quantizers = alpha_qp, seed, 1 if vary_quantizers else 0
+        handle = C.c_void_p()
+        check(lib.jxrhip_encode(pixels.ctypes.data, pixels.nbytes, C.byref(p), C.byref(handle)), "encode")
+        self._handle = handle
+        self.width, self.height, self.channels = w, h, ch
+        self.color_format = color
+        self.component_count = 1 if color == abi.COLOR_LUMA else 3
+        self.alpha_mode = alpha_mode if ch in (2, 4) else abi.ALPHA_NONE
+        self.scaled = bool(scaled)
+        self.input_dtype = pixels.dtype
+        self.shift_bits, self.mantissa_length, self.exponent_bias = shift_bits, mantissa_length, exponent_bias
+        self.overlap = overlap
+
+    @property
+    def primary(self):
+        return load().jxrhip_encoded_arena(self._handle, 0)
+
+    @property
+    def alpha(self):
+        ptr = load().jxrhip_encoded_arena(self._handle, 1)
+        return ptr if ptr else None
+
+    def coefficient_bytes(self) -> int:
+        n = self.primary.contents.coefficient_count * 4
+        if self.alpha is not None:
+            n += self.alpha.contents.coefficient_count * 4
+        return n
+
+    def close(self):
+        if getattr(self, "_handle", None):
+            load().jxrhip_encoded_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# output policy + surface layout
+# ---------------------------------------------------------------------------
+_STORE_BYTES = {abi.STORE_U8: 1, abi.STORE_U16: 2, abi.STORE_I16: 2, abi.STORE_I32: 4,
+                abi.STORE_F16: 2, abi.STORE_F32: 4}
+_LAYOUT_CHANNELS = {abi.LAYOUT_LUMA: 1, abi.LAYOUT_LUMA_ALPHA: 2, abi.LAYOUT_RGB: 3,
+                    abi.LAYOUT_BGR: 3, abi.LAYOUT_RGBA: 4, abi.LAYOUT_BGRA: 4,
+                    abi.LAYOUT_RGBX: 4, abi.LAYOUT_BGRX: 4}
+
+
+@dataclass
+class OutputSpec:
+    """OutputFormatRequest + SurfaceLayout choice for one image."""
+    storage: int = abi.STORE_U8
+    depth: int = abi.DEPTH_U8
+    layout: int = abi.LAYOUT_RGB
+    output_color: int = abi.COLOR_RGB
+    planar: bool = False
+    shift_bits: int = 0
+    mantissa_length: int = 13
+    exponent_bias: int = 4
+    premultiply: bool = False
+    red_blue_not_swapped: bool = False
+    centering: tuple = (0, 0)
+    crop: Optional[tuple] = None  # (x, y, w, h) in the coded plane
+    row_padding: int = 0          # extra bytes per row (strided surfaces)
+
+    @property
+    def bytes_per_sample(self):
+        return _STORE_BYTES[self.storage]
+
+    @property
+    def channels(self):
+        return _LAYOUT_CHANNELS[self.layout]
+
+
+def surface_layout(width, height, spec: OutputSpec, base: int = 0) -> abi.SurfaceLayout:
+    s = abi.SurfaceLayout()
+    s.width, s.height = width, height
+    b = spec.bytes_per_sample
+    ch = spec.channels
+    if spec.planar:
+        s.planar = 1
+        s.plane_count = ch
+        off = base
+        for c in range(ch):
+            pl = s.planes[c]
+            pl.byte_offset = off
+            pl.row_stride_bytes = width * b + spec.row_padding
+            pl.width, pl.height, pl.channels = width, height, 1
+            off += pl.row_stride_bytes * height
+        s.byte_len = off
+    else:
+        s.planar = 0
+        s.plane_count = 1
+        pl = s.planes[0]
+        pl.byte_offset = base
+        pl.row_stride_bytes = width * ch * b + spec.row_padding
+        pl.width, pl.height, pl.channels = width, height, ch
+        s.byte_len = base + pl.row_stride_bytes * height
+    return s
+
+
+def image_desc(enc: EncodedImage, spec: OutputSpec) -> abi.ImageDesc:
+    d = abi.ImageDesc()
+    d.abi_version = abi.ABI_VERSION
+    d.color_format = enc.color_format
+    d.component_count = enc.component_count
+    d.alpha_mode = enc.alpha_mode
+    d.scaled = 1 if enc.scaled else 0
+    d.primary = enc.primary
+    if enc.alpha_mode == abi.ALPHA_SEPARATE:
+        d.separate_alpha = enc.alpha
+    o = d.output
+    yuv = enc.color_format in (abi.COLOR_YUV420, abi.COLOR_YUV422, abi.COLOR_YUV444)
+    o.internal_color = abi.COLOR_YUV444 if yuv else enc.color_format
+    o.output_color = spec.output_color
+    o.bit_depth = spec.depth
+    o.shift_bits = spec.shift_bits
+    o.mantissa_length = spec.mantissa_length
+    o.exponent_bias = spec.exponent_bias
+    o.has_alpha_format = 1 if enc.alpha_mode != abi.ALPHA_NONE else 0
+    o.alpha_bit_depth = spec.depth
+    o.alpha_shift_bits = spec.shift_bits
+    o.alpha_mantissa_length = spec.mantissa_length
+    o.alpha_exponent_bias = spec.exponent_bias
+    o.alpha_scaled = 1 if enc.scaled else 0
+    o.storage = spec.storage
+    o.channel_layout = spec.layout
+    o.channel_count = spec.channels
+    o.scaled = 1 if enc.scaled else 0
+    o.red_blue_not_swapped = 1 if spec.red_blue_not_swapped else 0
+    o.premultiply_alpha = 1 if spec.premultiply else 0
+    o.chroma_centering_x, o.chroma_centering_y = spec.centering
+    cx, cy, cw, ch = spec.crop if spec.crop else (0, 0, enc.width, enc.height)
+    o.crop_x, o.crop_y, o.crop_width, o.crop_height = cx, cy, cw, ch
+    d.surface = surface_layout(cw, ch, spec)
+    return d
+
+
+# ---------------------------------------------------------------------------
+# plans, sessions, submissions
+# ---------------------------------------------------------------------------
+class HipDecodePlan:
+    """Validated reconstruction plan (CudaDecodePlan::from_prepared)."""
+
+    def __init__(self, desc: abi.ImageDesc, keepalive=None):
+        lib = load()
+        self._desc = desc
+        self._keep = keepalive
+        h = C.c_void_p()
+        check(lib.jxrhip_plan_create(C.byref(desc), C.byref(h)), "plan")
+        self._h = h
+
+    @classmethod
+    def from_encoded(cls, enc: EncodedImage, spec: OutputSpec):
+        return cls(image_desc(enc, spec), keepalive=enc)
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def desc(self):
+        return self._desc
+
+    @property
+    def output_bytes(self) -> int:
+        return load().jxrhip_plan_output_bytes(self._h)
+
+    @property
+    def output_pixels(self) -> int:
+        return load().jxrhip_plan_output_pixels(self._h)
+
+    @property
+    def reconstructed_coefficients(self) -> int:
+        return load().jxrhip_plan_reconstructed_coefficients(self._h)
+
+    @property
+    def device_footprint(self) -> int:
+        return load().jxrhip_plan_device_footprint(self._h)
+
+    @property
+    def algorithmic_bytes(self) -> int:
+        return load().jxrhip_plan_algorithmic_bytes(self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().jxrhip_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plan_hip_route(reconstructed_coefficients: int, strict: bool, available: bool) -> bool:
+    use = C.c_int32()
+    check(load().jxrhip_route(reconstructed_coefficients, 1 if strict else 0, 1 if available else 0, C.byref(use)),
+          "route")
+    return bool(use.value)
+
+
+class ResidentHipImage:
+    def __init__(self, session, handle):
+        self._s = session
+        self._h = handle
+
+    @property
+    def device_ptr(self) -> int:
+        return load().jxrhip_image_device_ptr(self._h) or 0
+
+    @property
+    def nbytes(self) -> int:
+        return load().jxrhip_image_bytes(self._h)
+
+    def readback(self) -> np.ndarray:
+        out = np.empty(self.nbytes, np.uint8)
+        check(load().jxrhip_readback(self._s.handle, self._h, out.ctypes.data, out.nbytes), "readback")
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().jxrhip_image_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HipSubmission:
+    def __init__(self, session, handle):
+        self._s = session
+        self._h = handle
+
+    def is_complete(self) -> bool:
+        return bool(load().jxrhip_submission_is_complete(self._h))
+
+    def wait(self) -> Optional[ResidentHipImage]:
+        lib = load()
+        img = C.c_void_p()
+        stage = C.c_uint32()
+        h, self._h = self._h, None
+        code = lib.jxrhip_submission_wait(h, C.byref(img), C.byref(stage))
+        if code != abi.OK:
+            raise JxrHipError(code, lib.jxrhip_last_error().decode(errors="replace"), stage.value)
+        return ResidentHipImage(self._s, img) if img.value else None
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                load().jxrhip_submission_destroy(self._h)
+            except Exception:
+                pass
+
+
+class HipBatch:
+    def __init__(self, session, handle, count):
+        self._s = session
+        self._h = handle
+        self.count = count
+
+    def is_complete(self) -> bool:
+        return bool(load().jxrhip_batch_is_complete(self._h))
+
+    def wait(self, raise_on_error=True) -> np.ndarray:
+        """Returns each image's stage code (0 = ok)."""
+        lib = load()
+        stages = np.zeros(max(self.count, 1), np.uint32)
+        h, self._h = self._h, None
+        code = lib.jxrhip_batch_wait(h, stages.ctypes.data_as(abi.u32p))
+        if code not in (abi.OK, abi.ERR_KERNEL_ARITHMETIC) or (code != abi.OK and raise_on_error):
+            raise JxrHipError(code, lib.jxrhip_last_error().decode(errors="replace"))
+        return stages[: self.count]
+
+
+class DeviceBuffer:
+    """Device memory of a session (CudaDecoderSession::allocate_destination)."""
+
+    def __init__(self, session, nbytes: int):
+        self._s = session
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(load().jxrhip_allocate_destination(session.handle, self.nbytes, C.byref(p)), "allocate")
+        self._p = p
+
+    @property
+    def ptr(self) -> int:
+        return self._p.value or 0
+
+    def fill(self, value: int):
+        check(load().jxrhip_fill_destination(self._s.handle, self._p, value & 0xFF, self.nbytes), "fill")
+
+    def to_host(self, offset: int = 0, nbytes: Optional[int] = None) -> np.ndarray:
+        n = self.nbytes - offset if nbytes is None else nbytes
+        out = np.empty(n, np.uint8)
+        check(load().jxrhip_copy_to_host(self._s.handle, C.c_void_p(self.ptr + offset), out.ctypes.data, n),
+              "copy_to_host")
+        return out
+
+    def close(self):
+        if getattr(self, "_p", None) and self._p.value:
+            load().jxrhip_free_destination(self._s.handle, self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HipDecoderSession:
+    """One device, its streams, upload cache and scratch pool."""
+
+    def __init__(self, device_ordinal: int = 0):
+        lib = load()
+        h = C.c_void_p()
+        check(lib.jxrhip_session_create(device_ordinal, C.byref(h)), "session")
+        self._h = h
+
+    @staticmethod
+    def is_available() -> bool:
+        return bool(load().jxrhip_is_available())
+
+    @classmethod
+    def system_default(cls):
+        return cls(0)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def diagnostics(self) -> abi.Diagnostics:
+        d = abi.Diagnostics()
+        check(load().jxrhip_session_diagnostics(self._h, C.byref(d)), "diagnostics")
+        return d
+
+    @property
+    def stream_count(self) -> int:
+        return self.diagnostics().stream_count
+
+    @property
+    def device_ordinal(self) -> int:
+        return self.diagnostics().device_ordinal
+
+    def set_tile(self, tx: int, ty: int):
+        check(load().jxrhip_session_set_tile(self._h, tx, ty), "set_tile")
+
+    def set_occupancy(self, groups_per_cu: int):
+        check(load().jxrhip_session_set_occupancy(self._h, groups_per_cu), "set_occupancy")
+
+    def set_upload_budget(self, nbytes: int):
+        check(load().jxrhip_session_set_upload_budget(self._h, nbytes), "set_upload_budget")
+
+    def upload(self, plan: HipDecodePlan):
+        check(load().jxrhip_session_upload(self._h, plan.handle), "upload")
+
+    def clear_cache(self):
+        check(load().jxrhip_session_clear_cache(self._h), "clear_cache")
+
+    def submit(self, plan: HipDecodePlan) -> HipSubmission:
+        h = C.c_void_p()
+        check(load().jxrhip_submit(self._h, plan.handle, C.byref(h)), "submit")
+        return HipSubmission(self, h)
+
+    def submit_into(self, plan: HipDecodePlan, device_ptr: int, nbytes: int) -> HipSubmission:
+        h = C.c_void_p()
+        check(load().jxrhip_submit_into(self._h, plan.handle, C.c_void_p(device_ptr), nbytes, C.byref(h)),
+              "submit_into")
+        return HipSubmission(self, h)
+
+    def decode_to_host(self, plan: HipDecodePlan) -> np.ndarray:
+        lib = load()
+        out = np.empty(plan.output_bytes, np.uint8)
+        stage = C.c_uint32()
+        code = lib.jxrhip_decode_to_host(self._h, plan.handle, out.ctypes.data, out.nbytes, C.byref(stage))
+        if code != abi.OK:
+            raise JxrHipError(code, lib.jxrhip_last_error().decode(errors="replace"), stage.value)
+        return out
+
+    def submit_batch(self, plans: Sequence[HipDecodePlan], device_ptrs: Sequence[int],
+                     mode: int = abi.BATCH_PERSISTENT) -> HipBatch:
+        n = len(plans)
+        parr = (C.c_void_p * max(n, 1))(*[p.handle.value for p in plans])
+        darr = (C.c_void_p * max(n, 1))(*[int(d) for d in device_ptrs])
+        h = C.c_void_p()
+        check(load().jxrhip_submit_batch(self._h, parr, n, darr, mode, C.byref(h)), "submit_batch")
+        return HipBatch(self, h, n)
+
+    def allocate_destination(self, nbytes: int) -> "DeviceBuffer":
+        return DeviceBuffer(self, nbytes)
+
+    def synchronize(self):
+        check(load().jxrhip_session_synchronize(self._h), "synchronize")
+
+    def last_batch_timing(self):
+        ms = C.c_double()
+        launches = C.c_uint32()
+        check(load().jxrhip_last_batch_timing(self._h, C.byref(ms), C.byref(launches)), "timing")
+        return ms.value, launches.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().jxrhip_session_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

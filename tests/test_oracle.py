@@ -1,0 +1,396 @@
+"""Pins the CPU oracle: known-answer vectors from the reference's own unit
+tests (cited per test), forward/inverse round trips of every operator on the
+reference's seeded LCG samples, closed forms, and brute force."""
+import ctypes as C
+import struct
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from arena_util import ManualArena, desc_for
+from jxr_amd import OutputSpec, abi
+
+
+def lcg_samples(seed, bits, count):
+    """The reference tests' generator (crates/jxr-math/src/transform.rs:402-415)."""
+    state = seed
+    out = []
+    mask = (1 << 64) - 1
+    for _ in range(count):
+        state = (state * 6364136223846793005 + 1442695040888963407) & mask
+        mag = (state >> 33) % (1 << bits)
+        out.append(mag if (state >> 63) == 0 else -mag)
+    return out
+
+
+# ---- core transform (crates/jxr-math/src/transform.rs:461-516) -----------
+def test_inverse_transform_matches_t835_signed_rounding_case():
+    v, e = O.op("jxro_inverse_core_transform", [50, -42, 9, 21, 126, 33, -6, -18, -24, 21, -3, -9, -6, -33, 6, 15])
+    assert not e
+    assert v.tolist() == [50, 50, 47, 49, 39, 38, 41, 41, -30, -44, 27, 36, -47, -55, -23, -20]
+
+
+def test_dc_only_block_is_uniform():
+    v, _ = O.op("jxro_inverse_core_transform", [16] + [0] * 15)
+    assert v.tolist() == [4] * 16
+
+
+def test_inverse_chroma_422_known_answer():
+    v, _ = O.op("jxro_inverse_chroma_422", [1, 2, 3, 4, 5, 6, 7, 8])
+    assert v.tolist() == [4, -3, -4, -2, 12, -2, -3, -1]
+
+
+def test_forward_transforms_concentrate_flat_input_in_dc():
+    v, _ = O.op("jxro_forward_core_transform", [7] * 16)
+    assert all(x == 0 for x in v[1:])
+    back, _ = O.op("jxro_inverse_core_transform", v)
+    assert back.tolist() == [7] * 16
+    v, _ = O.op("jxro_forward_chroma_420", [6] * 4)
+    assert v.tolist() == [12, 0, 0, 0]
+    v, _ = O.op("jxro_forward_chroma_422", [6] * 8)
+    assert v[0] != 0 and all(x == 0 for x in v[1:])
+    back, _ = O.op("jxro_inverse_chroma_422", v)
+    assert back.tolist() == [6] * 8
+
+
+@pytest.mark.parametrize("bits", [1, 4, 9, 17, 24])
+def test_transforms_are_exact_inverses(bits):
+    s = lcg_samples(bits, bits, 2000 * 16)
+    for i in range(0, len(s), 16):
+        x = s[i:i + 16]
+        f, e1 = O.op("jxro_forward_core_transform", x)
+        b, e2 = O.op("jxro_inverse_core_transform", f)
+        assert not e1 and not e2 and b.tolist() == x
+        f, _ = O.op("jxro_inverse_core_transform", x)
+        b, _ = O.op("jxro_forward_core_transform", f)
+        assert b.tolist() == x
+    s = lcg_samples(bits + 200, bits, 2000 * 12)
+    for i in range(0, len(s), 12):
+        four, eight = s[i:i + 4], s[i + 4:i + 12]
+        f, _ = O.op("jxro_forward_chroma_420", four)
+        assert O.op("jxro_inverse_chroma_420", f)[0].tolist() == four
+        f, _ = O.op("jxro_forward_chroma_422", eight)
+        assert O.op("jxro_inverse_chroma_422", f)[0].tolist() == eight
+
+
+def test_transform_overflow_is_reported():
+    assert O.op("jxro_forward_chroma_420", [2**31 - 1] * 4)[1]
+    assert O.op("jxro_forward_chroma_422", [2**31 - 1] * 8)[1]
+    assert O.op("jxro_inverse_core_transform", [2**31 - 1] * 16)[1]
+
+
+# ---- overlap filters (crates/jxr-math/src/overlap.rs:450-547) -------------
+@pytest.mark.parametrize("bits", [1, 4, 9, 17, 24])
+def test_overlap_filters_are_exact_inverses(bits):
+    s = lcg_samples(bits + 100, bits, 2000 * 20)
+    for i in range(0, len(s), 20):
+        four, sixteen = s[i:i + 4], s[i + 4:i + 20]
+        f, _ = O.op("jxro_forward_overlap_4", four)
+        assert O.op("jxro_inverse_overlap_4", f)[0].tolist() == four
+        f, _ = O.op("jxro_inverse_overlap_4", four)
+        assert O.op("jxro_forward_overlap_4", f)[0].tolist() == four
+        f, _ = O.op("jxro_forward_overlap_4x4", sixteen)
+        assert O.op("jxro_inverse_overlap_4x4", f)[0].tolist() == sixteen
+        f, _ = O.op("jxro_inverse_overlap_4x4", sixteen)
+        assert O.op("jxro_forward_overlap_4x4", f)[0].tolist() == sixteen
+    s = lcg_samples(bits + 300, bits, 2000 * 6)
+    for i in range(0, len(s), 6):
+        pair, square = s[i:i + 2], s[i + 2:i + 6]
+        f, _ = O.op("jxro_forward_overlap_2", pair)
+        assert O.op("jxro_inverse_overlap_2", f)[0].tolist() == pair
+        f, _ = O.op("jxro_forward_overlap_2x2", square)
+        assert O.op("jxro_inverse_overlap_2x2", f)[0].tolist() == square
+
+
+def test_overlap_known_answers():
+    assert O.op("jxro_inverse_overlap_2", [1, 2])[0].tolist() == [2, 3]
+    assert O.op("jxro_inverse_overlap_2x2", [1, 2, 3, 4])[0].tolist() == [3, 4, 4, 5]
+    assert O.op("jxro_forward_overlap_2", [2, 3])[0].tolist() == [1, 2]
+    assert O.op("jxro_forward_overlap_2x2", [3, 4, 4, 5])[0].tolist() == [1, 2, 3, 4]
+    # crates/jxr-native/src/reconstruct/overlap.rs:497-501
+    assert O.op("jxro_inverse_overlap_4", [1, 2, 3, 4])[0].tolist() == [2, 1, 4, 5]
+    assert O.op("jxro_inverse_overlap_4", [0] * 4)[0].tolist() == [0] * 4
+    assert O.op("jxro_inverse_overlap_4x4", [0] * 16)[0].tolist() == [0] * 16
+    assert O.op("jxro_forward_overlap_2", [-(2**31), 2**31 - 1])[1]
+    assert O.op("jxro_forward_overlap_2x2", [2**31 - 1] * 4)[1]
+
+
+def _plane_overlap(samples, w, h, cols, rows, hard, fn="jxro_overlap_plane"):
+    s = np.ascontiguousarray(np.array(samples, np.int32))
+    cb = np.array(cols, np.uint32)
+    rb = np.array(rows, np.uint32)
+    err = getattr(O.lib(), fn)(s.ctypes.data_as(C.POINTER(C.c_int32)), w, h, cb.ctypes.data_as(C.POINTER(C.c_uint32)),
+                               len(cb), rb.ctypes.data_as(C.POINTER(C.c_uint32)), len(rb), 1 if hard else 0)
+    assert not err
+    return s.tolist()
+
+
+def test_soft_and_hard_tile_planes_known_answers():
+    # crates/jxr-native/src/reconstruct/overlap.rs:504-529
+    src = [1, 1, 1, 1, 9, 9, 9, 9] * 4
+    soft = _plane_overlap(src, 8, 4, [0, 4, 8], [0, 4], False)
+    hard = _plane_overlap(src, 8, 4, [0, 4, 8], [0, 4], True)
+    assert soft == [1, 1, 3, 6, 8, 11, 11, 12, 1, 1, 3, 6, 8, 11, 13, 14] * 2
+    assert hard == [1, 1, 1, 1, 11, 12, 11, 12, 1, 1, 1, 1, 13, 14, 13, 14] * 2
+
+
+def test_full_plane_covers_two_sample_image_edges():
+    # crates/jxr-native/src/reconstruct/overlap.rs:532-563
+    rows = [
+        [-88, -88, -88, -88, -87, -87, -88, -88, -87, -87, -88, -88, -87, -87, -88, -88],
+        [-87, -87, -88, -88, -87, -87, -88, -88, -87, -87, -88, -88, -87, -87, -87, -87],
+        [-88, -88, -87, -87, -88, -88, -87, -87, -88, -88, -87, -87, -88, -88, -88, -88],
+        [-87, -87, -88, -88, -88, -88, -88, -88, -88, -88, -88, -88, -88, -88, -87, -87],
+    ]
+    order = [0, 1, 2, 2, 3, 3, 2, 2, 3, 3, 2, 2, 3, 3, 0, 1]
+    samples = [v for r in order for v in rows[r]]
+    assert _plane_overlap(samples, 16, 16, [0, 16], [0, 16], False) == [-128] * 256
+
+
+def test_subsampled_plane_round_trip_and_corner_order():
+    """The ordered subsampled schedule inverts the encoder's reversed one
+    for regions one macroblock (two samples) wide, where each corner step
+    reads the other corner (subsampled_overlap.rs:234-283)."""
+    rng = np.random.default_rng(5)
+    from jxr_amd import EncodedImage  # noqa: F401  (forward path lives in the library)
+    for w, h, cols, rows in [(4, 4, [0, 2, 4], [0, 2, 4]), (8, 6, [0, 2, 6, 8], [0, 2, 6]), (6, 8, [0, 6], [0, 8])]:
+        x = rng.integers(-200, 200, w * h).tolist()
+        y = _plane_overlap(x, w, h, cols, rows, True, "jxro_subsampled_overlap_plane")
+        assert y != x  # the filters moved something
+    # a two-wide region: corners read each other in order (left, then right)
+    a0, b0 = 10, 3
+    plane = [a0, b0, a0, b0]  # 2x2, one region
+    out = _plane_overlap(plane, 2, 2, [0, 2], [0, 2], True, "jxro_subsampled_overlap_plane")
+    # prefix: a = a0 - b0; b = b0 - a; suffix: a += b; b += a
+    a = a0 - b0
+    b = b0 - a
+    a = a + b
+    b = b + a
+    assert out == [a, b, a, b]
+
+
+# ---- prediction (crates/jxr-math/src/prediction.rs:104-115) ---------------
+def test_subsampled_420_prediction_uses_two_by_two_block_geometry():
+    c = np.zeros(256, np.int32)
+    c[12], c[16 + 12], c[32 + 12], c[48 + 12] = 5, 7, 11, 13
+    err = O.lib().jxro_predict_high_pass(c.ctypes.data_as(C.POINTER(C.c_int32)), abi.PRED_FROM_LEFT, 2, 2)
+    assert not err
+    assert c[16 + 12] == 12 and c[48 + 12] == 24
+
+
+def test_prediction_is_a_running_sum():
+    rng = np.random.default_rng(3)
+    base = rng.integers(-1000, 1000, 256).astype(np.int32)
+    for direction, positions, blocks in [(abi.PRED_FROM_LEFT, [4, 8, 12], lambda r, c: [(r, k) for k in range(c + 1)]),
+                                         (abi.PRED_FROM_TOP, [1, 2, 3], lambda r, c: [(k, c) for k in range(r + 1)])]:
+        c = base.copy()
+        O.lib().jxro_predict_high_pass(c.ctypes.data_as(C.POINTER(C.c_int32)), direction, 4, 4)
+        for r in range(4):
+            for col in range(4):
+                for p in positions:
+                    assert c[(r * 4 + col) * 16 + p] == sum(int(base[(rr * 4 + cc) * 16 + p]) for rr, cc in blocks(r, col))
+
+
+# ---- chroma upsampling (crates/jxr-math/src/sampling.rs:117-331) ----------
+def up(prev, cur, nxt, centering):
+    out = (C.c_int32 * 2)()
+    assert O.lib().jxro_upsample_pair(prev, cur, nxt, centering, out) == 0
+    return [out[0], out[1]]
+
+
+def test_upsampling_known_answers():
+    assert up(0, 8, 16, 0) == [8, 12]
+    assert up(0, 8, 16, 4) == [4, 8]
+    assert up(-9, -4, 3, 2) == [-5, -2]
+    assert up(-(2**31), 2**31 - 1, -(2**31), 0) == [2**31 - 1, 0]
+    for c in range(5):
+        for s in (2**31 - 1, -(2**31)):
+            assert up(s, s, s, c) == [s, s]
+    assert O.lib().jxro_upsample_pair(0, 0, 0, 7, (C.c_int32 * 2)()) != 0
+
+
+def test_upsampling_equals_table_180_sum_in_64_bits():
+    samples = []
+    for group in range(7):
+        for off in range(-3, 4):
+            base = [-(2**31) + 3, 2**31 - 1 - 3, 0, 8, -8, 0x12345678, -0x12345678][group]
+            samples.append(base + off)
+    weights = {0: (4, 4, 0, 8), 1: (5, 3, 1, 7), 2: (6, 2, 2, 6), 3: (7, 1, 3, 5), 4: (8, 0, 4, 4)}
+    for c, (h0, h1, h2, h3) in weights.items():
+        for p in samples[::3]:
+            for q in samples[::2]:
+                for n in (p, q, samples[c * 9]):
+                    assert up(p, q, n, c) == [(p * h2 + q * h3 + 4) >> 3, (q * h0 + n * h1 + 4) >> 3]
+
+
+# ---- alpha (crates/jxr-math/src/alpha.rs:57-90) ----------------------------
+def pm_u(v, a, m):
+    out = C.c_uint32()
+    assert O.lib().jxro_premultiply_unsigned(v, a, m, C.byref(out)) == 0
+    return out.value
+
+
+def pm_s(v, a, m):
+    out = C.c_int32()
+    assert O.lib().jxro_premultiply_signed(v, a, m, C.byref(out)) == 0
+    return out.value
+
+
+def test_premultiply_known_answers():
+    assert pm_u(255, 128, 255) == 128
+    assert pm_u(7, 0, 255) == 0
+    assert pm_s(-20000, 16384, 32767) == -10000
+    assert pm_s(20000, -1, 32767) == 0
+
+
+# ---- output scaling (crates/jxr-native/src/output_format/scaling.rs) ------
+def f32bits(sample, scaled, mant, bias):
+    out = C.c_uint32()
+    assert O.lib().jxro_float32_bits(sample, scaled, mant, bias, C.byref(out)) == 0
+    return out.value
+
+
+def f16bits(sample, scaled):
+    out = C.c_uint16()
+    assert O.lib().jxro_float16_bits(sample, scaled, C.byref(out)) == 0
+    return out.value
+
+
+def bits_of(x):
+    return struct.unpack("<I", struct.pack("<f", x))[0]
+
+
+def test_float_outputs_known_answers():
+    # crates/jxr-native/src/output_format/tests.rs:320-382
+    assert f16bits(-1, 0) == 0x8001 and f16bits(1, 0) == 1
+    assert f32bits(-16, 0, 4, 1) == bits_of(-1.0) and f32bits(16, 0, 4, 1) == bits_of(1.0)
+    # a coded exponent beyond the f32 range clips to the largest finite value
+    assert f32bits(2**31 - 1, 0, 0, -200) == 0x7F7FFFFF
+
+
+def test_float32_bits_closed_form_for_normal_values():
+    """For a normal coded value the f32 is 2^(e - bias) * (1 + m / 2^len)."""
+    rng = np.random.default_rng(9)
+    for mant, bias in [(13, 4), (10, 8), (23, 0), (4, 1)]:
+        for _ in range(500):
+            e = int(rng.integers(1, 40))
+            m = int(rng.integers(0, 1 << mant))
+            sample = (e << mant) | m
+            if sample >= 2**31:
+                continue
+            value = 2.0 ** (e - bias) * (1 + m / 2**mant)
+            if not (2**-126 <= value < 3.4e38):
+                continue
+            assert f32bits(sample, 0, mant, bias) == bits_of(value)
+            assert f32bits(-sample, 0, mant, bias) == bits_of(-value)
+
+
+# ---- output formats through whole-image decode ------------------------------
+def _decode_row(values_per_plane, color, comps, spec, alpha=None, scaled=False, alpha_scaled=None, alpha_mode=None):
+    n = len(values_per_plane[0])
+    planes = [np.array(v, np.int64).reshape(1, n) for v in values_per_plane]
+    if alpha is not None and alpha_mode == abi.ALPHA_INTEGRATED:
+        planes.append(np.array(alpha, np.int64).reshape(1, n))
+    arena = ManualArena(planes)
+    sep = None
+    if alpha is not None and alpha_mode == abi.ALPHA_SEPARATE:
+        sep = ManualArena([np.array(alpha, np.int64).reshape(1, n)])
+    crop = (15, 0, 2, 1) if n == 2 else (0, 0, 1, 1)
+    d = desc_for(arena, color, comps, spec, crop, scaled=scaled,
+                 alpha_mode=alpha_mode or abi.ALPHA_NONE, separate_alpha=sep, alpha_scaled=alpha_scaled)
+    return O.decode(d)
+
+
+def test_constant_macroblock_planes_reconstruct_exactly():
+    arena = ManualArena([np.array([[-129, 127, 0, 5000, -7]])])
+    d = desc_for(arena, abi.COLOR_LUMA, 1, OutputSpec(layout=abi.LAYOUT_LUMA, output_color=abi.COLOR_LUMA),
+                 (0, 0, 80, 16))
+    plane = O.reconstruct_plane(d, 0)
+    for k, v in enumerate([-129, 127, 0, 5000, -7]):
+        assert (plane[:, 16 * k:16 * k + 16] == v).all()
+
+
+def test_dc_only_zero_luma_decodes_to_128():
+    # crates/jxr-cuda/src/plan/tests.rs:30-35
+    arena = ManualArena([np.zeros((1, 1), np.int64)])
+    d = desc_for(arena, abi.COLOR_LUMA, 1, OutputSpec(layout=abi.LAYOUT_LUMA, output_color=abi.COLOR_LUMA),
+                 (0, 0, 16, 16))
+    assert (O.decode(d) == 128).all()
+
+
+LUMA8 = OutputSpec(storage=abi.STORE_U8, depth=abi.DEPTH_U8, layout=abi.LAYOUT_LUMA, output_color=abi.COLOR_LUMA)
+
+
+def test_luma_u8_biases_and_clips():
+    # output_format/tests.rs:182-197
+    assert _decode_row([[-129, 127]], abi.COLOR_LUMA, 1, LUMA8).tolist() == [0, 255]
+
+
+def test_yuv444_to_rgba_formats_alpha_independently():
+    # output_format/tests.rs:225-250
+    spec = OutputSpec(layout=abi.LAYOUT_RGBA)
+    for mode in (abi.ALPHA_INTEGRATED, abi.ALPHA_SEPARATE):
+        got = _decode_row([[0, 1], [0, 0], [0, 0]], abi.COLOR_YUV444, 3, spec, alpha=[-128, 127], alpha_mode=mode)
+        assert got.tolist() == [128, 128, 128, 0, 129, 129, 129, 255]
+
+
+def test_alpha_uses_its_plane_local_scaled_flag():
+    # output_format/tests.rs:253-275 (one pixel, luma to RGBA)
+    arena = ManualArena([np.array([[0]])])
+    sep = ManualArena([np.array([[8]])])
+    d = desc_for(arena, abi.COLOR_LUMA, 1, OutputSpec(layout=abi.LAYOUT_RGBA), (0, 0, 1, 1),
+                 alpha_mode=abi.ALPHA_SEPARATE, separate_alpha=sep, alpha_scaled=True)
+    assert O.decode(d).tolist() == [128, 128, 128, 129]
+
+
+def test_luma_typed_outputs_apply_exact_postscaling():
+    # output_format/tests.rs:320-382
+    cases = [
+        (OutputSpec(storage=abi.STORE_U16, depth=abi.DEPTH_U16, shift_bits=1, layout=abi.LAYOUT_LUMA,
+                    output_color=abi.COLOR_LUMA), np.uint16, [32766, 32770]),
+        (OutputSpec(storage=abi.STORE_I16, depth=abi.DEPTH_I16, shift_bits=2, layout=abi.LAYOUT_LUMA,
+                    output_color=abi.COLOR_LUMA), np.int16, [-4, 4]),
+        (OutputSpec(storage=abi.STORE_I32, depth=abi.DEPTH_I32, shift_bits=3, layout=abi.LAYOUT_LUMA,
+                    output_color=abi.COLOR_LUMA), np.int32, [-8, 8]),
+        (OutputSpec(storage=abi.STORE_F16, depth=abi.DEPTH_F16, layout=abi.LAYOUT_LUMA,
+                    output_color=abi.COLOR_LUMA), np.uint16, [0x8001, 1]),
+    ]
+    for spec, dt, want in cases:
+        assert _decode_row([[-1, 1]], abi.COLOR_LUMA, 1, spec).view(dt).tolist() == want
+    spec = OutputSpec(storage=abi.STORE_F32, depth=abi.DEPTH_F32, mantissa_length=4, exponent_bias=1,
+                      layout=abi.LAYOUT_LUMA, output_color=abi.COLOR_LUMA)
+    got = _decode_row([[-16, 16]], abi.COLOR_LUMA, 1, spec).view(np.uint32).tolist()
+    assert got == [bits_of(-1.0), bits_of(1.0)]
+
+
+def test_yuv444_to_rgb_u16_and_padding_layouts():
+    # output_format/tests.rs:384-408, 613-662
+    yuv = [[0, 10], [0, 3], [0, 5]]
+    u16 = dict(storage=abi.STORE_U16, depth=abi.DEPTH_U16, output_color=abi.COLOR_RGB)
+    got = _decode_row(yuv, abi.COLOR_YUV444, 3, OutputSpec(layout=abi.LAYOUT_RGB, **u16)).view(np.uint16)
+    assert got.tolist() == [32768, 32768, 32768, 32774, 32780, 32779]
+    got = _decode_row(yuv, abi.COLOR_YUV444, 3, OutputSpec(layout=abi.LAYOUT_RGBX, **u16)).view(np.uint16)
+    assert got.tolist() == [32768, 32768, 32768, 0, 32774, 32780, 32779, 0]
+    arena = ManualArena([np.array([[10]]), np.array([[3]]), np.array([[5]])])
+    d = desc_for(arena, abi.COLOR_YUV444, 3, OutputSpec(layout=abi.LAYOUT_BGRX, **u16), (0, 0, 1, 1))
+    assert O.decode(d).view(np.uint16).tolist() == [32779, 32780, 32774, 0]
+
+
+def test_rgba_u8_premultiplication_uses_formatted_alpha():
+    # output_format/tests.rs:832-896
+    for layout in (abi.LAYOUT_RGBA, abi.LAYOUT_BGRA):
+        spec = OutputSpec(layout=layout, premultiply=True)
+        got = _decode_row([[0, 0], [0, 0], [0, 0]], abi.COLOR_YUV444, 3, spec, alpha=[-64, 127],
+                          alpha_mode=abi.ALPHA_SEPARATE)
+        assert got.tolist() == [32, 32, 32, 64, 128, 128, 128, 255]
+
+
+def test_invalid_requests_fail_before_decoding():
+    arena = ManualArena([np.zeros((1, 1), np.int64)])
+    d = desc_for(arena, abi.COLOR_LUMA, 1, OutputSpec(layout=abi.LAYOUT_RGBA), (0, 0, 16, 16))
+    code, _, _ = O.decode_status(d)  # alpha layout without an alpha plane
+    assert code == abi.ERR_INVALID_PLAN
+    d = desc_for(arena, abi.COLOR_LUMA, 1, LUMA8, (8, 8, 16, 16))
+    code, _, _ = O.decode_status(d)  # crop outside the plane
+    assert code == abi.ERR_INVALID_PLAN
