@@ -330,53 +330,55 @@ __device__ __forceinline__ bool row_boundary(const DevPlane &p, uint32_t mby, ui
   return m == 0 || m == mby || mask_bit(p.rowmask, m);
 }
 
-// ---- stage 1: first level ----------------------------------------------------
-__device__ void stage_first(const DevImage &im, const DevPlane &p, const PlaneGeo &g,
-                            int *low, int tmx, int tmy, int nx, int ny, uint32_t *status) {
-  const int h = (int)low_halo(p.bc);
-  const int ax = nx + 2 * h, ay = ny + 2 * h;
+// ---- stage bodies, one work item each ---------------------------------------
+// Dequantize DC/LP and run the first-level inverse transform of macroblock
+// (mx, my), written to the plane's low region.
+__device__ void first_level_mb(const DevImage &im, const DevPlane &p, const PlaneGeo &g, int *low,
+                               int mx, int my, uint32_t *status) {
+  const MbRec r = p.mbs[(uint32_t)my * im.mbx + (uint32_t)mx];
+  const int *c = p.coef + r.coef_off;
+  const bool full = p.bc == 4;
   const uint32_t blocks = p.bc * p.br;
-  for (int t = threadIdx.x; t < ax * ay; t += blockDim.x) {
-    int mx = tmx - h + t % ax, my = tmy - h + t / ax;
-    if (mx < 0 || my < 0 || mx >= (int)im.mbx || my >= (int)im.mby) continue;
-    const MbRec r = p.mbs[(uint32_t)my * im.mbx + (uint32_t)mx];
-    const int *c = p.coef + r.coef_off;
-    int v[16];
-    bool ovf = false;
-    v[0] = dequant_checked(c[0], r.qdc, ovf);
+  int v[16];
+  bool ovf = false;
+  v[0] = dequant_checked(c[0], r.qdc, ovf);
+  const bool lp = r.bands != JXRHIP_BANDS_DC_ONLY;
+  const uint32_t stride = r.bands >= JXRHIP_BANDS_NO_FLEXBITS ? 16u : 1u;
 #pragma unroll
-    for (int k = 1; k < 16; k++) v[k] = 0;
-    if (r.bands != JXRHIP_BANDS_DC_ONLY) {
-      const uint32_t stride = r.bands >= JXRHIP_BANDS_NO_FLEXBITS ? 16u : 1u;
-      if (blocks == 16) {
+  for (uint32_t k = 1; k < 16; k++) v[k] = (lp && k < blocks) ? dequant_checked(c[k * stride], r.qlp, ovf) : 0;
+  uint32_t mag = 0;
 #pragma unroll
-        for (int k = 1; k < 16; k++) v[k] = dequant_checked(c[k * stride], r.qlp, ovf);
-      } else {
-        for (uint32_t k = 1; k < blocks; k++) v[k] = dequant_checked(c[k * stride], r.qlp, ovf);
+  for (int k = 0; k < 16; k++) mag |= magbits(v[k]);
+  if (full) {
+    ovf |= RUN_BOUNDED(mag, kPctBound, [&](auto &a) { inverse_core(v, a); });
+  } else if (p.br == 4) {
+    ovf |= RUN_BOUNDED(mag, kPctBound, [&](auto &a) { inverse_chroma422(v, a); });
+  } else {
+    ovf |= RUN_BOUNDED(mag, kPctBound, [&](auto &a) { inverse_chroma420(v, a); });
+  }
+  if (p.scale_first) {
+#pragma unroll
+    for (uint32_t k = 0; k < 16; k++) {
+      int d;
+      ovf |= k < blocks && __builtin_mul_overflow(v[k], 2, &d);
+      v[k] = (int)((uint32_t)v[k] << 1);
+    }
+  }
+  if (ovf) fail(status, JXRHIP_STAGE_FIRST_TRANSFORM);
+  int *o = low + (my * (int)p.br - g.loy) * (int)g.lrw + (mx * (int)p.bc - g.lox);
+  const int rw = (int)g.lrw;
+  if (full) {
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+      for (int cc = 0; cc < 4; cc++) o[rr * rw + cc] = v[rr * 4 + cc];
+  } else {
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++)
+      if (rr < (int)p.br) {
+        o[rr * rw] = v[rr * 2];
+        o[rr * rw + 1] = v[rr * 2 + 1];
       }
-    }
-    uint32_t mag = 0;
-#pragma unroll
-    for (int k = 0; k < 16; k++) mag |= magbits(v[k]);
-    if (blocks == 16) {
-      ovf |= RUN_BOUNDED(mag, kPctBound, [&](auto &a) { inverse_core(v, a); });
-    } else if (p.br == 4) {
-      ovf |= RUN_BOUNDED(mag, kPctBound, [&](auto &a) { inverse_chroma422(v, a); });
-    } else {
-      ovf |= RUN_BOUNDED(mag, kPctBound, [&](auto &a) { inverse_chroma420(v, a); });
-    }
-    if (p.scale_first) {
-      for (uint32_t k = 0; k < blocks; k++) {
-        int d;
-        ovf |= __builtin_mul_overflow(v[k], 2, &d);
-        v[k] = d;
-      }
-    }
-    if (ovf) fail(status, JXRHIP_STAGE_FIRST_TRANSFORM);
-    const int bx = mx * (int)p.bc - g.lox, by = my * (int)p.br - g.loy;
-    for (uint32_t rr = 0; rr < p.br; rr++)
-      for (uint32_t cc = 0; cc < p.bc; cc++)
-        low[(by + (int)rr) * (int)g.lrw + bx + (int)cc] = v[rr * p.bc + cc];
   }
 }
 
@@ -386,74 +388,79 @@ __device__ void stage_first(const DevImage &im, const DevPlane &p, const PlaneGe
 // Whole slots are interior 4x4 windows; slots split along one axis are four
 // 4-tap edge strips across the split; slots split along both are four 2x2
 // corners (reconstruct/overlap.rs:124-323; device_plan/overlap.rs:125-178).
-template <class Bx, class By>
-__device__ void overlap_full(int *s, int rw, int ox, int oy, int kx0, int ky0, int nkx, int nky,
-                             int width, int height, Bx xsplit, By ysplit, uint32_t *status) {
-  for (int t = threadIdx.x; t < nkx * nky; t += blockDim.x) {
-    const int kx = kx0 + t % nkx, ky = ky0 + t / nkx;
-    const int x0 = 4 * kx + 2, y0 = 4 * ky + 2;
-    const bool sx = xsplit(kx), sy = ysplit(ky);
-    int *base = s + (y0 - oy) * rw + (x0 - ox);
-    bool ovf = false;
-    if (!sx && !sy) {
-      int v[16];
-      uint32_t mag = 0;
+// Only positions inside [0, width) x [0, height) are touched.
+__device__ void overlap_slot(int *s, int rw, int ox, int oy, int kx, int ky, int width, int height,
+                             bool sx, bool sy, uint32_t *status) {
+  const int x0 = 4 * kx + 2, y0 = 4 * ky + 2;
+  int *base = s + (y0 - oy) * rw + (x0 - ox);
+  bool ovf = false;
+  if (!sx && !sy) {
+    int v[16];
+    uint32_t mag = 0;
 #pragma unroll
-      for (int r = 0; r < 4; r++)
+    for (int r = 0; r < 4; r++) {
+      const int2 lo = *reinterpret_cast<const int2 *>(base + r * rw);
+      const int2 hi = *reinterpret_cast<const int2 *>(base + r * rw + 2);
+      v[r * 4 + 0] = lo.x;
+      v[r * 4 + 1] = lo.y;
+      v[r * 4 + 2] = hi.x;
+      v[r * 4 + 3] = hi.y;
+    }
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-          v[r * 4 + c] = base[r * rw + c];
-          mag |= magbits(v[r * 4 + c]);
-        }
-      ovf = RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap4x4(v, a); });
+    for (int k = 0; k < 16; k++) mag |= magbits(v[k]);
+    ovf = RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap4x4(v, a); });
 #pragma unroll
-      for (int r = 0; r < 4; r++)
+    for (int r = 0; r < 4; r++) {
+      *reinterpret_cast<int2 *>(base + r * rw) = make_int2(v[r * 4 + 0], v[r * 4 + 1]);
+      *reinterpret_cast<int2 *>(base + r * rw + 2) = make_int2(v[r * 4 + 2], v[r * 4 + 3]);
+    }
+  } else if (sx && !sy) {
+    // vertical 4-taps on each existing column
 #pragma unroll
-        for (int c = 0; c < 4; c++) base[r * rw + c] = v[r * 4 + c];
-    } else if (sx && !sy) {
-      // vertical 4-taps on each existing column
-      for (int c = 0; c < 4; c++) {
-        const int x = x0 + c;
-        if (x < 0 || x >= width) continue;
-        int *q = base + c;
-        int a0 = q[0], a1 = q[rw], a2 = q[2 * rw], a3 = q[3 * rw];
-        uint32_t mag = magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
-        ovf |= RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap4(a0, a1, a2, a3, a); });
-        q[0] = a0;
-        q[rw] = a1;
-        q[2 * rw] = a2;
-        q[3 * rw] = a3;
-      }
-    } else if (!sx && sy) {
-      for (int r = 0; r < 4; r++) {
-        const int y = y0 + r;
-        if (y < 0 || y >= height) continue;
-        int *q = base + r * rw;
-        int a0 = q[0], a1 = q[1], a2 = q[2], a3 = q[3];
-        uint32_t mag = magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
+    for (int c = 0; c < 4; c++) {
+      const int x = x0 + c;
+      if (x < 0 || x >= width) continue;
+      int *q = base + c;
+      int a0 = q[0], a1 = q[rw], a2 = q[2 * rw], a3 = q[3 * rw];
+      const uint32_t mag = magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
+      ovf |= RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap4(a0, a1, a2, a3, a); });
+      q[0] = a0;
+      q[rw] = a1;
+      q[2 * rw] = a2;
+      q[3 * rw] = a3;
+    }
+  } else if (!sx && sy) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int y = y0 + r;
+      if (y < 0 || y >= height) continue;
+      int *q = base + r * rw;
+      int a0 = q[0], a1 = q[1], a2 = q[2], a3 = q[3];
+      const uint32_t mag = magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
+      ovf |= RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap4(a0, a1, a2, a3, a); });
+      q[0] = a0;
+      q[1] = a1;
+      q[2] = a2;
+      q[3] = a3;
+    }
+  } else {
+#pragma unroll
+    for (int qy = 0; qy < 2; qy++)
+#pragma unroll
+      for (int qx = 0; qx < 2; qx++) {
+        const int x = x0 + 2 * qx, y = y0 + 2 * qy;
+        if (x < 0 || x >= width || y < 0 || y >= height) continue;
+        int *q = base + 2 * qy * rw + 2 * qx;
+        int a0 = q[0], a1 = q[1], a2 = q[rw], a3 = q[rw + 1];
+        const uint32_t mag = magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
         ovf |= RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap4(a0, a1, a2, a3, a); });
         q[0] = a0;
         q[1] = a1;
-        q[2] = a2;
-        q[3] = a3;
+        q[rw] = a2;
+        q[rw + 1] = a3;
       }
-    } else {
-      for (int qy = 0; qy < 2; qy++)
-        for (int qx = 0; qx < 2; qx++) {
-          const int x = x0 + 2 * qx, y = y0 + 2 * qy;
-          if (x < 0 || x >= width || y < 0 || y >= height) continue;
-          int *q = base + 2 * qy * rw + 2 * qx;
-          int a0 = q[0], a1 = q[1], a2 = q[rw], a3 = q[rw + 1];
-          uint32_t mag = magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
-          ovf |= RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap4(a0, a1, a2, a3, a); });
-          q[0] = a0;
-          q[1] = a1;
-          q[rw] = a2;
-          q[rw + 1] = a3;
-        }
-    }
-    if (ovf) fail(status, JXRHIP_STAGE_OVERLAP);
   }
+  if (ovf) fail(status, JXRHIP_STAGE_OVERLAP);
 }
 
 // ---- subsampled first-level overlap (subsampled_overlap.rs:110-283) -----
@@ -462,6 +469,7 @@ __device__ void overlap_full(int *s, int rw, int ox, int oy, int kx0, int ky0, i
 // two vertical pairs (the regions' side columns). y on a boundary: two
 // horizontal pairs on the regions' boundary rows. Both: region corners, which
 // the corner residual steps before and after the pairs handle.
+
 // Nearest region boundaries (MB columns) at or left of m, and right of m.
 __device__ int boundary_at_or_left(const DevPlane &p, int m) {
   if (!p.colmask) return 0;
@@ -476,178 +484,184 @@ __device__ int boundary_right(const DevPlane &p, uint32_t mbx, int m) {
   return (int)mbx;
 }
 
-__device__ void subsampled_corners(const DevImage &im, const DevPlane &p, const PlaneGeo &g,
-                                   int *low, int tmx, int tmy, int nx, int ny, bool add,
-                                   uint32_t *status) {
-  // One thread per (region, boundary row) held in the low region runs that
-  // row's two corners in the reference's order (left, then right): in a
-  // region one macroblock wide each corner is the other's neighbour.
+struct CornerRange {
+  int mx0, mx1, my0, my1;
+  __device__ int count() const { return (mx1 - mx0) * (my1 - my0) * 2; }
+};
+__device__ CornerRange corner_range(const DevImage &im, const DevPlane &p, int tmx, int tmy, int nx, int ny) {
   const int h = (int)low_halo(p.bc);
-  const int mx0 = max(tmx - h, 0), mx1 = min(tmx + nx + h, (int)im.mbx);
-  const int my0 = max(tmy - h, 0), my1 = min(tmy + ny + h, (int)im.mby);
-  const int ncols = mx1 - mx0;
-  const int nrows = (my1 - my0) * 2;
-  for (int t = threadIdx.x; t < ncols * nrows; t += blockDim.x) {
-    const int m = mx0 + t % ncols, ri = t / ncols;
-    const int L = boundary_at_or_left(p, m);
-    if (m != max(L, mx0)) continue;  // the region's first held column owns it
-    const int R = boundary_right(p, im.mbx, m);
-    const int my = my0 + ri / 2;
-    int row;
-    if ((ri & 1) == 0) {
-      if (!row_boundary(p, im.mby, (uint32_t)my)) continue;
-      row = my * (int)p.br;
-    } else {
-      if (!row_boundary(p, im.mby, (uint32_t)(my + 1))) continue;
-      row = (my + 1) * (int)p.br - 1;
-    }
-    int *base = low + (row - g.loy) * (int)g.lrw;
-    const int cl = L * 2, cr = R * 2 - 1;  // corner columns (low coords)
-    const int lo = g.lox, hi = g.lox + (int)g.lrw;  // columns held
-    Checked a;
-    if (!add) {
-      if (cl >= lo && cl + 1 < hi) base[cl - lo] = a.sub(base[cl - lo], base[cl + 1 - lo]);
-      if (cr - 1 >= lo && cr < hi) base[cr - lo] = a.sub(base[cr - lo], base[cr - 1 - lo]);
-    } else {
-      if (cl >= lo && cl + 1 < hi) base[cl - lo] = a.add(base[cl - lo], base[cl + 1 - lo]);
-      if (cr - 1 >= lo && cr < hi) base[cr - lo] = a.add(base[cr - lo], base[cr - 1 - lo]);
-    }
-    if (a.ovf) fail(status, JXRHIP_STAGE_OVERLAP);
-  }
+  return CornerRange{max(tmx - h, 0), min(tmx + nx + h, (int)im.mbx), max(tmy - h, 0),
+                     min(tmy + ny + h, (int)im.mby)};
 }
 
-__device__ void subsampled_cells(const DevImage &im, const DevPlane &p, const PlaneGeo &g, int *low,
-                                 int tmx, int tmy, int nx, int ny, uint32_t *status) {
-  // cells x in [2(tmx-1), 2(tmx+nx+1)], y in [br*tmy, br*(tmy+ny)] (step 2)
-  const int cx0 = 2 * (tmx - 1), cx1 = 2 * (tmx + nx + 1);
-  const int cy0 = (int)p.br * tmy, cy1 = (int)p.br * (tmy + ny);
-  const int ncx = (cx1 - cx0) / 2 + 1, ncy = (cy1 - cy0) / 2 + 1;
+// One (region, boundary row) of the low region: that row's two corners in the
+// reference's order (left, then right); in a region one macroblock wide each
+// corner is the other's neighbour.
+__device__ void subsampled_corner_item(const DevImage &im, const DevPlane &p, const PlaneGeo &g, int *low,
+                                       const CornerRange &cr, int t, bool add, uint32_t *status) {
+  const int ncols = cr.mx1 - cr.mx0;
+  const int m = cr.mx0 + t % ncols, ri = t / ncols;
+  const int L = boundary_at_or_left(p, m);
+  if (m != max(L, cr.mx0)) return;  // the region's first held column owns it
+  const int R = boundary_right(p, im.mbx, m);
+  const int my = cr.my0 + ri / 2;
+  int row;
+  if ((ri & 1) == 0) {
+    if (!row_boundary(p, im.mby, (uint32_t)my)) return;
+    row = my * (int)p.br;
+  } else {
+    if (!row_boundary(p, im.mby, (uint32_t)(my + 1))) return;
+    row = (my + 1) * (int)p.br - 1;
+  }
+  int *base = low + (row - g.loy) * (int)g.lrw;
+  const int cl = L * 2, crr = R * 2 - 1;          // corner columns (low coords)
+  const int lo = g.lox, hi = g.lox + (int)g.lrw;  // columns held
+  Checked a;
+  if (!add) {
+    if (cl >= lo && cl + 1 < hi) base[cl - lo] = a.sub(base[cl - lo], base[cl + 1 - lo]);
+    if (crr - 1 >= lo && crr < hi) base[crr - lo] = a.sub(base[crr - lo], base[crr - 1 - lo]);
+  } else {
+    if (cl >= lo && cl + 1 < hi) base[cl - lo] = a.add(base[cl - lo], base[cl + 1 - lo]);
+    if (crr - 1 >= lo && crr < hi) base[crr - lo] = a.add(base[crr - lo], base[crr - 1 - lo]);
+  }
+  if (a.ovf) fail(status, JXRHIP_STAGE_OVERLAP);
+}
+
+// cells x in [2(tmx-1), 2(tmx+nx+1)], y in [br*tmy, br*(tmy+ny)], step 2
+__device__ __forceinline__ int cell_columns(int nx) { return nx + 3; }
+__device__ __forceinline__ int cell_rows(const DevPlane &p, int ny) { return ((int)p.br * ny) / 2 + 1; }
+
+__device__ void subsampled_cell_item(const DevImage &im, const DevPlane &p, const PlaneGeo &g, int *low,
+                                     int tmx, int tmy, int nx, int t, uint32_t *status) {
+  const int ncx = cell_columns(nx);
+  const int x = 2 * (tmx - 1) + 2 * (t % ncx), y = (int)p.br * tmy + 2 * (t / ncx);
   const int W = (int)g.lw, H = (int)g.lh;
-  for (int t = threadIdx.x; t < ncx * ncy; t += blockDim.x) {
-    const int x = cx0 + 2 * (t % ncx), y = cy0 + 2 * (t / ncx);
-    if (x < 0 || x > W || y < 0 || y > H) continue;
-    const bool sx = (x % 2 == 0) && col_boundary(p, im.mbx, (uint32_t)(x / 2));
-    const bool sy = (y % (int)p.br == 0) && row_boundary(p, im.mby, (uint32_t)(y / (int)p.br));
-    int *q = low + (y - 1 - g.loy) * (int)g.lrw + (x - 1 - g.lox);
-    const int rw = (int)g.lrw;
-    bool ovf = false;
-    if (!sx && !sy) {
-      int a0 = q[0], a1 = q[1], a2 = q[rw], a3 = q[rw + 1];
-      uint32_t mag = magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
-      ovf = RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap2x2(a0, a1, a2, a3, a); });
-      q[0] = a0;
-      q[1] = a1;
-      q[rw] = a2;
-      q[rw + 1] = a3;
-    } else if (sx && !sy) {
-      for (int c = 0; c < 2; c++) {
-        const int xx = x - 1 + c;
-        if (xx < 0 || xx >= W) continue;
-        int a0 = q[c], a1 = q[rw + c];
-        uint32_t mag = magbits(a0) | magbits(a1);
-        ovf |= RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap2(a0, a1, a); });
-        q[c] = a0;
-        q[rw + c] = a1;
-      }
-    } else if (!sx && sy) {
-      for (int r = 0; r < 2; r++) {
-        const int yy = y - 1 + r;
-        if (yy < 0 || yy >= H) continue;
-        int a0 = q[r * rw], a1 = q[r * rw + 1];
-        uint32_t mag = magbits(a0) | magbits(a1);
-        ovf |= RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap2(a0, a1, a); });
-        q[r * rw] = a0;
-        q[r * rw + 1] = a1;
-      }
+  if (x < 0 || x > W || y < 0 || y > H) return;
+  const bool sx = col_boundary(p, im.mbx, (uint32_t)(x / 2));
+  const bool sy = (y % (int)p.br == 0) && row_boundary(p, im.mby, (uint32_t)(y / (int)p.br));
+  const int rw = (int)g.lrw;
+  int *q = low + (y - 1 - g.loy) * rw + (x - 1 - g.lox);
+  bool ovf = false;
+  if (!sx && !sy) {
+    int a0 = q[0], a1 = q[1], a2 = q[rw], a3 = q[rw + 1];
+    const uint32_t mag = magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
+    ovf = RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap2x2(a0, a1, a2, a3, a); });
+    q[0] = a0;
+    q[1] = a1;
+    q[rw] = a2;
+    q[rw + 1] = a3;
+  } else if (sx && !sy) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const int xx = x - 1 + c;
+      if (xx < 0 || xx >= W) continue;
+      int a0 = q[c], a1 = q[rw + c];
+      const uint32_t mag = magbits(a0) | magbits(a1);
+      ovf |= RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap2(a0, a1, a); });
+      q[c] = a0;
+      q[rw + c] = a1;
     }
-    if (ovf) fail(status, JXRHIP_STAGE_OVERLAP);
+  } else if (!sx && sy) {
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      const int yy = y - 1 + r;
+      if (yy < 0 || yy >= H) continue;
+      int a0 = q[r * rw], a1 = q[r * rw + 1];
+      const uint32_t mag = magbits(a0) | magbits(a1);
+      ovf |= RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap2(a0, a1, a); });
+      q[r * rw] = a0;
+      q[r * rw + 1] = a1;
+    }
   }
+  if (ovf) fail(status, JXRHIP_STAGE_OVERLAP);
 }
 
-// ---- stage 3: high-pass prediction + second level --------------------------
-__device__ void stage_second(const DevImage &im, const DevPlane &p, const PlaneGeo &g,
-                             const int *low, int *smp, int tmx, int tmy, int nx, int ny,
-                             uint32_t *status) {
-  const int bx0 = max(tmx * (int)p.bc - 1, 0);
-  const int bx1 = min((tmx + nx) * (int)p.bc + 1, (int)g.lw);
-  const int by0 = max(tmy * (int)p.br - 1, 0);
-  const int by1 = min((tmy + ny) * (int)p.br + 1, (int)g.lh);
-  const int nbx = bx1 - bx0, nby = by1 - by0;
-  for (int t = threadIdx.x; t < nbx * nby; t += blockDim.x) {
-    const int bx = bx0 + t % nbx, by = by0 + t / nbx;
-    const int mx = bx / (int)p.bc, my = by / (int)p.br;
-    const int cc = bx - mx * (int)p.bc, rr = by - my * (int)p.br;
-    const MbRec r = p.mbs[(uint32_t)my * im.mbx + (uint32_t)mx];
-    int v[16];
+// ---- high-pass prediction + second level, one 4x4 block ---------------------
+// Blocks of the tile plus a one-block halo: bx in [bx0, bx1), by in [by0, by1).
+struct BlockRange {
+  int bx0, bx1, by0, by1;
+  __device__ int count() const { return (bx1 - bx0) * (by1 - by0); }
+};
+__device__ BlockRange block_range(const DevPlane &p, const PlaneGeo &g, int tmx, int tmy, int nx, int ny) {
+  return BlockRange{max(tmx * (int)p.bc - 1, 0), min((tmx + nx) * (int)p.bc + 1, (int)g.lw),
+                    max(tmy * (int)p.br - 1, 0), min((tmy + ny) * (int)p.br + 1, (int)g.lh)};
+}
+
+__device__ void second_level_block(const DevImage &im, const DevPlane &p, const PlaneGeo &g, const int *low,
+                                   int *smp, const BlockRange &br, int t, uint32_t *status) {
+  const int nbx = br.bx1 - br.bx0;
+  const int bx = br.bx0 + t % nbx, by = br.by0 + t / nbx;
+  const int mx = bx / (int)p.bc, my = by / (int)p.br;
+  const int cc = bx - mx * (int)p.bc, rr = by - my * (int)p.br;
+  const MbRec r = p.mbs[(uint32_t)my * im.mbx + (uint32_t)mx];
+  int v[16];
 #pragma unroll
-    for (int k = 0; k < 16; k++) v[k] = 0;
-    bool povf = false, ovf = false;
-    if (r.bands >= JXRHIP_BANDS_NO_FLEXBITS) {
-      const int *c = p.coef + r.coef_off;
-      const int k = rr * (int)p.bc + cc;
-      const int *blk = c + k * 16;
-      if ((r.coef_off & 3u) == 0) {
-        const int4 *b4 = reinterpret_cast<const int4 *>(blk);
-        int4 q0 = b4[0], q1 = b4[1], q2 = b4[2], q3 = b4[3];
-        v[1] = q0.y; v[2] = q0.z; v[3] = q0.w;
-        v[4] = q1.x; v[5] = q1.y; v[6] = q1.z; v[7] = q1.w;
-        v[8] = q2.x; v[9] = q2.y; v[10] = q2.z; v[11] = q2.w;
-        v[12] = q3.x; v[13] = q3.y; v[14] = q3.z; v[15] = q3.w;
-      } else {
+  for (int k = 0; k < 16; k++) v[k] = 0;
+  bool povf = false, ovf = false;
+  if (r.bands >= JXRHIP_BANDS_NO_FLEXBITS) {
+    const int *c = p.coef + r.coef_off;
+    const int k = rr * (int)p.bc + cc;
+    const int *blk = c + k * 16;
+    if ((r.coef_off & 3u) == 0) {
+      const int4 *b4 = reinterpret_cast<const int4 *>(blk);
+      const int4 q0 = b4[0], q1 = b4[1], q2 = b4[2], q3 = b4[3];
+      v[1] = q0.y; v[2] = q0.z; v[3] = q0.w;
+      v[4] = q1.x; v[5] = q1.y; v[6] = q1.z; v[7] = q1.w;
+      v[8] = q2.x; v[9] = q2.y; v[10] = q2.z; v[11] = q2.w;
+      v[12] = q3.x; v[13] = q3.y; v[14] = q3.z; v[15] = q3.w;
+    } else {
 #pragma unroll
-        for (int j = 1; j < 16; j++) v[j] = blk[j];
-      }
-      // running sums of the residuals (prediction.rs:53-97)
-      if (r.pred == JXRHIP_PRED_FROM_LEFT && cc > 0) {
-        const int *row0 = c + rr * (int)p.bc * 16;
-        int s4 = row0[4], s8 = row0[8], s12 = row0[12];
-        for (int j = 1; j < cc; j++) {
-          const int *b = row0 + j * 16;
-          povf |= __builtin_add_overflow(b[4], s4, &s4);
-          povf |= __builtin_add_overflow(b[8], s8, &s8);
-          povf |= __builtin_add_overflow(b[12], s12, &s12);
-        }
-        povf |= __builtin_add_overflow(v[4], s4, &v[4]);
-        povf |= __builtin_add_overflow(v[8], s8, &v[8]);
-        povf |= __builtin_add_overflow(v[12], s12, &v[12]);
-      } else if (r.pred == JXRHIP_PRED_FROM_TOP && rr > 0) {
-        const int *col0 = c + cc * 16;
-        const int st = (int)p.bc * 16;
-        int s1 = col0[1], s2 = col0[2], s3 = col0[3];
-        for (int j = 1; j < rr; j++) {
-          const int *b = col0 + j * st;
-          povf |= __builtin_add_overflow(b[1], s1, &s1);
-          povf |= __builtin_add_overflow(b[2], s2, &s2);
-          povf |= __builtin_add_overflow(b[3], s3, &s3);
-        }
-        povf |= __builtin_add_overflow(v[1], s1, &v[1]);
-        povf |= __builtin_add_overflow(v[2], s2, &v[2]);
-        povf |= __builtin_add_overflow(v[3], s3, &v[3]);
-      }
-      uint32_t mag = 0;
-#pragma unroll
-      for (int j = 1; j < 16; j++) mag |= magbits(v[j]);
-      if ((unsigned long long)(mag + 1ull) * r.qhp <= 0x7fffffffull) {
-#pragma unroll
-        for (int j = 1; j < 16; j++) v[j] *= (int)r.qhp;
-      } else {
-#pragma unroll
-        for (int j = 1; j < 16; j++) v[j] = dequant_checked(v[j], r.qhp, ovf);
-      }
+      for (int j = 1; j < 16; j++) v[j] = blk[j];
     }
-    v[0] = low[(by - g.loy) * (int)g.lrw + (bx - g.lox)];
+    // running sums of the residuals (prediction.rs:53-97)
+    if (r.pred == JXRHIP_PRED_FROM_LEFT && cc > 0) {
+      const int *row0 = c + rr * (int)p.bc * 16;
+      int s4 = row0[4], s8 = row0[8], s12 = row0[12];
+      for (int j = 1; j < cc; j++) {
+        const int *b = row0 + j * 16;
+        povf |= __builtin_add_overflow(b[4], s4, &s4);
+        povf |= __builtin_add_overflow(b[8], s8, &s8);
+        povf |= __builtin_add_overflow(b[12], s12, &s12);
+      }
+      povf |= __builtin_add_overflow(v[4], s4, &v[4]);
+      povf |= __builtin_add_overflow(v[8], s8, &v[8]);
+      povf |= __builtin_add_overflow(v[12], s12, &v[12]);
+    } else if (r.pred == JXRHIP_PRED_FROM_TOP && rr > 0) {
+      const int *col0 = c + cc * 16;
+      const int st = (int)p.bc * 16;
+      int s1 = col0[1], s2 = col0[2], s3 = col0[3];
+      for (int j = 1; j < rr; j++) {
+        const int *b = col0 + j * st;
+        povf |= __builtin_add_overflow(b[1], s1, &s1);
+        povf |= __builtin_add_overflow(b[2], s2, &s2);
+        povf |= __builtin_add_overflow(b[3], s3, &s3);
+      }
+      povf |= __builtin_add_overflow(v[1], s1, &v[1]);
+      povf |= __builtin_add_overflow(v[2], s2, &v[2]);
+      povf |= __builtin_add_overflow(v[3], s3, &v[3]);
+    }
     uint32_t mag = 0;
 #pragma unroll
-    for (int j = 0; j < 16; j++) mag |= magbits(v[j]);
-    ovf |= RUN_BOUNDED(mag, kPctBound, [&](auto &a) { inverse_core(v, a); });
-    if (povf) fail(status, JXRHIP_STAGE_PREDICTION);
-    if (ovf) fail(status, JXRHIP_STAGE_SECOND_TRANSFORM);
-    int *o = smp + (by * 4 - g.soy) * (int)g.srw + (bx * 4 - g.sox);
+    for (int j = 1; j < 16; j++) mag |= magbits(v[j]);
+    if ((unsigned long long)(mag + 1ull) * r.qhp <= 0x7fffffffull) {
 #pragma unroll
-    for (int y = 0; y < 4; y++)
+      for (int j = 1; j < 16; j++) v[j] *= (int)r.qhp;
+    } else {
 #pragma unroll
-      for (int x = 0; x < 4; x++) o[y * (int)g.srw + x] = v[y * 4 + x];
+      for (int j = 1; j < 16; j++) v[j] = dequant_checked(v[j], r.qhp, ovf);
+    }
   }
+  v[0] = low[(by - g.loy) * (int)g.lrw + (bx - g.lox)];
+  uint32_t mag = 0;
+#pragma unroll
+  for (int j = 0; j < 16; j++) mag |= magbits(v[j]);
+  ovf |= RUN_BOUNDED(mag, kPctBound, [&](auto &a) { inverse_core(v, a); });
+  if (povf) fail(status, JXRHIP_STAGE_PREDICTION);
+  if (ovf) fail(status, JXRHIP_STAGE_SECOND_TRANSFORM);
+  int *o = smp + (by * 4 - g.soy) * (int)g.srw + (bx * 4 - g.sox);
+#pragma unroll
+  for (int y = 0; y < 4; y++)
+    *reinterpret_cast<int4 *>(o + y * (int)g.srw) = make_int4(v[y * 4], v[y * 4 + 1], v[y * 4 + 2], v[y * 4 + 3]);
 }
 
 // ---- output stage -------------------------------------------------------
@@ -663,38 +677,25 @@ __device__ __forceinline__ int upsample_pick(int prev, int cur, int next, uint32
   return phase == 0 ? weighted(prev, h2, cur, h3) : weighted(cur, h0, next, h1);
 }
 
+// The output stage's view of a tile: the image descriptor and per-plane
+// geometry (both in LDS) and the dynamic LDS holding the sample regions.
 struct OutCtx {
   const DevImage *im;
-  const int *S[kMaxPlanes];
-  PlaneGeo G[kMaxPlanes];
+  const PlaneGeo *G;
+  const int *soff;
+  const int *lds;
 };
 
 __device__ __forceinline__ int sample_at(const OutCtx &o, int pl, int x, int y) {
   const PlaneGeo &g = o.G[pl];
-  return o.S[pl][(y - g.soy) * (int)g.srw + (x - g.sox)];
+  return o.lds[o.soff[pl] + (y - g.soy) * (int)g.srw + (x - g.sox)];
 }
 __device__ __forceinline__ int chroma_clamped(const OutCtx &o, int pl, int cx, int cy) {
   const PlaneGeo &g = o.G[pl];
   cx = min(max(cx, 0), (int)g.pw - 1);
   cy = min(max(cy, 0), (int)g.ph - 1);
-  return o.S[pl][(cy - g.soy) * (int)g.srw + (cx - g.sox)];
+  return o.lds[o.soff[pl] + (cy - g.soy) * (int)g.srw + (cx - g.sox)];
 }
-__device__ int chroma_at(const OutCtx &o, int pl, int x, int y) {
-  const DevImage &im = *o.im;
-  const int cx = x >> 1;
-  if (im.upsample == 2) {
-    return upsample_pick(chroma_clamped(o, pl, cx - 1, y), chroma_clamped(o, pl, cx, y),
-                         chroma_clamped(o, pl, cx + 1, y), im.centering_x, x & 1);
-  }
-  const int cy = y >> 1;
-  int vert[3];
-#pragma unroll
-  for (int k = -1; k <= 1; k++)
-    vert[k + 1] = upsample_pick(chroma_clamped(o, pl, cx + k, cy - 1), chroma_clamped(o, pl, cx + k, cy),
-                                chroma_clamped(o, pl, cx + k, cy + 1), im.centering_y, y & 1);
-  return upsample_pick(vert[0], vert[1], vert[2], im.centering_x, x & 1);
-}
-
 // scaling.rs:104-124; returns false on overflow
 __device__ __forceinline__ bool apply_scale(const ChanScale &s, int v, int &out) {
   if (s.flags & 2u) {
@@ -788,88 +789,83 @@ __device__ __forceinline__ void store_sample(uint8_t *dst, uint32_t value, uint3
     *reinterpret_cast<uint32_t *>(dst) = value;
 }
 
-__device__ void stage_output(const OutCtx &o, uint8_t *out, int tmx, int tmy, int nx, int ny,
-                             uint32_t *status) {
-  const DevImage &im = *o.im;
-  // window-relative full-resolution rectangle of the tile, clipped to the crop
-  const int x0 = max(tmx * 16, (int)im.crop_x), x1 = min((tmx + nx) * 16, (int)(im.crop_x + im.crop_w));
-  const int y0 = max(tmy * 16, (int)im.crop_y), y1 = min((tmy + ny) * 16, (int)(im.crop_y + im.crop_h));
-  if (x1 <= x0 || y1 <= y0) return;
-  const int w = x1 - x0, h = y1 - y0;
-  const bool unaligned_ok = true;
-  (void)unaligned_ok;
-  for (int t = threadIdx.x; t < w * h; t += blockDim.x) {
-    const int x = x0 + t % w, y = y0 + t / w;
-    int in[4] = {0, 0, 0, 0};
-    for (uint32_t c = 0; c < im.colors && c < 4; c++)
-      in[c] = (im.upsample && (c == 1 || c == 2)) ? chroma_at(o, (int)c, x, y) : sample_at(o, (int)c, x, y);
-    bool ok = true;
-    int conv[4] = {0, 0, 0, 0};
-    switch (im.conv) {
-      case kConvLuma: conv[0] = in[0]; break;
-      case kConvLumaRgb: conv[0] = conv[1] = conv[2] = in[0]; break;
-      case kConvYuvRgb: {
-        // output_format/color.rs:100-110
-        int t0, g, rr, b;
-        ok &= !__builtin_sub_overflow(0, in[1], &t0);
-        ok &= !__builtin_sub_overflow(in[0], t0 >> 1, &g);
-        int s;
-        ok &= !__builtin_add_overflow(t0, g, &s);
-        ok &= !__builtin_sub_overflow(s, (in[2] >> 1) + (in[2] & 1), &rr);
-        ok &= !__builtin_add_overflow(in[2], rr, &b);
-        conv[0] = rr;
-        conv[1] = g;
-        conv[2] = b;
-        break;
-      }
-      case kConvYuvkCmyk: {
-        int black, m1, magenta, c1, cyan, yellow;
-        ok &= !__builtin_add_overflow(in[3], in[0] >> 1, &black);
-        ok &= !__builtin_sub_overflow(black, in[0], &m1);
-        ok &= !__builtin_sub_overflow(m1, in[1] >> 1, &magenta);
-        ok &= !__builtin_add_overflow(in[1], magenta, &c1);
-        ok &= !__builtin_add_overflow(c1, in[2] >> 1, &cyan);
-        ok &= !__builtin_sub_overflow(cyan, in[2], &yellow);
-        conv[0] = cyan;
-        conv[1] = magenta;
-        conv[2] = yellow;
-        conv[3] = black;
-        break;
-      }
-      case kConvYuvkDirect:
-        conv[0] = in[1];
-        conv[1] = in[2];
-        conv[2] = in[3];
-        conv[3] = in[0];
-        break;
-      default:
-        for (int c = 0; c < 4; c++) conv[c] = in[c];
-        break;
+// Converted, ordered and formatted samples of one pixel (packing.rs:333-357,
+// premultiply.rs); returns false on arithmetic overflow.
+__device__ __forceinline__ bool format_pixel(const DevImage &im, const int in[4], int alpha, uint32_t stored[6]) {
+  bool ok = true;
+  int conv[4] = {0, 0, 0, 0};
+  switch (im.conv) {
+    case kConvLuma: conv[0] = in[0]; break;
+    case kConvLumaRgb: conv[0] = conv[1] = conv[2] = in[0]; break;
+    case kConvYuvRgb: {
+      // output_format/color.rs:100-110
+      int t0, g, rr, b, s;
+      ok &= !__builtin_sub_overflow(0, in[1], &t0);
+      ok &= !__builtin_sub_overflow(in[0], t0 >> 1, &g);
+      ok &= !__builtin_add_overflow(t0, g, &s);
+      ok &= !__builtin_sub_overflow(s, (in[2] >> 1) + (in[2] & 1), &rr);
+      ok &= !__builtin_add_overflow(in[2], rr, &b);
+      conv[0] = rr;
+      conv[1] = g;
+      conv[2] = b;
+      break;
     }
-    int ord[6] = {0, 0, 0, 0, 0, 0};
-    if (im.reverse) {
-      ord[0] = conv[2];
-      ord[1] = conv[1];
-      ord[2] = conv[0];
+    case kConvYuvkCmyk: {
+      int black, m1, magenta, c1, cyan, yellow;
+      ok &= !__builtin_add_overflow(in[3], in[0] >> 1, &black);
+      ok &= !__builtin_sub_overflow(black, in[0], &m1);
+      ok &= !__builtin_sub_overflow(m1, in[1] >> 1, &magenta);
+      ok &= !__builtin_add_overflow(in[1], magenta, &c1);
+      ok &= !__builtin_add_overflow(c1, in[2] >> 1, &cyan);
+      ok &= !__builtin_sub_overflow(cyan, in[2], &yellow);
+      conv[0] = cyan;
+      conv[1] = magenta;
+      conv[2] = yellow;
+      conv[3] = black;
+      break;
+    }
+    case kConvYuvkDirect:
+      conv[0] = in[1];
+      conv[1] = in[2];
+      conv[2] = in[3];
+      conv[3] = in[0];
+      break;
+    default:
+      conv[0] = in[0];
+      conv[1] = in[1];
+      conv[2] = in[2];
+      conv[3] = in[3];
+      break;
+  }
+  int ord[6] = {0, 0, 0, 0, 0, 0};
+  if (im.reverse) {
+    ord[0] = conv[2];
+    ord[1] = conv[1];
+    ord[2] = conv[0];
+  } else {
+    ord[0] = conv[0];
+    ord[1] = conv[1];
+    ord[2] = conv[2];
+    ord[3] = conv[3];
+  }
+  const uint32_t pc = im.primary_count;
+#pragma unroll
+  for (uint32_t c = 0; c < 6; c++)
+    if (im.has_alpha && c == pc) ord[c] = alpha;
+#pragma unroll
+  for (uint32_t c = 0; c < 6; c++) {
+    stored[c] = 0;
+    if (c >= im.channels) continue;
+    const bool is_alpha = im.has_alpha && c == pc;
+    const ChanScale &cs = is_alpha ? im.alpha_scale : im.color_scale[c];
+    if (im.storage == JXRHIP_STORE_F16) {
+      stored[c] = (cs.flags & 2u) ? 0u : f16_bits(ord[c], is_alpha ? im.f_alpha_scaled : im.f_scaled, ok);
+    } else if (im.storage == JXRHIP_STORE_F32) {
+      stored[c] = (cs.flags & 2u) ? 0u
+                                  : f32_bits(ord[c], is_alpha ? im.f_alpha_scaled : im.f_scaled,
+                                             is_alpha ? im.f_alpha_mant : im.f_mant,
+                                             is_alpha ? im.f_alpha_bias : im.f_bias, ok);
     } else {
-      for (int c = 0; c < 4; c++) ord[c] = conv[c];
-    }
-    if (im.has_alpha) ord[im.primary_count] = sample_at(o, im.alpha_plane, x, y);
-    uint32_t stored[6];
-    for (uint32_t c = 0; c < im.channels; c++) {
-      const bool alpha = im.has_alpha && c == im.primary_count;
-      const ChanScale &cs = alpha ? im.alpha_scale : im.color_scale[c];
-      if (im.storage == JXRHIP_STORE_F16) {
-        stored[c] = (cs.flags & 2u) ? 0u : f16_bits(ord[c], alpha ? im.f_alpha_scaled : im.f_scaled, ok);
-        continue;
-      }
-      if (im.storage == JXRHIP_STORE_F32) {
-        stored[c] = (cs.flags & 2u) ? 0u
-                                    : f32_bits(ord[c], alpha ? im.f_alpha_scaled : im.f_scaled,
-                                               alpha ? im.f_alpha_mant : im.f_mant,
-                                               alpha ? im.f_alpha_bias : im.f_bias, ok);
-        continue;
-      }
       int v = 0;
       ok &= apply_scale(cs, ord[c], v);
       if (im.storage == JXRHIP_STORE_U8) v = min(max(v, 0), 255);
@@ -877,68 +873,210 @@ __device__ void stage_output(const OutCtx &o, uint8_t *out, int tmx, int tmy, in
       else if (im.storage == JXRHIP_STORE_I16) v = min(max(v, -32768), 32767);
       stored[c] = (uint32_t)v;
     }
-    if (im.premultiply && im.has_alpha) {
-      const uint32_t ai = im.channels - 1;
-      for (uint32_t c = 0; c < ai; c++) {
-        switch (im.storage) {
-          case JXRHIP_STORE_U8: stored[c] = pm_unsigned(stored[c], stored[ai], 255); break;
-          case JXRHIP_STORE_U16: stored[c] = pm_unsigned(stored[c], stored[ai], 65535); break;
-          case JXRHIP_STORE_I16: {
-            int r = 0;
-            ok &= pm_signed((int16_t)stored[c], (int16_t)stored[ai], 32767, r);
-            stored[c] = (uint32_t)r;
-            break;
-          }
-          case JXRHIP_STORE_I32: {
-            int r = 0;
-            ok &= pm_signed((int)stored[c], (int)stored[ai], INT32_MAX, r);
-            stored[c] = (uint32_t)r;
-            break;
-          }
-          case JXRHIP_STORE_F16: {
-            const uint32_t a = (stored[ai] & 0x8000u) ? 0u : (stored[ai] & 0x7fffu);
-            stored[c] = (stored[c] & 0x8000u) | pm_unsigned(stored[c] & 0x7fffu, a, 0x7fff);
-            break;
-          }
-          case JXRHIP_STORE_F32: {
-            const float a = __uint_as_float(stored[ai]);
-            const float ac = a < 0.0f ? 0.0f : (a > 1.0f ? 1.0f : a);
-            stored[c] = __float_as_uint(__fmul_rn(__uint_as_float(stored[c]), ac));
-            break;
-          }
-          default: break;
+  }
+  if (im.premultiply && im.has_alpha) {
+    const uint32_t ai = im.channels - 1;
+    uint32_t a = 0;
+#pragma unroll
+    for (uint32_t c = 0; c < 6; c++)
+      if (c == ai) a = stored[c];
+#pragma unroll
+    for (uint32_t c = 0; c < 5; c++) {
+      if (c >= ai) continue;
+      switch (im.storage) {
+        case JXRHIP_STORE_U8: stored[c] = pm_unsigned(stored[c], a, 255); break;
+        case JXRHIP_STORE_U16: stored[c] = pm_unsigned(stored[c], a, 65535); break;
+        case JXRHIP_STORE_I16: {
+          int r = 0;
+          ok &= pm_signed((int16_t)stored[c], (int16_t)a, 32767, r);
+          stored[c] = (uint32_t)r;
+          break;
         }
+        case JXRHIP_STORE_I32: {
+          int r = 0;
+          ok &= pm_signed((int)stored[c], (int)a, INT32_MAX, r);
+          stored[c] = (uint32_t)r;
+          break;
+        }
+        case JXRHIP_STORE_F16: {
+          const uint32_t am = (a & 0x8000u) ? 0u : (a & 0x7fffu);
+          stored[c] = (stored[c] & 0x8000u) | pm_unsigned(stored[c] & 0x7fffu, am, 0x7fff);
+          break;
+        }
+        case JXRHIP_STORE_F32: {
+          const float af = __uint_as_float(a);
+          const float ac = af < 0.0f ? 0.0f : (af > 1.0f ? 1.0f : af);
+          stored[c] = __float_as_uint(__fmul_rn(__uint_as_float(stored[c]), ac));
+          break;
+        }
+        default: break;
       }
     }
-    if (!ok) fail(status, JXRHIP_STAGE_OUTPUT);
-    const uint32_t ox = (uint32_t)(x - (int)im.crop_x), oy = (uint32_t)(y - (int)im.crop_y);
-    if (im.planar) {
-      for (uint32_t c = 0; c < im.channels; c++)
-        store_sample(out + im.plane_offset[c] + (uint64_t)oy * im.row_stride[c] + (uint64_t)ox * im.bytes,
-                     stored[c], im.bytes);
+  }
+  return ok;
+}
+
+// Four consecutive samples of one channel, stored at `dst` (the first
+// pixel's address); one vector store when all four are valid and the
+// address is aligned to the run, one store per valid sample otherwise.
+__device__ __forceinline__ void store_run(uint8_t *dst, const uint32_t s[4], uint32_t valid, uint32_t bytes,
+                                          size_t step) {
+  const uintptr_t addr = reinterpret_cast<uintptr_t>(dst);
+  if (valid == 0xFu && step == bytes && (addr & (4 * bytes - 1)) == 0) {
+    if (bytes == 1) {
+      *reinterpret_cast<uint32_t *>(dst) =
+          (s[0] & 0xFFu) | ((s[1] & 0xFFu) << 8) | ((s[2] & 0xFFu) << 16) | ((s[3] & 0xFFu) << 24);
+    } else if (bytes == 2) {
+      *reinterpret_cast<uint2 *>(dst) =
+          make_uint2((s[0] & 0xFFFFu) | (s[1] << 16), (s[2] & 0xFFFFu) | (s[3] << 16));
     } else {
-      uint8_t *px = out + im.plane_offset[0] + (uint64_t)oy * im.row_stride[0] +
-                    (uint64_t)ox * im.channels * im.bytes;
-      for (uint32_t c = 0; c < im.channels; c++) store_sample(px + c * im.bytes, stored[c], im.bytes);
+      *reinterpret_cast<uint4 *>(dst) = make_uint4(s[0], s[1], s[2], s[3]);
+    }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    if (valid & (1u << i)) store_sample(dst + i * step, s[i], bytes);
+}
+
+// One output group: pixels x = 4gx..4gx+3, y = 2gy..2gy+1 (window coords)
+// inside [x0, x1) x [y0, y1). 4:2:0 chroma is interpolated once for the
+// group: three chroma rows and four chroma columns give both rows' values.
+__device__ void output_group(const OutCtx &o, uint8_t *out, int gx, int gy, int x0, int x1, int y0, int y1,
+                             uint32_t *status) {
+  const DevImage &im = *o.im;
+  const int xb = 4 * gx, yb = 2 * gy;
+  int chroma[2][2][4];  // [U/V][row][pixel]
+  if (im.upsample == 1) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      int vert[2][4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const int cx = 2 * gx - 1 + c;
+        const int a = chroma_clamped(o, 1 + q, cx, gy - 1), b = chroma_clamped(o, 1 + q, cx, gy),
+                  d = chroma_clamped(o, 1 + q, cx, gy + 1);
+        vert[0][c] = upsample_pick(a, b, d, im.centering_y, 0);
+        vert[1][c] = upsample_pick(a, b, d, im.centering_y, 1);
+      }
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int k = i >> 1;
+          chroma[q][j][i] = upsample_pick(vert[j][k], vert[j][k + 1], vert[j][k + 2], im.centering_x, i & 1);
+        }
+    }
+  } else if (im.upsample == 2) {
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        int h[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) h[c] = chroma_clamped(o, 1 + q, 2 * gx - 1 + c, yb + j);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int k = i >> 1;
+          chroma[q][j][i] = upsample_pick(h[k], h[k + 1], h[k + 2], im.centering_x, i & 1);
+        }
+      }
+  }
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int y = yb + j;
+    if (y < y0 || y >= y1) continue;
+    uint32_t st[6][4];
+    uint32_t valid = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int x = xb + i;
+#pragma unroll
+      for (int c = 0; c < 6; c++) st[c][i] = 0;
+      if (x < x0 || x >= x1) continue;
+      valid |= 1u << i;
+      int in[4] = {0, 0, 0, 0};
+      in[0] = sample_at(o, 0, x, y);
+      if (im.colors > 1) {
+        if (im.upsample) {
+          in[1] = chroma[0][j][i];
+          in[2] = chroma[1][j][i];
+        } else {
+          in[1] = sample_at(o, 1, x, y);
+          in[2] = sample_at(o, 2, x, y);
+        }
+        if (im.colors > 3) in[3] = sample_at(o, 3, x, y);
+      }
+      const int a = im.has_alpha ? sample_at(o, im.alpha_plane, x, y) : 0;
+      uint32_t px[6];
+      ok &= format_pixel(im, in, a, px);
+#pragma unroll
+      for (int c = 0; c < 6; c++) st[c][i] = px[c];
+    }
+    if (!valid) continue;
+    const uint32_t oy = (uint32_t)(y - (int)im.crop_y);
+    const int ox0 = xb - (int)im.crop_x;  // output x of pixel 0 (may be < 0 when invalid)
+    if (im.planar || im.channels == 1) {
+#pragma unroll
+      for (uint32_t c = 0; c < 6; c++) {
+        if (c >= im.channels) continue;
+        const uint32_t pl = im.planar ? c : 0;
+        uint8_t *row = out + im.plane_offset[pl] + (uint64_t)oy * im.row_stride[pl];
+        store_run(row + (int64_t)ox0 * (int64_t)im.bytes, st[c], valid, im.bytes, im.bytes);
+      }
+    } else {
+      uint8_t *row = out + im.plane_offset[0] + (uint64_t)oy * im.row_stride[0];
+      const size_t pitch = (size_t)im.channels * im.bytes;
+#pragma unroll
+      for (uint32_t c = 0; c < 6; c++) {
+        if (c >= im.channels) continue;
+        store_run(row + (int64_t)ox0 * (int64_t)pitch + c * im.bytes, st[c], valid, im.bytes, pitch);
+      }
     }
   }
+  if (!ok) fail(status, JXRHIP_STAGE_OUTPUT);
 }
 
 // ---- one tile, every plane, then output -----------------------------------
-__device__ void process_tile(const Slot &slot, uint32_t tile, const KernelArgs &args, int *lds) {
-  const DevImage &im = *slot.img;
-  const int tx = (int)(tile % slot.tiles_x), ty = (int)(tile / slot.tiles_x);
-  const int TX = (int)args.tile_x, TY = (int)args.tile_y;
-  const int tmx = tx * TX, tmy = ty * TY;
+// Everything a tile indexes by plane lives in LDS: a copy of the image
+// descriptor, the per-plane region geometry and the per-stage work counts.
+// Each stage runs its work items for every plane in one flattened loop over
+// the workgroup, so a barrier separates stages, not planes.
+struct TileShared {
+  DevImage im;
+  PlaneGeo g[kMaxPlanes];
+  BlockRange br[kMaxPlanes];
+  CornerRange cr[kMaxPlanes];
+  int soff[kMaxPlanes];  // sample region of each plane, in dynamic LDS words
+  int loff[kMaxPlanes];  // low region of each plane
+  int sA[kMaxPlanes + 1], sB1[kMaxPlanes + 1], sB2[kMaxPlanes + 1], sB3[kMaxPlanes + 1];
+  int sC[kMaxPlanes + 1], sD[kMaxPlanes + 1];
+  int tmx, tmy, nx, ny;
+  int any_lp, any_sub, any_pot, pad;
+};
+
+__device__ __forceinline__ int plane_of(const int *start, int np, int t) {
+  int pl = 0;
+  while (pl + 1 < np && t >= start[pl + 1]) pl++;
+  return pl;
+}
+
+// Thread 0 lays out the tile: regions, ranges and work counts.
+__device__ void tile_setup(TileShared &ts, uint32_t tile, uint32_t tiles_x, int TX, int TY) {
+  const DevImage &im = ts.im;
+  const int tmx = (int)(tile % tiles_x) * TX, tmy = (int)(tile / tiles_x) * TY;
   const int nx = min(TX, (int)im.mbx - tmx), ny = min(TY, (int)im.mby - tmy);
-  uint32_t *status = slot.status;
-  OutCtx o;
-  o.im = &im;
-  // LDS: sample regions of every plane, then one shared low region
-  int *cursor = lds;
-  for (uint32_t pl = 0; pl < im.plane_count; pl++) {
+  const int np = (int)im.plane_count;
+  ts.tmx = tmx;
+  ts.tmy = tmy;
+  ts.nx = nx;
+  ts.ny = ny;
+  int cursor = 0;
+  int any_lp = 0, any_sub = 0, any_pot = 0;
+  for (int pl = 0; pl < np; pl++) {
     const DevPlane &p = im.planes[pl];
-    PlaneGeo &g = o.G[pl];
+    PlaneGeo &g = ts.g[pl];
     g.bc = p.bc;
     g.br = p.br;
     g.mw = p.bc * 4;
@@ -956,55 +1094,136 @@ __device__ void process_tile(const Slot &slot, uint32_t tile, const KernelArgs &
     g.loy = (tmy - h) * (int)p.br;
     g.lrw = (uint32_t)(TX + 2 * h) * p.bc;
     g.lrh = (uint32_t)(TY + 2 * h) * p.br;
-    o.S[pl] = cursor;
-    cursor += g.srw * g.srh;
+    ts.soff[pl] = cursor;
+    cursor += (int)(g.srw * g.srh);
+    any_lp |= p.overlap == 2;
+    any_sub |= p.overlap == 2 && p.bc == 2;
+    any_pot |= p.overlap >= 1;
+    ts.br[pl] = block_range(p, g, tmx, tmy, nx, ny);
+    ts.cr[pl] = corner_range(im, p, tmx, tmy, nx, ny);
   }
-  int *low = cursor;
-  for (uint32_t pl = 0; pl < im.plane_count; pl++) {
+  for (int pl = 0; pl < np; pl++) {
+    ts.loff[pl] = cursor;
+    cursor += (int)(ts.g[pl].lrw * ts.g[pl].lrh);
+  }
+  ts.any_lp = any_lp;
+  ts.any_sub = any_sub;
+  ts.any_pot = any_pot;
+  ts.sA[0] = ts.sB1[0] = ts.sB2[0] = ts.sB3[0] = ts.sC[0] = ts.sD[0] = 0;
+  for (int pl = 0; pl < np; pl++) {
     const DevPlane &p = im.planes[pl];
-    const PlaneGeo &g = o.G[pl];
-    int *smp = const_cast<int *>(o.S[pl]);
-    stage_first(im, p, g, low, tmx, tmy, nx, ny, status);
-    __syncthreads();
-    if (p.overlap == 2) {
+    const int h = (int)low_halo(p.bc);
+    const bool lp = p.overlap == 2, sub = lp && p.bc == 2;
+    ts.sA[pl + 1] = ts.sA[pl] + (nx + 2 * h) * (ny + 2 * h);
+    ts.sB1[pl + 1] = ts.sB1[pl] + (!lp ? 0 : p.bc == 4 ? (nx + 1) * (ny + 1) : ts.cr[pl].count());
+    ts.sB2[pl + 1] = ts.sB2[pl] + (sub ? cell_columns(nx) * cell_rows(p, ny) : 0);
+    ts.sB3[pl + 1] = ts.sB3[pl] + (sub ? ts.cr[pl].count() : 0);
+    ts.sC[pl + 1] = ts.sC[pl] + ts.br[pl].count();
+    ts.sD[pl + 1] = ts.sD[pl] + (p.overlap >= 1 ? (nx * (int)p.bc + 1) * (ny * (int)p.br + 1) : 0);
+  }
+}
+
+__device__ void process_tile(const Slot &slot, uint32_t tile, const KernelArgs &args, int *lds, TileShared &ts) {
+  {
+    const uint4 *src = reinterpret_cast<const uint4 *>(slot.img);
+    uint4 *dst = reinterpret_cast<uint4 *>(&ts.im);
+    for (int i = threadIdx.x; i < (int)(sizeof(DevImage) / sizeof(uint4)); i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) tile_setup(ts, tile, slot.tiles_x, (int)args.tile_x, (int)args.tile_y);
+  __syncthreads();
+  const DevImage &im = ts.im;
+  const int np = (int)im.plane_count;
+  const int tmx = ts.tmx, tmy = ts.tmy, nx = ts.nx, ny = ts.ny;
+  uint32_t *status = slot.status;
+
+  // stage A: first level of the tile's macroblocks and their halo
+  for (int t = threadIdx.x; t < ts.sA[np]; t += blockDim.x) {
+    const int pl = plane_of(ts.sA, np, t);
+    const int u = t - ts.sA[pl];
+    const DevPlane &p = im.planes[pl];
+    const int h = (int)low_halo(p.bc);
+    const int ax = nx + 2 * h;
+    const int mx = tmx - h + u % ax, my = tmy - h + u / ax;
+    if (mx < 0 || my < 0 || mx >= (int)im.mbx || my >= (int)im.mby) continue;
+    first_level_mb(im, p, ts.g[pl], lds + ts.loff[pl], mx, my, status);
+  }
+  __syncthreads();
+
+  // stage B: first-level overlap (mode 2)
+  if (ts.any_lp) {
+    for (int t = threadIdx.x; t < ts.sB1[np]; t += blockDim.x) {
+      const int pl = plane_of(ts.sB1, np, t);
+      const int u = t - ts.sB1[pl];
+      const DevPlane &p = im.planes[pl];
+      const PlaneGeo &g = ts.g[pl];
       if (p.bc == 4) {
-        auto xs = [&](int k) { return col_boundary(p, im.mbx, (uint32_t)(k + 1)); };
-        auto ys = [&](int k) { return row_boundary(p, im.mby, (uint32_t)(k + 1)); };
-        overlap_full(low, (int)g.lrw, g.lox, g.loy, tmx - 1, tmy - 1, nx + 1, ny + 1, (int)g.lw,
-                     (int)g.lh, xs, ys, status);
+        const int kx = tmx - 1 + u % (nx + 1), ky = tmy - 1 + u / (nx + 1);
+        overlap_slot(lds + ts.loff[pl], (int)g.lrw, g.lox, g.loy, kx, ky, (int)g.lw, (int)g.lh,
+                     col_boundary(p, im.mbx, (uint32_t)(kx + 1)), row_boundary(p, im.mby, (uint32_t)(ky + 1)),
+                     status);
       } else {
-        subsampled_corners(im, p, g, low, tmx, tmy, nx, ny, false, status);
-        __syncthreads();
-        subsampled_cells(im, p, g, low, tmx, tmy, nx, ny, status);
-        __syncthreads();
-        subsampled_corners(im, p, g, low, tmx, tmy, nx, ny, true, status);
+        subsampled_corner_item(im, p, g, lds + ts.loff[pl], ts.cr[pl], u, false, status);
+      }
+    }
+    if (ts.any_sub) {
+      __syncthreads();
+      for (int t = threadIdx.x; t < ts.sB2[np]; t += blockDim.x) {
+        const int pl = plane_of(ts.sB2, np, t);
+        subsampled_cell_item(im, im.planes[pl], ts.g[pl], lds + ts.loff[pl], tmx, tmy, nx, t - ts.sB2[pl], status);
       }
       __syncthreads();
+      for (int t = threadIdx.x; t < ts.sB3[np]; t += blockDim.x) {
+        const int pl = plane_of(ts.sB3, np, t);
+        subsampled_corner_item(im, im.planes[pl], ts.g[pl], lds + ts.loff[pl], ts.cr[pl], t - ts.sB3[pl], true,
+                               status);
+      }
     }
-    stage_second(im, p, g, low, smp, tmx, tmy, nx, ny, status);
     __syncthreads();
-    if (p.overlap >= 1) {
-      const int mw = (int)g.mw, mh = (int)g.mh;
-      auto xs = [&](int k) {
-        const int pos = 4 * k + 4;
-        return pos % mw == 0 && col_boundary(p, im.mbx, (uint32_t)(pos / mw));
-      };
-      auto ys = [&](int k) {
-        const int pos = 4 * k + 4;
-        return pos % mh == 0 && row_boundary(p, im.mby, (uint32_t)(pos / mh));
-      };
-      overlap_full(smp, (int)g.srw, g.sox, g.soy, tmx * (int)p.bc - 1, tmy * (int)p.br - 1,
-                   nx * (int)p.bc + 1, ny * (int)p.br + 1, (int)g.pw, (int)g.ph, xs, ys, status);
-      __syncthreads();
-    }
   }
-  stage_output(o, slot.out, tmx, tmy, nx, ny, status);
+
+  // stage C: high-pass prediction and second level, tile blocks + halo
+  for (int t = threadIdx.x; t < ts.sC[np]; t += blockDim.x) {
+    const int pl = plane_of(ts.sC, np, t);
+    second_level_block(im, im.planes[pl], ts.g[pl], lds + ts.loff[pl], lds + ts.soff[pl], ts.br[pl],
+                       t - ts.sC[pl], status);
+  }
+  __syncthreads();
+
+  // stage D: sample-level overlap (modes 1, 2)
+  if (ts.any_pot) {
+    for (int t = threadIdx.x; t < ts.sD[np]; t += blockDim.x) {
+      const int pl = plane_of(ts.sD, np, t);
+      const int u = t - ts.sD[pl];
+      const DevPlane &p = im.planes[pl];
+      const PlaneGeo &g = ts.g[pl];
+      const int nkx = nx * (int)p.bc + 1;
+      const int kx = tmx * (int)p.bc - 1 + u % nkx, ky = tmy * (int)p.br - 1 + u / nkx;
+      const int px = 4 * kx + 4, py = 4 * ky + 4;
+      const bool xs = px % (int)g.mw == 0 && col_boundary(p, im.mbx, (uint32_t)(px / (int)g.mw));
+      const bool ys = py % (int)g.mh == 0 && row_boundary(p, im.mby, (uint32_t)(py / (int)g.mh));
+      overlap_slot(lds + ts.soff[pl], (int)g.srw, g.sox, g.soy, kx, ky, (int)g.pw, (int)g.ph, xs, ys, status);
+    }
+    __syncthreads();
+  }
+
+  // stage E: output groups of 4x2 pixels over the tile, clipped to the crop
+  const int x0 = max(tmx * 16, (int)im.crop_x), x1 = min((tmx + nx) * 16, (int)(im.crop_x + im.crop_w));
+  const int y0 = max(tmy * 16, (int)im.crop_y), y1 = min((tmy + ny) * 16, (int)(im.crop_y + im.crop_h));
+  if (x1 <= x0 || y1 <= y0) return;
+  const int gx0 = x0 >> 2, gx1 = ((x1 - 1) >> 2) + 1;
+  const int gy0 = y0 >> 1, gy1 = ((y1 - 1) >> 1) + 1;
+  const int ngx = gx1 - gx0;
+  OutCtx o{&im, ts.g, ts.soff, lds};
+  for (int t = threadIdx.x; t < ngx * (gy1 - gy0); t += blockDim.x)
+    output_group(o, slot.out, gx0 + t % ngx, gy0 + t / ngx, x0, x1, y0, y1, status);
 }
 
 __global__ void __launch_bounds__(kThreads) jxr_reconstruct_tiles(KernelArgs args) {
-  extern __shared__ int lds[];
+  HIP_DYNAMIC_SHARED(int, lds)
+  __shared__ TileShared ts;
   if (!args.persistent) {
-    process_tile(args.inline_slot, blockIdx.x, args, lds);
+    process_tile(args.inline_slot, blockIdx.x, args, lds, ts);
     return;
   }
   // Persistent: pull work items until the queue is drained; every
@@ -1026,7 +1245,7 @@ __global__ void __launch_bounds__(kThreads) jxr_reconstruct_tiles(KernelArgs arg
     const uint32_t item = next_item;
     if (item >= args.item_count) break;
     const Slot &slot = args.slots[next_slot];
-    process_tile(slot, item - slot.tile_start, args, lds);
+    process_tile(slot, item - slot.tile_start, args, lds, ts);
   }
 }
 

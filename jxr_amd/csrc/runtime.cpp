@@ -565,7 +565,15 @@ struct jxrhip_session {
   double last_ms = 0;
   uint32_t last_launches = 0;
   jxrhip_diagnostics diag{};
+  // status words of in-flight per-image submissions (device + pinned host
+  // mirror) and reusable completion events, so a submission allocates nothing
+  uint32_t *status_dev = nullptr;
+  uint32_t *status_host = nullptr;
+  std::vector<uint32_t> free_status;
+  std::vector<hipEvent_t> free_events;
 };
+
+static constexpr uint32_t kStatusSlots = 16384;
 
 struct jxrhip_image {
   jxrhip_session *session = nullptr;
@@ -577,8 +585,7 @@ struct jxrhip_submission {
   jxrhip_session *session = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  DeviceBuffer status;    // device status word
-  uint32_t *host_status = nullptr;  // pinned
+  uint32_t slot = UINT32_MAX;  // status word index
   DeviceBuffer output;    // owned output (submit), empty for submit_into
   uint64_t output_bytes = 0;
   std::vector<DeviceBuffer> transient;
@@ -616,6 +623,11 @@ extern "C" jxrhip_status jxrhip_session_create(int32_t ordinal, jxrhip_session *
     s->max_lds = 65536;
   }
   HIP_TRY(s->pool.get(256, s->counter, s->diag.scratch_hits, s->diag.scratch_misses), "counter");
+  HIP_TRY(hipMalloc((void **)&s->status_dev, kStatusSlots * sizeof(uint32_t)), "status words");
+  HIP_TRY(hipHostMalloc((void **)&s->status_host, kStatusSlots * sizeof(uint32_t), hipHostMallocDefault),
+          "status words");
+  s->free_status.reserve(kStatusSlots);
+  for (uint32_t i = kStatusSlots; i-- > 0;) s->free_status.push_back(i);
   s->diag.device_ordinal = ordinal;
   s->diag.stream_count = kStreams;
   s->diag.compute_units = s->cus;
@@ -631,6 +643,9 @@ extern "C" void jxrhip_session_destroy(jxrhip_session *s) {
   for (auto &kv : s->coef_cache) s->pool.put(kv.second.buf);
   for (auto &kv : s->desc_cache) s->pool.put(kv.second.buf);
   s->pool.put(s->counter);
+  if (s->status_dev) (void)hipFree(s->status_dev);
+  if (s->status_host) (void)hipHostFree(s->status_host);
+  for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
   for (uint32_t i = 0; i < kStreams; i++)
     if (s->streams[i]) (void)hipStreamDestroy(s->streams[i]);
   if (s->t0) (void)hipEventDestroy(s->t0);
@@ -830,13 +845,12 @@ extern "C" jxrhip_status jxrhip_session_clear_cache(jxrhip_session *s) {
 // Tile shape of a plan set: the largest candidate whose LDS fits the budget.
 static void choose_tile(const jxrhip_session *s, const jxrhip_plan *p, uint32_t &tx, uint32_t &ty,
                         uint32_t &words) {
+  // the kernel keeps a sample region and a low region for every plane
   auto need = [&](uint32_t x, uint32_t y) {
-    uint32_t w = 0, lowmax = 0;
-    for (const PlanPlane &pp : p->planes) {
-      w += sample_region_words(pp.bc, pp.br, x, y);
-      lowmax = std::max(lowmax, low_region_words(pp.bc, pp.br, x, y));
-    }
-    return w + lowmax;
+    uint32_t w = 0;
+    for (const PlanPlane &pp : p->planes)
+      w += sample_region_words(pp.bc, pp.br, x, y) + low_region_words(pp.bc, pp.br, x, y);
+    return w;
   };
   const uint32_t budget = 48 * 1024 / 4;
   if (s->tile_x) {
@@ -873,10 +887,17 @@ static jxrhip_status submit_impl(jxrhip_session *s, const jxrhip_plan *p, void *
   if (!sub) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "out of host memory");
   sub->session = s;
   sub->stream = s->streams[s->next_stream++ % kStreams];
-  HIP_TRY(hipEventCreateWithFlags(&sub->done, hipEventDisableTiming), "hipEventCreate");
-  HIP_TRY(hipHostMalloc((void **)&sub->host_status, sizeof(uint32_t), hipHostMallocDefault), "hipHostMalloc");
-  *sub->host_status = 0;
-  HIP_TRY(s->pool.get(256, sub->status, s->diag.scratch_hits, s->diag.scratch_misses), "status word");
+  if (s->free_status.empty()) return fail(JXRHIP_ERR_BUDGET, "too many submissions in flight");
+  if (!s->free_events.empty()) {
+    sub->done = s->free_events.back();
+    s->free_events.pop_back();
+  } else {
+    HIP_TRY(hipEventCreateWithFlags(&sub->done, hipEventDisableTiming), "hipEventCreate");
+  }
+  sub->slot = s->free_status.back();
+  s->free_status.pop_back();
+  uint32_t *status_word = s->status_dev + sub->slot;
+  s->status_host[sub->slot] = 0;
   if (!dst) {
     HIP_TRY(s->pool.get(p->output_bytes, sub->output, s->diag.scratch_hits, s->diag.scratch_misses), "output");
     dst = sub->output.ptr;
@@ -891,7 +912,7 @@ static jxrhip_status submit_impl(jxrhip_session *s, const jxrhip_plan *p, void *
   KernelArgs args{};
   args.inline_slot.img = dimg;
   args.inline_slot.out = (uint8_t *)dst;
-  args.inline_slot.status = (uint32_t *)sub->status.ptr;
+  args.inline_slot.status = status_word;
   args.inline_slot.tiles_x = (p->mbx + tx - 1) / tx;
   args.inline_slot.tile_start = 0;
   args.persistent = 0;
@@ -899,9 +920,10 @@ static jxrhip_status submit_impl(jxrhip_session *s, const jxrhip_plan *p, void *
   args.tile_y = ty;
   args.lds_words = words;
   const uint32_t tiles = args.inline_slot.tiles_x * ((p->mby + ty - 1) / ty);
-  HIP_TRY(hipMemsetAsync(sub->status.ptr, 0, 4, sub->stream), "status reset");
+  HIP_TRY(hipMemsetAsync(status_word, 0, 4, sub->stream), "status reset");
   HIP_TRY(launch_tiles(args, tiles, sub->stream), "reconstruction launch");
-  HIP_TRY(hipMemcpyAsync(sub->host_status, sub->status.ptr, 4, hipMemcpyDeviceToHost, sub->stream), "status readback");
+  HIP_TRY(hipMemcpyAsync(s->status_host + sub->slot, status_word, 4, hipMemcpyDeviceToHost, sub->stream),
+          "status readback");
   HIP_TRY(hipEventRecord(sub->done, sub->stream), "hipEventRecord");
   s->diag.submissions++;
   s->diag.kernel_launches++;
@@ -927,12 +949,11 @@ static void release_submission(jxrhip_submission *sub) {
   jxrhip_session *s = sub->session;
   std::lock_guard<std::mutex> g(s->lock);
   make_current(s);
-  s->pool.put(sub->status);
   for (auto &b : sub->transient) s->pool.put(b);
   sub->transient.clear();
   if (sub->output.ptr) s->pool.put(sub->output);
-  if (sub->host_status) (void)hipHostFree(sub->host_status);
-  if (sub->done) (void)hipEventDestroy(sub->done);
+  if (sub->slot != UINT32_MAX) s->free_status.push_back(sub->slot);
+  if (sub->done) s->free_events.push_back(sub->done);
   delete sub;
 }
 
@@ -946,7 +967,7 @@ extern "C" jxrhip_status jxrhip_submission_wait(jxrhip_submission *sub, jxrhip_i
     release_submission(sub);
     return hip_fail(e, "hipEventSynchronize");
   }
-  const uint32_t status = *sub->host_status;
+  const uint32_t status = sub->session->status_host[sub->slot];
   if (status != 0) {
     if (stage) *stage = status;
     release_submission(sub);

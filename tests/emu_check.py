@@ -1,0 +1,61 @@
+"""Runs every parity case through the CPU build of the HIP path and compares
+with the oracle (run as a script, so it can start under the AddressSanitizer
+runtime). Exit status 0 when every output matches."""
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, HERE)
+
+import numpy as np  # noqa: E402
+
+import emu_lib  # noqa: E402
+
+
+def main():
+    asan = "--asan" in sys.argv
+    emu_lib.load(asan)
+    import oracle_lib as O
+    from cases import CASES, make
+    from jxr_amd import HipDecodePlan, HipDecoderSession, abi, image_desc
+
+    names = sorted(CASES)
+    only = [a for a in sys.argv[1:] if not a.startswith("--")]
+    if only:
+        names = [n for n in names if n in only]
+    session = HipDecoderSession(0)
+    failures = 0
+    tiles = [(0, 0), (1, 1), (3, 2), (5, 3)] if "--tiles" in sys.argv else [(0, 0)]
+    for tile in tiles:
+        session.set_tile(*tile)
+        for name in names:
+            enc, spec, _ = make(CASES[name])
+            plan = HipDecodePlan.from_encoded(enc, spec)
+            want = O.decode(image_desc(enc, spec))
+            got = session.decode_to_host(plan)
+            if not np.array_equal(got, want):
+                bad = np.flatnonzero(got != want)
+                print(f"MISMATCH {name} tile {tile}: {bad.size} bytes, first {bad[:6]}", flush=True)
+                failures += 1
+    session.set_tile(0, 0)
+    # persistent batch over every case at once
+    encs, plans, wants = [], [], []
+    for name in names:
+        enc, spec, _ = make(CASES[name])
+        encs.append(enc)
+        plans.append(HipDecodePlan.from_encoded(enc, spec))
+        wants.append(O.decode(image_desc(enc, spec)))
+    for mode in (abi.BATCH_PERSISTENT, abi.BATCH_PER_IMAGE):
+        bufs = [session.allocate_destination(p.output_bytes) for p in plans]
+        stages = session.submit_batch(plans, [b.ptr for b in bufs], mode).wait()
+        for name, b, want, st in zip(names, bufs, wants, stages):
+            if st or not np.array_equal(b.to_host(), want):
+                print(f"MISMATCH batch mode {mode} {name} stage {st}", flush=True)
+                failures += 1
+    print(f"emulated parity: {failures} failures over {len(names)} cases", flush=True)
+    sys.exit(1 if failures else 0)
+
+
+if __name__ == "__main__":
+    main()
