@@ -99,6 +99,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-per-image", action="store_true")
     ap.add_argument("--tile", default=None, help="TXxTY macroblocks per work item")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also copy 256 MiB device-to-device (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
@@ -133,6 +135,13 @@ def main():
         f"in {time.perf_counter() - t0:.1f} s")
     pixels = sum(p.output_pixels for p in plans)
     alg = sum(p.algorithmic_bytes for p in plans)
+    if args.calibrate:
+        a, b = session.allocate_destination(256 << 20), session.allocate_destination(256 << 20)
+        a.fill(1)
+        session.copy_device(b.ptr, a.ptr, 256 << 20)
+        log("[bench] calibration: copied 268435456 bytes device to device")
+        a.close()
+        b.close()
 
     batched = time_path(session, plans, ptrs, abi.BATCH_PERSISTENT, args.steps, args.warmup)
     log(f"[bench] batched: median wall {np.median(batched['wall_ms']):.3f} ms, "

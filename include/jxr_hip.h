@@ -333,6 +333,9 @@ jxrhip_status jxrhip_fill_destination(jxrhip_session *session, void *device_ptr,
                                       uint64_t bytes);
 jxrhip_status jxrhip_copy_to_host(jxrhip_session *session, const void *device_ptr, void *host,
                                   uint64_t bytes);
+/* Device-to-device copy of `bytes` between destinations, on the session's
+ * first stream (waits for completion). */
+jxrhip_status jxrhip_copy_device(jxrhip_session *session, void *dst, const void *src, uint64_t bytes);
 /* Waits for all work queued on the session's streams. */
 jxrhip_status jxrhip_session_synchronize(jxrhip_session *session);
 

@@ -238,6 +238,7 @@ SIGNATURES = {
     "jxrhip_fill_destination": (st, [vp, vp, C.c_uint8, C.c_uint64]),
     "jxrhip_copy_to_host": (st, [vp, vp, vp, C.c_uint64]),
     "jxrhip_session_synchronize": (st, [vp]),
+    "jxrhip_copy_device": (st, [vp, vp, vp, C.c_uint64]),
     "jxrhip_image_device_ptr": (vp, [vp]),
     "jxrhip_image_bytes": (C.c_uint64, [vp]),
     "jxrhip_image_destroy": (None, [vp]),

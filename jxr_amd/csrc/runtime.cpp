@@ -1025,6 +1025,13 @@ extern "C" jxrhip_status jxrhip_copy_to_host(jxrhip_session *s, const void *ptr,
   HIP_TRY(hipMemcpy(host, ptr, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
   return JXRHIP_OK;
 }
+extern "C" jxrhip_status jxrhip_copy_device(jxrhip_session *s, void *dst, const void *src, uint64_t bytes) {
+  if (!s || !dst || !src) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  make_current(s);
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s->streams[0]), "hipMemcpyAsync");
+  HIP_TRY(hipStreamSynchronize(s->streams[0]), "hipStreamSynchronize");
+  return JXRHIP_OK;
+}
 extern "C" jxrhip_status jxrhip_session_synchronize(jxrhip_session *s) {
   if (!s) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null session");
   make_current(s);

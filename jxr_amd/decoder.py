@@ -462,6 +462,9 @@ class HipDecoderSession:
     def synchronize(self):
         check(load().jxrhip_session_synchronize(self._h), "synchronize")
 
+    def copy_device(self, dst: int, src: int, nbytes: int):
+        check(load().jxrhip_copy_device(self._h, C.c_void_p(dst), C.c_void_p(src), nbytes), "copy_device")
+
     def last_batch_timing(self):
         ms = C.c_double()
         launches = C.c_uint32()
