@@ -39,7 +39,43 @@
 #include "../../include/jxr_hip.h"
 #include "recon_abi.h"
 
+// Global-memory pointers are qualified so that loads and stores through
+// pointers read from the LDS copy of the image descriptor compile to global_*
+// instructions rather than flat ones (the CPU test build defines it empty).
+#ifndef JXR_GLOBAL
+#define JXR_GLOBAL __attribute__((address_space(1)))
+#endif
+
 namespace jxr {
+
+template <class T>
+__device__ __forceinline__ const JXR_GLOBAL T *gptr(const T *p) {
+  return (const JXR_GLOBAL T *)p;
+}
+template <class T>
+__device__ __forceinline__ JXR_GLOBAL T *gptr_mut(T *p) {
+  return (JXR_GLOBAL T *)p;
+}
+// Plain vector types (no constructors), so they load and store through
+// qualified pointers.
+typedef int v4i __attribute__((vector_size(16)));
+typedef uint32_t v4u __attribute__((vector_size(16)));
+typedef uint32_t v2u __attribute__((vector_size(8)));
+
+__device__ __forceinline__ MbRec load_mb(const MbRec *mbs, uint32_t i) {
+  const JXR_GLOBAL v4u *q = (const JXR_GLOBAL v4u *)gptr(mbs + i);
+  const v4u a = q[0], b = q[1];
+  MbRec r;
+  r.coef_off = a[0];
+  r.qdc = a[1];
+  r.qlp = a[2];
+  r.qhp = a[3];
+  r.bands = b[0];
+  r.pred = b[1];
+  r.pad0 = b[2];
+  r.pad1 = b[3];
+  return r;
+}
 
 // ---- arithmetic policies ------------------------------------------------
 struct Fast {
@@ -299,7 +335,7 @@ __device__ __forceinline__ void inv_overlap2x2(int &v0, int &v1, int &v2, int &v
   }())
 
 __device__ __forceinline__ void fail(uint32_t *status, uint32_t code) {
-  atomicCAS(status, 0u, code);
+  atomicCAS((uint32_t *)gptr_mut(status), 0u, code);
 }
 
 // dequantize with a 64-bit product checked against i32 (quantization.rs:46-49)
@@ -309,8 +345,23 @@ __device__ __forceinline__ int dequant_checked(int v, uint32_t step, bool &ovf) 
   return (int)p;
 }
 
+// ---- work-item indexing ----------------------------------------------------
+// Quotient by a runtime divisor d as a multiply-high with m = ceil(2^32 / d)
+// (m = 0 encodes d = 1). Exact for n * d < 2^32, far above any work-item
+// count of a tile; gfx950 has no integer divide instruction.
+__device__ __forceinline__ uint32_t div_magic(uint32_t d) { return 0xFFFFFFFFu / d + 1u; }
+__device__ __forceinline__ uint32_t div_by(uint32_t n, uint32_t m) { return m ? __umulhi(n, m) : n; }
+struct DivMod {
+  int q, r;
+};
+__device__ __forceinline__ DivMod divmod(int n, uint32_t d, uint32_t m) {
+  const int q = (int)div_by((uint32_t)n, m);
+  return DivMod{q, n - q * (int)d};
+}
+
 // ---- tile geometry ---------------------------------------------------------
 struct PlaneGeo {
+  uint32_t bcs, brs;           // log2 of bc, br
   uint32_t bc, br, mw, mh;     // blocks and samples per macroblock
   uint32_t pw, ph;             // plane window in samples
   int sox, soy;                // sample-region origin (window coords)
@@ -321,7 +372,7 @@ struct PlaneGeo {
 };
 
 __device__ __forceinline__ bool mask_bit(const uint32_t *m, uint32_t i) {
-  return m && ((m[i >> 5] >> (i & 31)) & 1u);
+  return m && ((gptr(m)[i >> 5] >> (i & 31)) & 1u);
 }
 __device__ __forceinline__ bool col_boundary(const DevPlane &p, uint32_t mbx, uint32_t m) {
   return m == 0 || m == mbx || mask_bit(p.colmask, m);
@@ -333,10 +384,10 @@ __device__ __forceinline__ bool row_boundary(const DevPlane &p, uint32_t mby, ui
 // ---- stage bodies, one work item each ---------------------------------------
 // Dequantize DC/LP and run the first-level inverse transform of macroblock
 // (mx, my), written to the plane's low region.
-__device__ void first_level_mb(const DevImage &im, const DevPlane &p, const PlaneGeo &g, int *low,
+__device__ __forceinline__ void first_level_mb(const DevImage &im, const DevPlane &p, const PlaneGeo &g, int *low,
                                int mx, int my, uint32_t *status) {
-  const MbRec r = p.mbs[(uint32_t)my * im.mbx + (uint32_t)mx];
-  const int *c = p.coef + r.coef_off;
+  const MbRec r = load_mb(p.mbs, (uint32_t)my * im.mbx + (uint32_t)mx);
+  const JXR_GLOBAL int *c = gptr(p.coef) + r.coef_off;
   const bool full = p.bc == 4;
   const uint32_t blocks = p.bc * p.br;
   int v[16];
@@ -365,7 +416,7 @@ __device__ void first_level_mb(const DevImage &im, const DevPlane &p, const Plan
     }
   }
   if (ovf) fail(status, JXRHIP_STAGE_FIRST_TRANSFORM);
-  int *o = low + (my * (int)p.br - g.loy) * (int)g.lrw + (mx * (int)p.bc - g.lox);
+  int *o = low + ((my << g.brs) - g.loy) * (int)g.lrw + ((mx << g.bcs) - g.lox);
   const int rw = (int)g.lrw;
   if (full) {
 #pragma unroll
@@ -389,7 +440,7 @@ __device__ void first_level_mb(const DevImage &im, const DevPlane &p, const Plan
 // 4-tap edge strips across the split; slots split along both are four 2x2
 // corners (reconstruct/overlap.rs:124-323; device_plan/overlap.rs:125-178).
 // Only positions inside [0, width) x [0, height) are touched.
-__device__ void overlap_slot(int *s, int rw, int ox, int oy, int kx, int ky, int width, int height,
+__device__ __forceinline__ void overlap_slot(int *s, int rw, int ox, int oy, int kx, int ky, int width, int height,
                              bool sx, bool sy, uint32_t *status) {
   const int x0 = 4 * kx + 2, y0 = 4 * ky + 2;
   int *base = s + (y0 - oy) * rw + (x0 - ox);
@@ -471,13 +522,13 @@ __device__ void overlap_slot(int *s, int rw, int ox, int oy, int kx, int ky, int
 // the corner residual steps before and after the pairs handle.
 
 // Nearest region boundaries (MB columns) at or left of m, and right of m.
-__device__ int boundary_at_or_left(const DevPlane &p, int m) {
+__device__ __forceinline__ int boundary_at_or_left(const DevPlane &p, int m) {
   if (!p.colmask) return 0;
   for (int b = m; b > 0; b--)
     if (mask_bit(p.colmask, (uint32_t)b)) return b;
   return 0;
 }
-__device__ int boundary_right(const DevPlane &p, uint32_t mbx, int m) {
+__device__ __forceinline__ int boundary_right(const DevPlane &p, uint32_t mbx, int m) {
   if (!p.colmask) return (int)mbx;
   for (int b = m + 1; b < (int)mbx; b++)
     if (mask_bit(p.colmask, (uint32_t)b)) return b;
@@ -488,7 +539,7 @@ struct CornerRange {
   int mx0, mx1, my0, my1;
   __device__ int count() const { return (mx1 - mx0) * (my1 - my0) * 2; }
 };
-__device__ CornerRange corner_range(const DevImage &im, const DevPlane &p, int tmx, int tmy, int nx, int ny) {
+__device__ __forceinline__ CornerRange corner_range(const DevImage &im, const DevPlane &p, int tmx, int tmy, int nx, int ny) {
   const int h = (int)low_halo(p.bc);
   return CornerRange{max(tmx - h, 0), min(tmx + nx + h, (int)im.mbx), max(tmy - h, 0),
                      min(tmy + ny + h, (int)im.mby)};
@@ -497,10 +548,9 @@ __device__ CornerRange corner_range(const DevImage &im, const DevPlane &p, int t
 // One (region, boundary row) of the low region: that row's two corners in the
 // reference's order (left, then right); in a region one macroblock wide each
 // corner is the other's neighbour.
-__device__ void subsampled_corner_item(const DevImage &im, const DevPlane &p, const PlaneGeo &g, int *low,
-                                       const CornerRange &cr, int t, bool add, uint32_t *status) {
-  const int ncols = cr.mx1 - cr.mx0;
-  const int m = cr.mx0 + t % ncols, ri = t / ncols;
+__device__ __forceinline__ void subsampled_corner_item(const DevImage &im, const DevPlane &p, const PlaneGeo &g, int *low,
+                                       const CornerRange &cr, DivMod t, bool add, uint32_t *status) {
+  const int m = cr.mx0 + t.r, ri = t.q;
   const int L = boundary_at_or_left(p, m);
   if (m != max(L, cr.mx0)) return;  // the region's first held column owns it
   const int R = boundary_right(p, im.mbx, m);
@@ -508,10 +558,10 @@ __device__ void subsampled_corner_item(const DevImage &im, const DevPlane &p, co
   int row;
   if ((ri & 1) == 0) {
     if (!row_boundary(p, im.mby, (uint32_t)my)) return;
-    row = my * (int)p.br;
+    row = my << g.brs;
   } else {
     if (!row_boundary(p, im.mby, (uint32_t)(my + 1))) return;
-    row = (my + 1) * (int)p.br - 1;
+    row = ((my + 1) << g.brs) - 1;
   }
   int *base = low + (row - g.loy) * (int)g.lrw;
   const int cl = L * 2, crr = R * 2 - 1;          // corner columns (low coords)
@@ -531,14 +581,13 @@ __device__ void subsampled_corner_item(const DevImage &im, const DevPlane &p, co
 __device__ __forceinline__ int cell_columns(int nx) { return nx + 3; }
 __device__ __forceinline__ int cell_rows(const DevPlane &p, int ny) { return ((int)p.br * ny) / 2 + 1; }
 
-__device__ void subsampled_cell_item(const DevImage &im, const DevPlane &p, const PlaneGeo &g, int *low,
-                                     int tmx, int tmy, int nx, int t, uint32_t *status) {
-  const int ncx = cell_columns(nx);
-  const int x = 2 * (tmx - 1) + 2 * (t % ncx), y = (int)p.br * tmy + 2 * (t / ncx);
+__device__ __forceinline__ void subsampled_cell_item(const DevImage &im, const DevPlane &p, const PlaneGeo &g, int *low,
+                                     int tmx, int tmy, DivMod t, uint32_t *status) {
+  const int x = 2 * (tmx - 1) + 2 * t.r, y = (tmy << g.brs) + 2 * t.q;
   const int W = (int)g.lw, H = (int)g.lh;
   if (x < 0 || x > W || y < 0 || y > H) return;
-  const bool sx = col_boundary(p, im.mbx, (uint32_t)(x / 2));
-  const bool sy = (y % (int)p.br == 0) && row_boundary(p, im.mby, (uint32_t)(y / (int)p.br));
+  const bool sx = col_boundary(p, im.mbx, (uint32_t)(x >> 1));
+  const bool sy = (y & ((int)g.br - 1)) == 0 && row_boundary(p, im.mby, (uint32_t)(y >> g.brs));
   const int rw = (int)g.lrw;
   int *q = low + (y - 1 - g.loy) * rw + (x - 1 - g.lox);
   bool ovf = false;
@@ -582,43 +631,42 @@ struct BlockRange {
   int bx0, bx1, by0, by1;
   __device__ int count() const { return (bx1 - bx0) * (by1 - by0); }
 };
-__device__ BlockRange block_range(const DevPlane &p, const PlaneGeo &g, int tmx, int tmy, int nx, int ny) {
+__device__ __forceinline__ BlockRange block_range(const DevPlane &p, const PlaneGeo &g, int tmx, int tmy, int nx, int ny) {
   return BlockRange{max(tmx * (int)p.bc - 1, 0), min((tmx + nx) * (int)p.bc + 1, (int)g.lw),
                     max(tmy * (int)p.br - 1, 0), min((tmy + ny) * (int)p.br + 1, (int)g.lh)};
 }
 
-__device__ void second_level_block(const DevImage &im, const DevPlane &p, const PlaneGeo &g, const int *low,
-                                   int *smp, const BlockRange &br, int t, uint32_t *status) {
-  const int nbx = br.bx1 - br.bx0;
-  const int bx = br.bx0 + t % nbx, by = br.by0 + t / nbx;
-  const int mx = bx / (int)p.bc, my = by / (int)p.br;
-  const int cc = bx - mx * (int)p.bc, rr = by - my * (int)p.br;
-  const MbRec r = p.mbs[(uint32_t)my * im.mbx + (uint32_t)mx];
+__device__ __forceinline__ void second_level_block(const DevImage &im, const DevPlane &p, const PlaneGeo &g, const int *low,
+                                   int *smp, const BlockRange &br, DivMod t, uint32_t *status) {
+  const int bx = br.bx0 + t.r, by = br.by0 + t.q;
+  const int mx = bx >> g.bcs, my = by >> g.brs;
+  const int cc = bx & ((int)g.bc - 1), rr = by & ((int)g.br - 1);
+  const MbRec r = load_mb(p.mbs, (uint32_t)my * im.mbx + (uint32_t)mx);
   int v[16];
 #pragma unroll
   for (int k = 0; k < 16; k++) v[k] = 0;
   bool povf = false, ovf = false;
   if (r.bands >= JXRHIP_BANDS_NO_FLEXBITS) {
-    const int *c = p.coef + r.coef_off;
-    const int k = rr * (int)p.bc + cc;
-    const int *blk = c + k * 16;
+    const JXR_GLOBAL int *c = gptr(p.coef) + r.coef_off;
+    const int k = (rr << g.bcs) + cc;
+    const JXR_GLOBAL int *blk = c + k * 16;
     if ((r.coef_off & 3u) == 0) {
-      const int4 *b4 = reinterpret_cast<const int4 *>(blk);
-      const int4 q0 = b4[0], q1 = b4[1], q2 = b4[2], q3 = b4[3];
-      v[1] = q0.y; v[2] = q0.z; v[3] = q0.w;
-      v[4] = q1.x; v[5] = q1.y; v[6] = q1.z; v[7] = q1.w;
-      v[8] = q2.x; v[9] = q2.y; v[10] = q2.z; v[11] = q2.w;
-      v[12] = q3.x; v[13] = q3.y; v[14] = q3.z; v[15] = q3.w;
+      const JXR_GLOBAL v4i *b4 = (const JXR_GLOBAL v4i *)blk;
+      const v4i q0 = b4[0], q1 = b4[1], q2 = b4[2], q3 = b4[3];
+      v[1] = q0[1]; v[2] = q0[2]; v[3] = q0[3];
+      v[4] = q1[0]; v[5] = q1[1]; v[6] = q1[2]; v[7] = q1[3];
+      v[8] = q2[0]; v[9] = q2[1]; v[10] = q2[2]; v[11] = q2[3];
+      v[12] = q3[0]; v[13] = q3[1]; v[14] = q3[2]; v[15] = q3[3];
     } else {
 #pragma unroll
       for (int j = 1; j < 16; j++) v[j] = blk[j];
     }
     // running sums of the residuals (prediction.rs:53-97)
     if (r.pred == JXRHIP_PRED_FROM_LEFT && cc > 0) {
-      const int *row0 = c + rr * (int)p.bc * 16;
+      const JXR_GLOBAL int *row0 = c + (rr << (g.bcs + 4));
       int s4 = row0[4], s8 = row0[8], s12 = row0[12];
       for (int j = 1; j < cc; j++) {
-        const int *b = row0 + j * 16;
+        const JXR_GLOBAL int *b = row0 + j * 16;
         povf |= __builtin_add_overflow(b[4], s4, &s4);
         povf |= __builtin_add_overflow(b[8], s8, &s8);
         povf |= __builtin_add_overflow(b[12], s12, &s12);
@@ -627,11 +675,11 @@ __device__ void second_level_block(const DevImage &im, const DevPlane &p, const 
       povf |= __builtin_add_overflow(v[8], s8, &v[8]);
       povf |= __builtin_add_overflow(v[12], s12, &v[12]);
     } else if (r.pred == JXRHIP_PRED_FROM_TOP && rr > 0) {
-      const int *col0 = c + cc * 16;
-      const int st = (int)p.bc * 16;
+      const JXR_GLOBAL int *col0 = c + cc * 16;
+      const int st = 16 << g.bcs;
       int s1 = col0[1], s2 = col0[2], s3 = col0[3];
       for (int j = 1; j < rr; j++) {
-        const int *b = col0 + j * st;
+        const JXR_GLOBAL int *b = col0 + j * st;
         povf |= __builtin_add_overflow(b[1], s1, &s1);
         povf |= __builtin_add_overflow(b[2], s2, &s2);
         povf |= __builtin_add_overflow(b[3], s3, &s3);
@@ -780,13 +828,13 @@ __device__ __forceinline__ bool pm_signed(int v, int a, int m, int &out) {
   return true;
 }
 
-__device__ __forceinline__ void store_sample(uint8_t *dst, uint32_t value, uint32_t bytes) {
+__device__ __forceinline__ void store_sample(JXR_GLOBAL uint8_t *dst, uint32_t value, uint32_t bytes) {
   if (bytes == 1)
     *dst = (uint8_t)value;
   else if (bytes == 2)
-    *reinterpret_cast<uint16_t *>(dst) = (uint16_t)value;
+    *reinterpret_cast<JXR_GLOBAL uint16_t *>(dst) = (uint16_t)value;
   else
-    *reinterpret_cast<uint32_t *>(dst) = value;
+    *reinterpret_cast<JXR_GLOBAL uint32_t *>(dst) = value;
 }
 
 // Converted, ordered and formatted samples of one pixel (packing.rs:333-357,
@@ -919,18 +967,19 @@ __device__ __forceinline__ bool format_pixel(const DevImage &im, const int in[4]
 // Four consecutive samples of one channel, stored at `dst` (the first
 // pixel's address); one vector store when all four are valid and the
 // address is aligned to the run, one store per valid sample otherwise.
-__device__ __forceinline__ void store_run(uint8_t *dst, const uint32_t s[4], uint32_t valid, uint32_t bytes,
-                                          size_t step) {
+__device__ __forceinline__ void store_run(JXR_GLOBAL uint8_t *dst, const uint32_t s[4], uint32_t valid,
+                                          uint32_t bytes, size_t step) {
   const uintptr_t addr = reinterpret_cast<uintptr_t>(dst);
   if (valid == 0xFu && step == bytes && (addr & (4 * bytes - 1)) == 0) {
     if (bytes == 1) {
-      *reinterpret_cast<uint32_t *>(dst) =
+      *reinterpret_cast<JXR_GLOBAL uint32_t *>(dst) =
           (s[0] & 0xFFu) | ((s[1] & 0xFFu) << 8) | ((s[2] & 0xFFu) << 16) | ((s[3] & 0xFFu) << 24);
     } else if (bytes == 2) {
-      *reinterpret_cast<uint2 *>(dst) =
-          make_uint2((s[0] & 0xFFFFu) | (s[1] << 16), (s[2] & 0xFFFFu) | (s[3] << 16));
+      const v2u w = {(s[0] & 0xFFFFu) | (s[1] << 16), (s[2] & 0xFFFFu) | (s[3] << 16)};
+      *reinterpret_cast<JXR_GLOBAL v2u *>(dst) = w;
     } else {
-      *reinterpret_cast<uint4 *>(dst) = make_uint4(s[0], s[1], s[2], s[3]);
+      const v4u w = {s[0], s[1], s[2], s[3]};
+      *reinterpret_cast<JXR_GLOBAL v4u *>(dst) = w;
     }
     return;
   }
@@ -942,7 +991,7 @@ __device__ __forceinline__ void store_run(uint8_t *dst, const uint32_t s[4], uin
 // One output group: pixels x = 4gx..4gx+3, y = 2gy..2gy+1 (window coords)
 // inside [x0, x1) x [y0, y1). 4:2:0 chroma is interpolated once for the
 // group: three chroma rows and four chroma columns give both rows' values.
-__device__ void output_group(const OutCtx &o, uint8_t *out, int gx, int gy, int x0, int x1, int y0, int y1,
+__device__ __forceinline__ void output_group(const OutCtx &o, JXR_GLOBAL uint8_t *out, int gx, int gy, int x0, int x1, int y0, int y1,
                              uint32_t *status) {
   const DevImage &im = *o.im;
   const int xb = 4 * gx, yb = 2 * gy;
@@ -1022,11 +1071,11 @@ __device__ void output_group(const OutCtx &o, uint8_t *out, int gx, int gy, int 
       for (uint32_t c = 0; c < 6; c++) {
         if (c >= im.channels) continue;
         const uint32_t pl = im.planar ? c : 0;
-        uint8_t *row = out + im.plane_offset[pl] + (uint64_t)oy * im.row_stride[pl];
+        JXR_GLOBAL uint8_t *row = out + im.plane_offset[pl] + (uint64_t)oy * im.row_stride[pl];
         store_run(row + (int64_t)ox0 * (int64_t)im.bytes, st[c], valid, im.bytes, im.bytes);
       }
     } else {
-      uint8_t *row = out + im.plane_offset[0] + (uint64_t)oy * im.row_stride[0];
+      JXR_GLOBAL uint8_t *row = out + im.plane_offset[0] + (uint64_t)oy * im.row_stride[0];
       const size_t pitch = (size_t)im.channels * im.bytes;
 #pragma unroll
       for (uint32_t c = 0; c < 6; c++) {
@@ -1043,11 +1092,19 @@ __device__ void output_group(const OutCtx &o, uint8_t *out, int gx, int gy, int 
 // descriptor, the per-plane region geometry and the per-stage work counts.
 // Each stage runs its work items for every plane in one flattened loop over
 // the workgroup, so a barrier separates stages, not planes.
+struct PlaneWork {
+  uint32_t wA, mA;   // stage A: macroblocks per row of the low window
+  uint32_t wB, mB;   // stage B1/B3: slots (full) or corner columns (subsampled)
+  uint32_t wB2, mB2; // stage B2: cell columns
+  uint32_t wC, mC;   // stage C: blocks per row
+  uint32_t wD, mD;   // stage D: overlap slots per row
+};
 struct TileShared {
   DevImage im;
   PlaneGeo g[kMaxPlanes];
   BlockRange br[kMaxPlanes];
   CornerRange cr[kMaxPlanes];
+  PlaneWork w[kMaxPlanes];
   int soff[kMaxPlanes];  // sample region of each plane, in dynamic LDS words
   int loff[kMaxPlanes];  // low region of each plane
   int sA[kMaxPlanes + 1], sB1[kMaxPlanes + 1], sB2[kMaxPlanes + 1], sB3[kMaxPlanes + 1];
@@ -1062,10 +1119,13 @@ __device__ __forceinline__ int plane_of(const int *start, int np, int t) {
   return pl;
 }
 
+__device__ __forceinline__ uint32_t log2_small(uint32_t v) { return v == 4 ? 2u : v == 2 ? 1u : 0u; }
+
 // Thread 0 lays out the tile: regions, ranges and work counts.
-__device__ void tile_setup(TileShared &ts, uint32_t tile, uint32_t tiles_x, int TX, int TY) {
+__device__ __forceinline__ void tile_setup(TileShared &ts, uint32_t tile, uint32_t tiles_x, int TX, int TY) {
   const DevImage &im = ts.im;
-  const int tmx = (int)(tile % tiles_x) * TX, tmy = (int)(tile / tiles_x) * TY;
+  const uint32_t ty_idx = tile / tiles_x;
+  const int tmx = (int)(tile - ty_idx * tiles_x) * TX, tmy = (int)ty_idx * TY;
   const int nx = min(TX, (int)im.mbx - tmx), ny = min(TY, (int)im.mby - tmy);
   const int np = (int)im.plane_count;
   ts.tmx = tmx;
@@ -1079,6 +1139,8 @@ __device__ void tile_setup(TileShared &ts, uint32_t tile, uint32_t tiles_x, int 
     PlaneGeo &g = ts.g[pl];
     g.bc = p.bc;
     g.br = p.br;
+    g.bcs = log2_small(p.bc);
+    g.brs = log2_small(p.br);
     g.mw = p.bc * 4;
     g.mh = p.br * 4;
     g.pw = im.mbx * g.mw;
@@ -1112,8 +1174,19 @@ __device__ void tile_setup(TileShared &ts, uint32_t tile, uint32_t tiles_x, int 
   ts.sA[0] = ts.sB1[0] = ts.sB2[0] = ts.sB3[0] = ts.sC[0] = ts.sD[0] = 0;
   for (int pl = 0; pl < np; pl++) {
     const DevPlane &p = im.planes[pl];
+    PlaneWork &w = ts.w[pl];
     const int h = (int)low_halo(p.bc);
     const bool lp = p.overlap == 2, sub = lp && p.bc == 2;
+    w.wA = (uint32_t)(nx + 2 * h);
+    w.wB = p.bc == 4 ? (uint32_t)(nx + 1) : (uint32_t)(ts.cr[pl].mx1 - ts.cr[pl].mx0);
+    w.wB2 = (uint32_t)cell_columns(nx);
+    w.wC = (uint32_t)(ts.br[pl].bx1 - ts.br[pl].bx0);
+    w.wD = (uint32_t)nx * p.bc + 1;
+    w.mA = div_magic(w.wA);
+    w.mB = div_magic(w.wB);
+    w.mB2 = div_magic(w.wB2);
+    w.mC = div_magic(w.wC);
+    w.mD = div_magic(w.wD);
     ts.sA[pl + 1] = ts.sA[pl] + (nx + 2 * h) * (ny + 2 * h);
     ts.sB1[pl + 1] = ts.sB1[pl] + (!lp ? 0 : p.bc == 4 ? (nx + 1) * (ny + 1) : ts.cr[pl].count());
     ts.sB2[pl + 1] = ts.sB2[pl] + (sub ? cell_columns(nx) * cell_rows(p, ny) : 0);
@@ -1123,11 +1196,12 @@ __device__ void tile_setup(TileShared &ts, uint32_t tile, uint32_t tiles_x, int 
   }
 }
 
-__device__ void process_tile(const Slot &slot, uint32_t tile, const KernelArgs &args, int *lds, TileShared &ts) {
+__device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, const KernelArgs &args, int *lds,
+                                             TileShared &ts) {
   {
-    const uint4 *src = reinterpret_cast<const uint4 *>(slot.img);
-    uint4 *dst = reinterpret_cast<uint4 *>(&ts.im);
-    for (int i = threadIdx.x; i < (int)(sizeof(DevImage) / sizeof(uint4)); i += blockDim.x) dst[i] = src[i];
+    const JXR_GLOBAL v4u *src = (const JXR_GLOBAL v4u *)gptr(slot.img);
+    v4u *dst = reinterpret_cast<v4u *>(&ts.im);
+    for (int i = threadIdx.x; i < (int)(sizeof(DevImage) / sizeof(v4u)); i += blockDim.x) dst[i] = src[i];
   }
   __syncthreads();
   if (threadIdx.x == 0) tile_setup(ts, tile, slot.tiles_x, (int)args.tile_x, (int)args.tile_y);
@@ -1140,11 +1214,10 @@ __device__ void process_tile(const Slot &slot, uint32_t tile, const KernelArgs &
   // stage A: first level of the tile's macroblocks and their halo
   for (int t = threadIdx.x; t < ts.sA[np]; t += blockDim.x) {
     const int pl = plane_of(ts.sA, np, t);
-    const int u = t - ts.sA[pl];
     const DevPlane &p = im.planes[pl];
     const int h = (int)low_halo(p.bc);
-    const int ax = nx + 2 * h;
-    const int mx = tmx - h + u % ax, my = tmy - h + u / ax;
+    const DivMod u = divmod(t - ts.sA[pl], ts.w[pl].wA, ts.w[pl].mA);
+    const int mx = tmx - h + u.r, my = tmy - h + u.q;
     if (mx < 0 || my < 0 || mx >= (int)im.mbx || my >= (int)im.mby) continue;
     first_level_mb(im, p, ts.g[pl], lds + ts.loff[pl], mx, my, status);
   }
@@ -1154,11 +1227,11 @@ __device__ void process_tile(const Slot &slot, uint32_t tile, const KernelArgs &
   if (ts.any_lp) {
     for (int t = threadIdx.x; t < ts.sB1[np]; t += blockDim.x) {
       const int pl = plane_of(ts.sB1, np, t);
-      const int u = t - ts.sB1[pl];
       const DevPlane &p = im.planes[pl];
       const PlaneGeo &g = ts.g[pl];
+      const DivMod u = divmod(t - ts.sB1[pl], ts.w[pl].wB, ts.w[pl].mB);
       if (p.bc == 4) {
-        const int kx = tmx - 1 + u % (nx + 1), ky = tmy - 1 + u / (nx + 1);
+        const int kx = tmx - 1 + u.r, ky = tmy - 1 + u.q;
         overlap_slot(lds + ts.loff[pl], (int)g.lrw, g.lox, g.loy, kx, ky, (int)g.lw, (int)g.lh,
                      col_boundary(p, im.mbx, (uint32_t)(kx + 1)), row_boundary(p, im.mby, (uint32_t)(ky + 1)),
                      status);
@@ -1170,13 +1243,14 @@ __device__ void process_tile(const Slot &slot, uint32_t tile, const KernelArgs &
       __syncthreads();
       for (int t = threadIdx.x; t < ts.sB2[np]; t += blockDim.x) {
         const int pl = plane_of(ts.sB2, np, t);
-        subsampled_cell_item(im, im.planes[pl], ts.g[pl], lds + ts.loff[pl], tmx, tmy, nx, t - ts.sB2[pl], status);
+        const DivMod u = divmod(t - ts.sB2[pl], ts.w[pl].wB2, ts.w[pl].mB2);
+        subsampled_cell_item(im, im.planes[pl], ts.g[pl], lds + ts.loff[pl], tmx, tmy, u, status);
       }
       __syncthreads();
       for (int t = threadIdx.x; t < ts.sB3[np]; t += blockDim.x) {
         const int pl = plane_of(ts.sB3, np, t);
-        subsampled_corner_item(im, im.planes[pl], ts.g[pl], lds + ts.loff[pl], ts.cr[pl], t - ts.sB3[pl], true,
-                               status);
+        const DivMod u = divmod(t - ts.sB3[pl], ts.w[pl].wB, ts.w[pl].mB);
+        subsampled_corner_item(im, im.planes[pl], ts.g[pl], lds + ts.loff[pl], ts.cr[pl], u, true, status);
       }
     }
     __syncthreads();
@@ -1185,8 +1259,8 @@ __device__ void process_tile(const Slot &slot, uint32_t tile, const KernelArgs &
   // stage C: high-pass prediction and second level, tile blocks + halo
   for (int t = threadIdx.x; t < ts.sC[np]; t += blockDim.x) {
     const int pl = plane_of(ts.sC, np, t);
-    second_level_block(im, im.planes[pl], ts.g[pl], lds + ts.loff[pl], lds + ts.soff[pl], ts.br[pl],
-                       t - ts.sC[pl], status);
+    const DivMod u = divmod(t - ts.sC[pl], ts.w[pl].wC, ts.w[pl].mC);
+    second_level_block(im, im.planes[pl], ts.g[pl], lds + ts.loff[pl], lds + ts.soff[pl], ts.br[pl], u, status);
   }
   __syncthreads();
 
@@ -1194,14 +1268,13 @@ __device__ void process_tile(const Slot &slot, uint32_t tile, const KernelArgs &
   if (ts.any_pot) {
     for (int t = threadIdx.x; t < ts.sD[np]; t += blockDim.x) {
       const int pl = plane_of(ts.sD, np, t);
-      const int u = t - ts.sD[pl];
       const DevPlane &p = im.planes[pl];
       const PlaneGeo &g = ts.g[pl];
-      const int nkx = nx * (int)p.bc + 1;
-      const int kx = tmx * (int)p.bc - 1 + u % nkx, ky = tmy * (int)p.br - 1 + u / nkx;
+      const DivMod u = divmod(t - ts.sD[pl], ts.w[pl].wD, ts.w[pl].mD);
+      const int kx = (tmx << g.bcs) - 1 + u.r, ky = (tmy << g.brs) - 1 + u.q;
       const int px = 4 * kx + 4, py = 4 * ky + 4;
-      const bool xs = px % (int)g.mw == 0 && col_boundary(p, im.mbx, (uint32_t)(px / (int)g.mw));
-      const bool ys = py % (int)g.mh == 0 && row_boundary(p, im.mby, (uint32_t)(py / (int)g.mh));
+      const bool xs = (px & ((int)g.mw - 1)) == 0 && col_boundary(p, im.mbx, (uint32_t)(px >> (g.bcs + 2)));
+      const bool ys = (py & ((int)g.mh - 1)) == 0 && row_boundary(p, im.mby, (uint32_t)(py >> (g.brs + 2)));
       overlap_slot(lds + ts.soff[pl], (int)g.srw, g.sox, g.soy, kx, ky, (int)g.pw, (int)g.ph, xs, ys, status);
     }
     __syncthreads();
@@ -1214,29 +1287,35 @@ __device__ void process_tile(const Slot &slot, uint32_t tile, const KernelArgs &
   const int gx0 = x0 >> 2, gx1 = ((x1 - 1) >> 2) + 1;
   const int gy0 = y0 >> 1, gy1 = ((y1 - 1) >> 1) + 1;
   const int ngx = gx1 - gx0;
+  const uint32_t mE = div_magic((uint32_t)ngx);
   OutCtx o{&im, ts.g, ts.soff, lds};
-  for (int t = threadIdx.x; t < ngx * (gy1 - gy0); t += blockDim.x)
-    output_group(o, slot.out, gx0 + t % ngx, gy0 + t / ngx, x0, x1, y0, y1, status);
+  for (int t = threadIdx.x; t < ngx * (gy1 - gy0); t += blockDim.x) {
+    const DivMod u = divmod(t, (uint32_t)ngx, mE);
+    output_group(o, gptr_mut(slot.out), gx0 + u.r, gy0 + u.q, x0, x1, y0, y1, status);
+  }
 }
 
+// Per-image launches take tile blockIdx.x of the inline slot; persistent
+// launches pull work items until the queue is drained, and every workgroup
+// reaches the exit once the counter passes item_count. One call site keeps
+// process_tile inlined once, so its LDS pointers stay LDS-addressed.
 __global__ void __launch_bounds__(kThreads) jxr_reconstruct_tiles(KernelArgs args) {
   HIP_DYNAMIC_SHARED(int, lds)
   __shared__ TileShared ts;
-  if (!args.persistent) {
-    process_tile(args.inline_slot, blockIdx.x, args, lds, ts);
-    return;
-  }
-  // Persistent: pull work items until the queue is drained; every
-  // workgroup reaches the exit once the counter passes item_count.
   __shared__ uint32_t next_item, next_slot;
-  for (;;) {
+  for (uint32_t round = 0;; round++) {
     __syncthreads();  // every lane has read the previous item
     if (threadIdx.x == 0) {
-      const uint32_t item = atomicAdd(args.counter, 1u);
-      uint32_t lo = 0, hi = args.slot_count;  // last slot with tile_start <= item
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (args.slots[mid].tile_start <= item) lo = mid; else hi = mid;
+      uint32_t item, lo = 0;
+      if (!args.persistent) {
+        item = round == 0 ? blockIdx.x : 0xFFFFFFFFu;
+      } else {
+        item = atomicAdd(args.counter, 1u);
+        uint32_t hi = args.slot_count;  // last slot with tile_start <= item
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (args.slots[mid].tile_start <= item) lo = mid; else hi = mid;
+        }
       }
       next_item = item;
       next_slot = lo;
@@ -1244,7 +1323,7 @@ __global__ void __launch_bounds__(kThreads) jxr_reconstruct_tiles(KernelArgs arg
     __syncthreads();
     const uint32_t item = next_item;
     if (item >= args.item_count) break;
-    const Slot &slot = args.slots[next_slot];
+    const Slot &slot = args.persistent ? args.slots[next_slot] : args.inline_slot;
     process_tile(slot, item - slot.tile_start, args, lds, ts);
   }
 }

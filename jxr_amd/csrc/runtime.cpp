@@ -920,6 +920,7 @@ static jxrhip_status submit_impl(jxrhip_session *s, const jxrhip_plan *p, void *
   args.tile_y = ty;
   args.lds_words = words;
   const uint32_t tiles = args.inline_slot.tiles_x * ((p->mby + ty - 1) / ty);
+  args.item_count = tiles;
   HIP_TRY(hipMemsetAsync(status_word, 0, 4, sub->stream), "status reset");
   HIP_TRY(launch_tiles(args, tiles, sub->stream), "reconstruction launch");
   HIP_TRY(hipMemcpyAsync(s->status_host + sub->slot, status_word, 4, hipMemcpyDeviceToHost, sub->stream),

@@ -22,6 +22,7 @@
 #define __forceinline__ inline
 #define __shared__ static
 #define __launch_bounds__(x)
+#define JXR_GLOBAL
 
 struct dim3 {
   unsigned x, y, z;
@@ -72,6 +73,7 @@ inline unsigned __float_as_uint(float f) {
   return u;
 }
 inline float __fmul_rn(float a, float b) { return a * b; }
+inline unsigned __umulhi(unsigned a, unsigned b) { return (unsigned)(((unsigned long long)a * b) >> 32); }
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
 
