@@ -706,10 +706,13 @@ __device__ __forceinline__ void second_level_block(const DevImage &im, const Dev
   ovf |= RUN_BOUNDED(mag, kPctBound, [&](auto &a) { inverse_core(v, a); });
   if (povf) fail(status, JXRHIP_STAGE_PREDICTION);
   if (ovf) fail(status, JXRHIP_STAGE_SECOND_TRANSFORM);
-  int *o = smp + (by * 4 - g.soy) * (int)g.srw + (bx * 4 - g.sox);
+  // rows outside the region (the outer half of a halo block) are not kept
+  const int r0 = by * 4 - g.soy;
+  int *o = smp + r0 * (int)g.srw + (bx * 4 - g.sox);
 #pragma unroll
   for (int y = 0; y < 4; y++)
-    *reinterpret_cast<int4 *>(o + y * (int)g.srw) = make_int4(v[y * 4], v[y * 4 + 1], v[y * 4 + 2], v[y * 4 + 3]);
+    if ((uint32_t)(r0 + y) < g.srh)
+      *reinterpret_cast<v4i *>(o + y * (int)g.srw) = v4i{v[y * 4], v[y * 4 + 1], v[y * 4 + 2], v[y * 4 + 3]};
 }
 
 // ---- output stage -------------------------------------------------------
@@ -734,10 +737,6 @@ struct OutCtx {
   const int *lds;
 };
 
-__device__ __forceinline__ int sample_at(const OutCtx &o, int pl, int x, int y) {
-  const PlaneGeo &g = o.G[pl];
-  return o.lds[o.soff[pl] + (y - g.soy) * (int)g.srw + (x - g.sox)];
-}
 __device__ __forceinline__ int chroma_clamped(const OutCtx &o, int pl, int cx, int cy) {
   const PlaneGeo &g = o.G[pl];
   cx = min(max(cx, 0), (int)g.pw - 1);
@@ -813,32 +812,37 @@ __device__ __forceinline__ uint32_t f32_bits(int v, bool scaled, uint32_t length
   return sign | ((uint32_t)exponent << 23) | (uint32_t)mantissa;
 }
 
-__device__ __forceinline__ uint32_t pm_unsigned(uint32_t v, uint32_t a, uint32_t m) {
-  v = v > m ? m : v;
-  a = a > m ? m : a;
-  return (uint32_t)(((unsigned long long)v * a + m / 2) / m);
+// round(v * a / M) with both clipped to M (premultiply.rs); for M <= 65535
+// the product plus M / 2 stays below 2^32, so 32-bit arithmetic is exact.
+template <uint32_t M>
+__device__ __forceinline__ uint32_t pm_unsigned(uint32_t v, uint32_t a) {
+  static_assert(M <= 65535u, "32-bit premultiply");
+  v = v > M ? M : v;
+  a = a > M ? M : a;
+  return (v * a + M / 2) / M;
 }
-__device__ __forceinline__ bool pm_signed(int v, int a, int m, int &out) {
-  a = a < 0 ? 0 : (a > m ? m : a);
-  unsigned long long mag = v < 0 ? (unsigned long long)(-(long long)v) : (unsigned long long)v;
-  unsigned long long r = (mag * (unsigned long long)a + (unsigned long long)m / 2) / (unsigned long long)m;
-  long long s = v < 0 ? -(long long)r : (long long)r;
-  if (s > INT32_MAX || s < INT32_MIN) return false;
-  out = (int)s;
-  return true;
-}
-
-__device__ __forceinline__ void store_sample(JXR_GLOBAL uint8_t *dst, uint32_t value, uint32_t bytes) {
-  if (bytes == 1)
-    *dst = (uint8_t)value;
-  else if (bytes == 2)
-    *reinterpret_cast<JXR_GLOBAL uint16_t *>(dst) = (uint16_t)value;
-  else
-    *reinterpret_cast<JXR_GLOBAL uint32_t *>(dst) = value;
+template <int M>
+__device__ __forceinline__ bool pm_signed(int v, int a, int &out) {
+  a = a < 0 ? 0 : (a > M ? M : a);
+  if constexpr (M <= 32767) {
+    // |v| <= 32768 for i16 samples: the product stays below 2^31
+    const uint32_t mag = v < 0 ? 0u - (uint32_t)v : (uint32_t)v;
+    const uint32_t r = (mag * (uint32_t)a + (uint32_t)M / 2) / (uint32_t)M;
+    out = v < 0 ? -(int)r : (int)r;
+    return true;
+  } else {
+    const unsigned long long mag = v < 0 ? (unsigned long long)(-(long long)v) : (unsigned long long)v;
+    const unsigned long long r = (mag * (unsigned long long)a + (unsigned long long)M / 2) / (unsigned long long)M;
+    const long long s = v < 0 ? -(long long)r : (long long)r;
+    if (s > INT32_MAX || s < INT32_MIN) return false;
+    out = (int)s;
+    return true;
+  }
 }
 
 // Converted, ordered and formatted samples of one pixel (packing.rs:333-357,
 // premultiply.rs); returns false on arithmetic overflow.
+template <uint32_t STORE>
 __device__ __forceinline__ bool format_pixel(const DevImage &im, const int in[4], int alpha, uint32_t stored[6]) {
   bool ok = true;
   int conv[4] = {0, 0, 0, 0};
@@ -906,9 +910,9 @@ __device__ __forceinline__ bool format_pixel(const DevImage &im, const int in[4]
     if (c >= im.channels) continue;
     const bool is_alpha = im.has_alpha && c == pc;
     const ChanScale &cs = is_alpha ? im.alpha_scale : im.color_scale[c];
-    if (im.storage == JXRHIP_STORE_F16) {
+    if constexpr (STORE == JXRHIP_STORE_F16) {
       stored[c] = (cs.flags & 2u) ? 0u : f16_bits(ord[c], is_alpha ? im.f_alpha_scaled : im.f_scaled, ok);
-    } else if (im.storage == JXRHIP_STORE_F32) {
+    } else if constexpr (STORE == JXRHIP_STORE_F32) {
       stored[c] = (cs.flags & 2u) ? 0u
                                   : f32_bits(ord[c], is_alpha ? im.f_alpha_scaled : im.f_scaled,
                                              is_alpha ? im.f_alpha_mant : im.f_mant,
@@ -916,9 +920,9 @@ __device__ __forceinline__ bool format_pixel(const DevImage &im, const int in[4]
     } else {
       int v = 0;
       ok &= apply_scale(cs, ord[c], v);
-      if (im.storage == JXRHIP_STORE_U8) v = min(max(v, 0), 255);
-      else if (im.storage == JXRHIP_STORE_U16) v = min(max(v, 0), im.u16_max);
-      else if (im.storage == JXRHIP_STORE_I16) v = min(max(v, -32768), 32767);
+      if constexpr (STORE == JXRHIP_STORE_U8) v = min(max(v, 0), 255);
+      else if constexpr (STORE == JXRHIP_STORE_U16) v = min(max(v, 0), im.u16_max);
+      else if constexpr (STORE == JXRHIP_STORE_I16) v = min(max(v, -32768), 32767);
       stored[c] = (uint32_t)v;
     }
   }
@@ -931,50 +935,96 @@ __device__ __forceinline__ bool format_pixel(const DevImage &im, const int in[4]
 #pragma unroll
     for (uint32_t c = 0; c < 5; c++) {
       if (c >= ai) continue;
-      switch (im.storage) {
-        case JXRHIP_STORE_U8: stored[c] = pm_unsigned(stored[c], a, 255); break;
-        case JXRHIP_STORE_U16: stored[c] = pm_unsigned(stored[c], a, 65535); break;
-        case JXRHIP_STORE_I16: {
-          int r = 0;
-          ok &= pm_signed((int16_t)stored[c], (int16_t)a, 32767, r);
-          stored[c] = (uint32_t)r;
-          break;
-        }
-        case JXRHIP_STORE_I32: {
-          int r = 0;
-          ok &= pm_signed((int)stored[c], (int)a, INT32_MAX, r);
-          stored[c] = (uint32_t)r;
-          break;
-        }
-        case JXRHIP_STORE_F16: {
-          const uint32_t am = (a & 0x8000u) ? 0u : (a & 0x7fffu);
-          stored[c] = (stored[c] & 0x8000u) | pm_unsigned(stored[c] & 0x7fffu, am, 0x7fff);
-          break;
-        }
-        case JXRHIP_STORE_F32: {
-          const float af = __uint_as_float(a);
-          const float ac = af < 0.0f ? 0.0f : (af > 1.0f ? 1.0f : af);
-          stored[c] = __float_as_uint(__fmul_rn(__uint_as_float(stored[c]), ac));
-          break;
-        }
-        default: break;
+      if constexpr (STORE == JXRHIP_STORE_U8) {
+        stored[c] = pm_unsigned<255>(stored[c], a);
+      } else if constexpr (STORE == JXRHIP_STORE_U16) {
+        stored[c] = pm_unsigned<65535>(stored[c], a);
+      } else if constexpr (STORE == JXRHIP_STORE_I16) {
+        int r = 0;
+        ok &= pm_signed<32767>((int16_t)stored[c], (int16_t)a, r);
+        stored[c] = (uint32_t)r;
+      } else if constexpr (STORE == JXRHIP_STORE_I32) {
+        int r = 0;
+        ok &= pm_signed<INT32_MAX>((int)stored[c], (int)a, r);
+        stored[c] = (uint32_t)r;
+      } else if constexpr (STORE == JXRHIP_STORE_F16) {
+        const uint32_t am = (a & 0x8000u) ? 0u : (a & 0x7fffu);
+        stored[c] = (stored[c] & 0x8000u) | pm_unsigned<0x7fff>(stored[c] & 0x7fffu, am);
+      } else {
+        const float af = __uint_as_float(a);
+        const float ac = af < 0.0f ? 0.0f : (af > 1.0f ? 1.0f : af);
+        stored[c] = __float_as_uint(__fmul_rn(__uint_as_float(stored[c]), ac));
       }
     }
   }
   return ok;
 }
 
-// Four consecutive samples of one channel, stored at `dst` (the first
-// pixel's address); one vector store when all four are valid and the
-// address is aligned to the run, one store per valid sample otherwise.
-__device__ __forceinline__ void store_run(JXR_GLOBAL uint8_t *dst, const uint32_t s[4], uint32_t valid,
-                                          uint32_t bytes, size_t step) {
+template <uint32_t STORE>
+struct StoreBytes {
+  static constexpr uint32_t value =
+      STORE == JXRHIP_STORE_U8 ? 1u : (STORE == JXRHIP_STORE_I32 || STORE == JXRHIP_STORE_F32) ? 4u : 2u;
+};
+
+// Dword vectors that need only 4-byte alignment (interleaved rows).
+typedef uint32_t v4u_a4 __attribute__((vector_size(16), aligned(4)));
+typedef uint32_t v2u_a4 __attribute__((vector_size(8), aligned(4)));
+
+template <uint32_t B>
+__device__ __forceinline__ void store_sample(JXR_GLOBAL uint8_t *dst, uint32_t value) {
+  if constexpr (B == 1)
+    *dst = (uint8_t)value;
+  else if constexpr (B == 2)
+    *reinterpret_cast<JXR_GLOBAL uint16_t *>(dst) = (uint16_t)value;
+  else
+    *reinterpret_cast<JXR_GLOBAL uint32_t *>(dst) = value;
+}
+
+// W consecutive dwords at a 4-byte aligned address.
+template <uint32_t W>
+__device__ __forceinline__ void store_dwords(JXR_GLOBAL uint8_t *dst, const uint32_t *w) {
+  uint32_t k = 0;
+#pragma unroll
+  for (; k + 4 <= W; k += 4) {
+    const v4u_a4 v = {w[k], w[k + 1], w[k + 2], w[k + 3]};
+    *reinterpret_cast<JXR_GLOBAL v4u_a4 *>(dst + 4 * k) = v;
+  }
+  if constexpr (W % 4 >= 2) {
+    const v2u_a4 v = {w[W - W % 4], w[W - W % 4 + 1]};
+    *reinterpret_cast<JXR_GLOBAL v2u_a4 *>(dst + 4 * (W - W % 4)) = v;
+  }
+  if constexpr (W % 2 == 1) *reinterpret_cast<JXR_GLOBAL uint32_t *>(dst + 4 * (W - 1)) = w[W - 1];
+}
+
+// Four pixels of CH interleaved B-byte samples as CH * B dwords.
+template <uint32_t B, uint32_t CH>
+__device__ __forceinline__ void store_interleaved(JXR_GLOBAL uint8_t *dst, const uint32_t st[6][4]) {
+  constexpr uint32_t W = CH * B;
+  constexpr uint32_t mask = B == 4 ? 0xFFFFFFFFu : (1u << (8 * B)) - 1u;
+  uint32_t w[W];
+#pragma unroll
+  for (uint32_t k = 0; k < W; k++) w[k] = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < 4; i++)
+#pragma unroll
+    for (uint32_t c = 0; c < CH; c++) {
+      const uint32_t byte = (i * CH + c) * B;
+      w[byte / 4] |= (st[c][i] & mask) << (8 * (byte % 4));
+    }
+  store_dwords<W>(dst, w);
+}
+
+// Four consecutive samples of one channel at `dst` (the first pixel's
+// address): one vector store when all four are valid and the address is
+// aligned to the run, one store per valid sample otherwise.
+template <uint32_t B>
+__device__ __forceinline__ void store_run(JXR_GLOBAL uint8_t *dst, const uint32_t s[4], uint32_t valid) {
   const uintptr_t addr = reinterpret_cast<uintptr_t>(dst);
-  if (valid == 0xFu && step == bytes && (addr & (4 * bytes - 1)) == 0) {
-    if (bytes == 1) {
+  if (valid == 0xFu && (addr & (4 * B - 1)) == 0) {
+    if constexpr (B == 1) {
       *reinterpret_cast<JXR_GLOBAL uint32_t *>(dst) =
           (s[0] & 0xFFu) | ((s[1] & 0xFFu) << 8) | ((s[2] & 0xFFu) << 16) | ((s[3] & 0xFFu) << 24);
-    } else if (bytes == 2) {
+    } else if constexpr (B == 2) {
       const v2u w = {(s[0] & 0xFFFFu) | (s[1] << 16), (s[2] & 0xFFFFu) | (s[3] << 16)};
       *reinterpret_cast<JXR_GLOBAL v2u *>(dst) = w;
     } else {
@@ -985,14 +1035,25 @@ __device__ __forceinline__ void store_run(JXR_GLOBAL uint8_t *dst, const uint32_
   }
 #pragma unroll
   for (int i = 0; i < 4; i++)
-    if (valid & (1u << i)) store_sample(dst + i * step, s[i], bytes);
+    if (valid & (1u << i)) store_sample<B>(dst + i * B, s[i]);
+}
+
+// Four consecutive full-resolution samples of plane pl starting at x, a
+// multiple of 4: sample regions start 4 samples left of a macroblock edge
+// and have row lengths and offsets that are multiples of 4 words, so the
+// quad is one aligned 16-byte LDS read.
+__device__ __forceinline__ v4i sample_quad(const OutCtx &o, int pl, int x, int y) {
+  const PlaneGeo &g = o.G[pl];
+  return *reinterpret_cast<const v4i *>(o.lds + o.soff[pl] + (y - g.soy) * (int)g.srw + (x - g.sox));
 }
 
 // One output group: pixels x = 4gx..4gx+3, y = 2gy..2gy+1 (window coords)
 // inside [x0, x1) x [y0, y1). 4:2:0 chroma is interpolated once for the
 // group: three chroma rows and four chroma columns give both rows' values.
-__device__ __forceinline__ void output_group(const OutCtx &o, JXR_GLOBAL uint8_t *out, int gx, int gy, int x0, int x1, int y0, int y1,
-                             uint32_t *status) {
+template <uint32_t STORE>
+__device__ __forceinline__ void output_group(const OutCtx &o, JXR_GLOBAL uint8_t *out, int gx, int gy, int x0,
+                                             int x1, int y0, int y1, bool &ok) {
+  constexpr uint32_t B = StoreBytes<STORE>::value;
   const DevImage &im = *o.im;
   const int xb = 4 * gx, yb = 2 * gy;
   int chroma[2][2][4];  // [U/V][row][pixel]
@@ -1031,58 +1092,79 @@ __device__ __forceinline__ void output_group(const OutCtx &o, JXR_GLOBAL uint8_t
         }
       }
   }
-  bool ok = true;
+  uint32_t valid = 0;
 #pragma unroll
+  for (int i = 0; i < 4; i++) valid |= (xb + i >= x0 && xb + i < x1) ? 1u << i : 0u;
+  const int ox0 = xb - (int)im.crop_x;  // output x of pixel 0 (may be < 0 when invalid)
+#pragma unroll 1
   for (int j = 0; j < 2; j++) {
     const int y = yb + j;
     if (y < y0 || y >= y1) continue;
+    const v4i l = sample_quad(o, 0, xb, y);
+    v4i u = {0, 0, 0, 0}, v = {0, 0, 0, 0}, k = {0, 0, 0, 0}, al = {0, 0, 0, 0};
+    if (im.colors > 1) {
+      if (im.upsample) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          u[i] = j ? chroma[0][1][i] : chroma[0][0][i];
+          v[i] = j ? chroma[1][1][i] : chroma[1][0][i];
+        }
+      } else {
+        u = sample_quad(o, 1, xb, y);
+        v = sample_quad(o, 2, xb, y);
+      }
+      if (im.colors > 3) k = sample_quad(o, 3, xb, y);
+    }
+    if (im.has_alpha) al = sample_quad(o, im.alpha_plane, xb, y);
     uint32_t st[6][4];
-    uint32_t valid = 0;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const int x = xb + i;
-#pragma unroll
-      for (int c = 0; c < 6; c++) st[c][i] = 0;
-      if (x < x0 || x >= x1) continue;
-      valid |= 1u << i;
-      int in[4] = {0, 0, 0, 0};
-      in[0] = sample_at(o, 0, x, y);
-      if (im.colors > 1) {
-        if (im.upsample) {
-          in[1] = chroma[0][j][i];
-          in[2] = chroma[1][j][i];
-        } else {
-          in[1] = sample_at(o, 1, x, y);
-          in[2] = sample_at(o, 2, x, y);
-        }
-        if (im.colors > 3) in[3] = sample_at(o, 3, x, y);
-      }
-      const int a = im.has_alpha ? sample_at(o, im.alpha_plane, x, y) : 0;
+      const int in[4] = {l[i], u[i], v[i], k[i]};
       uint32_t px[6];
-      ok &= format_pixel(im, in, a, px);
+      const bool good = format_pixel<STORE>(im, in, al[i], px);
+      ok &= good || !(valid & (1u << i));
 #pragma unroll
       for (int c = 0; c < 6; c++) st[c][i] = px[c];
     }
-    if (!valid) continue;
     const uint32_t oy = (uint32_t)(y - (int)im.crop_y);
-    const int ox0 = xb - (int)im.crop_x;  // output x of pixel 0 (may be < 0 when invalid)
     if (im.planar || im.channels == 1) {
 #pragma unroll
       for (uint32_t c = 0; c < 6; c++) {
         if (c >= im.channels) continue;
         const uint32_t pl = im.planar ? c : 0;
         JXR_GLOBAL uint8_t *row = out + im.plane_offset[pl] + (uint64_t)oy * im.row_stride[pl];
-        store_run(row + (int64_t)ox0 * (int64_t)im.bytes, st[c], valid, im.bytes, im.bytes);
+        store_run<B>(row + (int64_t)ox0 * B, st[c], valid);
       }
     } else {
       JXR_GLOBAL uint8_t *row = out + im.plane_offset[0] + (uint64_t)oy * im.row_stride[0];
-      const size_t pitch = (size_t)im.channels * im.bytes;
+      const uint32_t pitch = im.channels * B;
+      JXR_GLOBAL uint8_t *px0 = row + (int64_t)ox0 * pitch;
+      const bool aligned = valid == 0xFu && (reinterpret_cast<uintptr_t>(px0) & 3u) == 0;
+      if (aligned && im.channels == 3) {
+        store_interleaved<B, 3>(px0, st);
+      } else if (aligned && im.channels == 4) {
+        store_interleaved<B, 4>(px0, st);
+      } else {
 #pragma unroll
-      for (uint32_t c = 0; c < 6; c++) {
-        if (c >= im.channels) continue;
-        store_run(row + (int64_t)ox0 * (int64_t)pitch + c * im.bytes, st[c], valid, im.bytes, pitch);
+        for (int i = 0; i < 4; i++) {
+          if (!(valid & (1u << i))) continue;
+#pragma unroll
+          for (uint32_t c = 0; c < 6; c++)
+            if (c < im.channels) store_sample<B>(px0 + i * pitch + c * B, st[c][i]);
+        }
       }
     }
+  }
+}
+
+template <uint32_t STORE>
+__device__ __forceinline__ void output_tile(const OutCtx &o, JXR_GLOBAL uint8_t *out, int gx0, int gy0, int ngx,
+                                            int ngy, int x0, int x1, int y0, int y1, uint32_t *status) {
+  const uint32_t m = div_magic((uint32_t)ngx);
+  bool ok = true;
+  for (int t = threadIdx.x; t < ngx * ngy; t += blockDim.x) {
+    const DivMod u = divmod(t, (uint32_t)ngx, m);
+    output_group<STORE>(o, out, gx0 + u.r, gy0 + u.q, x0, x1, y0, y1, ok);
   }
   if (!ok) fail(status, JXRHIP_STAGE_OUTPUT);
 }
@@ -1121,48 +1203,79 @@ __device__ __forceinline__ int plane_of(const int *start, int np, int t) {
 
 __device__ __forceinline__ uint32_t log2_small(uint32_t v) { return v == 4 ? 2u : v == 2 ? 1u : 0u; }
 
-// Thread 0 lays out the tile: regions, ranges and work counts.
-__device__ __forceinline__ void tile_setup(TileShared &ts, uint32_t tile, uint32_t tiles_x, int TX, int TY) {
+// Tile layout, in two steps: thread pl lays out plane pl (regions, ranges,
+// work widths and counts), then thread 0 places the regions in LDS and sums
+// the per-plane work counts of every stage.
+__device__ __forceinline__ void plane_setup(TileShared &ts, int pl, uint32_t tile, uint32_t tiles_x, int TX,
+                                            int TY) {
   const DevImage &im = ts.im;
   const uint32_t ty_idx = tile / tiles_x;
   const int tmx = (int)(tile - ty_idx * tiles_x) * TX, tmy = (int)ty_idx * TY;
   const int nx = min(TX, (int)im.mbx - tmx), ny = min(TY, (int)im.mby - tmy);
+  if (pl == 0) {
+    ts.tmx = tmx;
+    ts.tmy = tmy;
+    ts.nx = nx;
+    ts.ny = ny;
+  }
+  const DevPlane &p = im.planes[pl];
+  PlaneGeo &g = ts.g[pl];
+  g.bc = p.bc;
+  g.br = p.br;
+  g.bcs = log2_small(p.bc);
+  g.brs = log2_small(p.br);
+  g.mw = p.bc * 4;
+  g.mh = p.br * 4;
+  g.pw = im.mbx * g.mw;
+  g.ph = im.mby * g.mh;
+  g.lw = im.mbx * p.bc;
+  g.lh = im.mby * p.br;
+  g.sox = tmx * (int)g.mw - 4;
+  g.soy = tmy * (int)g.mh - kSampleHaloRows;
+  g.srw = (uint32_t)TX * g.mw + 8;
+  g.srh = (uint32_t)TY * g.mh + 2 * kSampleHaloRows;
+  const int h = (int)low_halo(p.bc);
+  g.lox = (tmx - h) * (int)p.bc;
+  g.loy = (tmy - h) * (int)p.br;
+  g.lrw = (uint32_t)(TX + 2 * h) * p.bc;
+  g.lrh = (uint32_t)(TY + 2 * h) * p.br;
+  const BlockRange brg = block_range(p, g, tmx, tmy, nx, ny);
+  const CornerRange cr = corner_range(im, p, tmx, tmy, nx, ny);
+  ts.br[pl] = brg;
+  ts.cr[pl] = cr;
+  const bool lp = p.overlap == 2, sub = lp && p.bc == 2;
+  PlaneWork &w = ts.w[pl];
+  w.wA = (uint32_t)(nx + 2 * h);
+  w.wB = p.bc == 4 ? (uint32_t)(nx + 1) : (uint32_t)(cr.mx1 - cr.mx0);
+  w.wB2 = (uint32_t)cell_columns(nx);
+  w.wC = (uint32_t)(brg.bx1 - brg.bx0);
+  w.wD = (uint32_t)nx * p.bc + 1;
+  w.mA = div_magic(w.wA);
+  w.mB = div_magic(w.wB);
+  w.mB2 = div_magic(w.wB2);
+  w.mC = div_magic(w.wC);
+  w.mD = div_magic(w.wD);
+  // per-plane counts, summed by thread 0
+  ts.sA[pl + 1] = (nx + 2 * h) * (ny + 2 * h);
+  ts.sB1[pl + 1] = !lp ? 0 : p.bc == 4 ? (nx + 1) * (ny + 1) : cr.count();
+  ts.sB2[pl + 1] = sub ? cell_columns(nx) * cell_rows(p, ny) : 0;
+  ts.sB3[pl + 1] = sub ? cr.count() : 0;
+  ts.sC[pl + 1] = brg.count();
+  ts.sD[pl + 1] = p.overlap >= 1 ? (nx * (int)p.bc + 1) * (ny * (int)p.br + 1) : 0;
+}
+
+__device__ __forceinline__ void tile_finish(TileShared &ts) {
+  const DevImage &im = ts.im;
   const int np = (int)im.plane_count;
-  ts.tmx = tmx;
-  ts.tmy = tmy;
-  ts.nx = nx;
-  ts.ny = ny;
   int cursor = 0;
   int any_lp = 0, any_sub = 0, any_pot = 0;
   for (int pl = 0; pl < np; pl++) {
-    const DevPlane &p = im.planes[pl];
-    PlaneGeo &g = ts.g[pl];
-    g.bc = p.bc;
-    g.br = p.br;
-    g.bcs = log2_small(p.bc);
-    g.brs = log2_small(p.br);
-    g.mw = p.bc * 4;
-    g.mh = p.br * 4;
-    g.pw = im.mbx * g.mw;
-    g.ph = im.mby * g.mh;
-    g.lw = im.mbx * p.bc;
-    g.lh = im.mby * p.br;
-    g.sox = tmx * (int)g.mw - 4;
-    g.soy = tmy * (int)g.mh - 4;
-    g.srw = (uint32_t)TX * g.mw + 8;
-    g.srh = (uint32_t)TY * g.mh + 8;
-    const int h = (int)low_halo(p.bc);
-    g.lox = (tmx - h) * (int)p.bc;
-    g.loy = (tmy - h) * (int)p.br;
-    g.lrw = (uint32_t)(TX + 2 * h) * p.bc;
-    g.lrh = (uint32_t)(TY + 2 * h) * p.br;
     ts.soff[pl] = cursor;
-    cursor += (int)(g.srw * g.srh);
+    cursor += (int)(ts.g[pl].srw * ts.g[pl].srh);
+    const DevPlane &p = im.planes[pl];
     any_lp |= p.overlap == 2;
     any_sub |= p.overlap == 2 && p.bc == 2;
     any_pot |= p.overlap >= 1;
-    ts.br[pl] = block_range(p, g, tmx, tmy, nx, ny);
-    ts.cr[pl] = corner_range(im, p, tmx, tmy, nx, ny);
   }
   for (int pl = 0; pl < np; pl++) {
     ts.loff[pl] = cursor;
@@ -1173,38 +1286,31 @@ __device__ __forceinline__ void tile_setup(TileShared &ts, uint32_t tile, uint32
   ts.any_pot = any_pot;
   ts.sA[0] = ts.sB1[0] = ts.sB2[0] = ts.sB3[0] = ts.sC[0] = ts.sD[0] = 0;
   for (int pl = 0; pl < np; pl++) {
-    const DevPlane &p = im.planes[pl];
-    PlaneWork &w = ts.w[pl];
-    const int h = (int)low_halo(p.bc);
-    const bool lp = p.overlap == 2, sub = lp && p.bc == 2;
-    w.wA = (uint32_t)(nx + 2 * h);
-    w.wB = p.bc == 4 ? (uint32_t)(nx + 1) : (uint32_t)(ts.cr[pl].mx1 - ts.cr[pl].mx0);
-    w.wB2 = (uint32_t)cell_columns(nx);
-    w.wC = (uint32_t)(ts.br[pl].bx1 - ts.br[pl].bx0);
-    w.wD = (uint32_t)nx * p.bc + 1;
-    w.mA = div_magic(w.wA);
-    w.mB = div_magic(w.wB);
-    w.mB2 = div_magic(w.wB2);
-    w.mC = div_magic(w.wC);
-    w.mD = div_magic(w.wD);
-    ts.sA[pl + 1] = ts.sA[pl] + (nx + 2 * h) * (ny + 2 * h);
-    ts.sB1[pl + 1] = ts.sB1[pl] + (!lp ? 0 : p.bc == 4 ? (nx + 1) * (ny + 1) : ts.cr[pl].count());
-    ts.sB2[pl + 1] = ts.sB2[pl] + (sub ? cell_columns(nx) * cell_rows(p, ny) : 0);
-    ts.sB3[pl + 1] = ts.sB3[pl] + (sub ? ts.cr[pl].count() : 0);
-    ts.sC[pl + 1] = ts.sC[pl] + ts.br[pl].count();
-    ts.sD[pl + 1] = ts.sD[pl] + (p.overlap >= 1 ? (nx * (int)p.bc + 1) * (ny * (int)p.br + 1) : 0);
+    ts.sA[pl + 1] += ts.sA[pl];
+    ts.sB1[pl + 1] += ts.sB1[pl];
+    ts.sB2[pl + 1] += ts.sB2[pl];
+    ts.sB3[pl + 1] += ts.sB3[pl];
+    ts.sC[pl + 1] += ts.sC[pl];
+    ts.sD[pl + 1] += ts.sD[pl];
   }
 }
 
+// One tile of one image. `reload` is false when the previous tile of this
+// workgroup belonged to the same image, whose descriptor is still in LDS.
+template <uint32_t STORE>
 __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, const KernelArgs &args, int *lds,
-                                             TileShared &ts) {
-  {
+                                             TileShared &ts, bool reload) {
+  __syncthreads();  // the previous tile's output stage has read the tile state
+  if (reload) {
     const JXR_GLOBAL v4u *src = (const JXR_GLOBAL v4u *)gptr(slot.img);
     v4u *dst = reinterpret_cast<v4u *>(&ts.im);
     for (int i = threadIdx.x; i < (int)(sizeof(DevImage) / sizeof(v4u)); i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
   }
+  for (int pl = threadIdx.x; pl < (int)ts.im.plane_count; pl += blockDim.x)
+    plane_setup(ts, pl, tile, slot.tiles_x, (int)args.tile_x, (int)args.tile_y);
   __syncthreads();
-  if (threadIdx.x == 0) tile_setup(ts, tile, slot.tiles_x, (int)args.tile_x, (int)args.tile_y);
+  if (threadIdx.x == 0) tile_finish(ts);
   __syncthreads();
   const DevImage &im = ts.im;
   const int np = (int)im.plane_count;
@@ -1286,64 +1392,129 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
   if (x1 <= x0 || y1 <= y0) return;
   const int gx0 = x0 >> 2, gx1 = ((x1 - 1) >> 2) + 1;
   const int gy0 = y0 >> 1, gy1 = ((y1 - 1) >> 1) + 1;
-  const int ngx = gx1 - gx0;
-  const uint32_t mE = div_magic((uint32_t)ngx);
-  OutCtx o{&im, ts.g, ts.soff, lds};
-  for (int t = threadIdx.x; t < ngx * (gy1 - gy0); t += blockDim.x) {
-    const DivMod u = divmod(t, (uint32_t)ngx, mE);
-    output_group(o, gptr_mut(slot.out), gx0 + u.r, gy0 + u.q, x0, x1, y0, y1, status);
-  }
+  const int ngx = gx1 - gx0, ngy = gy1 - gy0;
+  const OutCtx o{&im, ts.g, ts.soff, lds};
+  output_tile<STORE>(o, gptr_mut(slot.out), gx0, gy0, ngx, ngy, x0, x1, y0, y1, status);
 }
 
-// Per-image launches take tile blockIdx.x of the inline slot; persistent
-// launches pull work items until the queue is drained, and every workgroup
-// reaches the exit once the counter passes item_count. One call site keeps
-// process_tile inlined once, so its LDS pointers stay LDS-addressed.
-__global__ void __launch_bounds__(kThreads) jxr_reconstruct_tiles(KernelArgs args) {
-  HIP_DYNAMIC_SHARED(int, lds)
+// Work assignment: workgroup b takes an arithmetic sequence of work items.
+// Workgroups are dispatched to the eight XCDs round-robin (b mod 8), and
+// each XCD's workgroups walk one contiguous eighth of the items, so tiles
+// processed at the same time on one XCD are neighbours whose halo reads meet
+// in that XCD's L2. Per-image launches run one item per workgroup (the grid
+// is rounded up to a multiple of 8); persistent launches size the grid to
+// the device and loop. Item i belongs to image item_slot[i].
+//
+// One call site keeps process_tile inlined once, so its LDS pointers stay
+// LDS-addressed. One instantiation per storage kind keeps each kernel's
+// registers and code to what its output format needs; every image of a
+// launch shares the kind.
+template <uint32_t STORE>
+__global__ void __launch_bounds__(kThreads, kMinWavesPerSimd) jxr_reconstruct_tiles(KernelArgs args) {
+  HIP_DYNAMIC_SHARED(v4i, lds_quads)  // 16-byte aligned: regions are read and written in quads
+  int *lds = reinterpret_cast<int *>(lds_quads);
   __shared__ TileShared ts;
-  __shared__ uint32_t next_item, next_slot;
-  for (uint32_t round = 0;; round++) {
-    __syncthreads();  // every lane has read the previous item
-    if (threadIdx.x == 0) {
-      uint32_t item, lo = 0;
-      if (!args.persistent) {
-        item = round == 0 ? blockIdx.x : 0xFFFFFFFFu;
-      } else {
-        item = atomicAdd(args.counter, 1u);
-        uint32_t hi = args.slot_count;  // last slot with tile_start <= item
-        while (hi - lo > 1) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (args.slots[mid].tile_start <= item) lo = mid; else hi = mid;
-        }
-      }
-      next_item = item;
-      next_slot = lo;
-    }
-    __syncthreads();
-    const uint32_t item = next_item;
-    if (item >= args.item_count) break;
-    const Slot &slot = args.persistent ? args.slots[next_slot] : args.inline_slot;
-    process_tile(slot, item - slot.tile_start, args, lds, ts);
+  const uint32_t G = gridDim.x, b = blockIdx.x, n = args.item_count;
+  uint32_t item, stride, end;
+  if ((G & 7u) == 0) {
+    const uint32_t x = b & 7u;
+    stride = G >> 3;
+    item = (uint32_t)(((uint64_t)n * x) >> 3) + (b >> 3);
+    end = (uint32_t)(((uint64_t)n * (x + 1)) >> 3);
+  } else {
+    item = b;
+    stride = G;
+    end = n;
+  }
+  if (item >= end) return;
+  const JXR_GLOBAL uint32_t *item_slot = gptr(args.item_slot);
+  uint32_t slot_index = args.persistent ? item_slot[item] : 0u;
+  uint32_t loaded = 0xFFFFFFFFu;
+  for (;;) {
+    const uint32_t next = item + stride;
+    // issued before the tile, used after it
+    const uint32_t next_slot = (args.persistent && next < end) ? item_slot[next] : 0u;
+    const Slot &slot = args.persistent ? args.slots[slot_index] : args.inline_slot;
+    process_tile<STORE>(slot, item - slot.tile_start, args, lds, ts, slot_index != loaded);
+    loaded = slot_index;
+    if (next >= end) break;
+    item = next;
+    slot_index = next_slot;
   }
 }
 
 // ---- host-side launch wrappers ----------------------------------------------
 hipError_t launch_tiles(const KernelArgs &args, uint32_t grid, hipStream_t stream) {
   const size_t lds = (size_t)args.lds_words * sizeof(int);
-  hipLaunchKernelGGL(jxr_reconstruct_tiles, dim3(grid), dim3(kThreads), lds, stream, args);
+  switch (args.storage) {
+    case JXRHIP_STORE_U8:
+      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_U8>, dim3(grid), dim3(kThreads), lds, stream, args);
+      break;
+    case JXRHIP_STORE_U16:
+      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_U16>, dim3(grid), dim3(kThreads), lds, stream, args);
+      break;
+    case JXRHIP_STORE_I16:
+      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_I16>, dim3(grid), dim3(kThreads), lds, stream, args);
+      break;
+    case JXRHIP_STORE_I32:
+      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_I32>, dim3(grid), dim3(kThreads), lds, stream, args);
+      break;
+    case JXRHIP_STORE_F16:
+      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_F16>, dim3(grid), dim3(kThreads), lds, stream, args);
+      break;
+    case JXRHIP_STORE_F32:
+      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_F32>, dim3(grid), dim3(kThreads), lds, stream, args);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
-hipError_t set_kernel_lds_limit(size_t bytes) {
-  return hipFuncSetAttribute(reinterpret_cast<const void *>(&jxr_reconstruct_tiles),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+hipError_t kernel_occupancy(uint32_t storage, size_t dynamic_lds, int *groups_per_cu) {
+  const void *k;
+  switch (storage) {
+    case JXRHIP_STORE_U8: k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_U8>); break;
+    case JXRHIP_STORE_U16: k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_U16>); break;
+    case JXRHIP_STORE_I16: k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_I16>); break;
+    case JXRHIP_STORE_I32: k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_I32>); break;
+    case JXRHIP_STORE_F16: k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_F16>); break;
+    case JXRHIP_STORE_F32: k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_F32>); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu, k, (int)kThreads, dynamic_lds);
 }
 
-__global__ void jxr_reset_counter(uint32_t *counter) { *counter = 0; }
+hipError_t set_kernel_lds_limit(size_t bytes) {
+  const void *kernels[] = {
+      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_U8>),
+      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_U16>),
+      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_I16>),
+      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_I32>),
+      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_F16>),
+      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_F32>)};
+  for (const void *k : kernels) {
+    const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
 
-hipError_t launch_reset(uint32_t *counter, hipStream_t stream) {
-  hipLaunchKernelGGL(jxr_reset_counter, dim3(1), dim3(1), 0, stream, counter);
+// item_slot[i] = index of the image that owns work item i: workgroup b
+// writes image b's items [tile_start_b, tile_start_{b+1}).
+__global__ void __launch_bounds__(kThreads) jxr_expand_slots(const Slot *slots, uint32_t slot_count,
+                                                             uint32_t item_count, uint32_t *item_slot) {
+  const uint32_t b = blockIdx.x;
+  if (b >= slot_count) return;
+  const uint32_t first = slots[b].tile_start;
+  const uint32_t last = b + 1 < slot_count ? slots[b + 1].tile_start : item_count;
+  for (uint32_t i = first + threadIdx.x; i < last; i += blockDim.x) item_slot[i] = b;
+}
+
+hipError_t launch_expand(const Slot *slots, uint32_t slot_count, uint32_t item_count, uint32_t *item_slot,
+                         hipStream_t stream) {
+  hipLaunchKernelGGL(jxr_expand_slots, dim3(slot_count), dim3(kThreads), 0, stream, slots, slot_count, item_count,
+                     item_slot);
   return hipGetLastError();
 }
 

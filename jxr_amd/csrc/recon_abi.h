@@ -8,6 +8,9 @@ namespace jxr {
 
 constexpr uint32_t kMaxPlanes = 5;     // Y, U, V (or CMYK/K), alpha
 constexpr uint32_t kThreads = 256;     // threads per workgroup (4 wave64)
+// Waves each SIMD must hold: four workgroups per compute unit (LDS allows
+// four 4:2:0 tiles of 4x4 macroblocks), so at most 128 VGPRs per lane.
+constexpr uint32_t kMinWavesPerSimd = 4;
 
 // Raster-ordered macroblock record (the reference's JxrMacroblockAbi words,
 // crates/jxr-math/data/reconstruction.abi:3, without coded_x/y because the
@@ -93,22 +96,29 @@ struct alignas(32) Slot {
 struct KernelArgs {
   Slot inline_slot;      // per-image launch: the one image
   const Slot *slots;     // persistent: images, ordered by tile_start
+  const uint32_t *item_slot;  // persistent: owning image of each work item
   uint32_t slot_count;
-  uint32_t item_count;   // persistent: total tiles
-  uint32_t *counter;     // persistent: next work item
+  uint32_t item_count;   // work items (tiles) of the launch
   uint32_t persistent;
   uint32_t tile_x, tile_y;  // macroblocks per tile
   uint32_t lds_words;    // dynamic LDS in 32-bit words
+  uint32_t storage;      // JXRHIP_STORE_* of every image of the launch
 };
 
 // Halo of the low-pass region in macroblocks: the subsampled first-level
 // overlap reads a corner's neighbour one pair beyond the block halo.
 __host__ __device__ inline uint32_t low_halo(uint32_t bc) { return bc == 2 ? 2u : 1u; }
 
+// Rows of second-level output kept above and below a tile: the sample-level
+// overlap reaches two rows past the tile edge and chroma upsampling one.
+// Columns keep the whole halo block (four samples), so that rows start on
+// 16-byte boundaries.
+constexpr int kSampleHaloRows = 2;
+
 // Dynamic LDS words one workgroup needs for a tile shape and plane set.
 __host__ __device__ inline uint32_t sample_region_words(uint32_t bc, uint32_t br, uint32_t tx,
                                                         uint32_t ty) {
-  return (tx * bc * 4 + 8) * (ty * br * 4 + 8);
+  return (tx * bc * 4 + 8) * (ty * br * 4 + 2 * kSampleHaloRows);
 }
 __host__ __device__ inline uint32_t low_region_words(uint32_t bc, uint32_t br, uint32_t tx,
                                                      uint32_t ty) {

@@ -31,7 +31,9 @@
 
 namespace jxr {
 hipError_t launch_tiles(const KernelArgs &args, uint32_t grid, hipStream_t stream);
-hipError_t launch_reset(uint32_t *counter, hipStream_t stream);
+hipError_t launch_expand(const Slot *slots, uint32_t slot_count, uint32_t item_count, uint32_t *item_slot,
+                         hipStream_t stream);
+hipError_t kernel_occupancy(uint32_t storage, size_t dynamic_lds, int *groups_per_cu);
 hipError_t set_kernel_lds_limit(size_t bytes);
 }  // namespace jxr
 
@@ -460,9 +462,11 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
     plan->alg_bytes = stored * 4 + mbs * sizeof(MbRec) + out_px_bytes;
   }
   plan->descriptor_bytes += sizeof(DevImage);
+  // bucket key: plane geometries and the storage kind (one kernel
+  // instantiation per storage kind)
   uint64_t key = plan->planes.size();
   for (const PlanPlane &p : plan->planes) key = key * 16 + p.bc * 4 + p.br;
-  plan->geometry_key = key;
+  plan->geometry_key = key * 16 + dv.storage;
   *out = plan.release();
   return JXRHIP_OK;
 }
@@ -561,7 +565,6 @@ struct jxrhip_session {
   uint64_t cache_bytes = 0, cache_budget = uint64_t(64) << 30, tick = 0;
   uint32_t next_stream = 0;
   uint32_t tile_x = 0, tile_y = 0, groups_per_cu = 0;
-  DeviceBuffer counter;
   double last_ms = 0;
   uint32_t last_launches = 0;
   jxrhip_diagnostics diag{};
@@ -622,7 +625,6 @@ extern "C" jxrhip_status jxrhip_session_create(int32_t ordinal, jxrhip_session *
     (void)hipGetLastError();
     s->max_lds = 65536;
   }
-  HIP_TRY(s->pool.get(256, s->counter, s->diag.scratch_hits, s->diag.scratch_misses), "counter");
   HIP_TRY(hipMalloc((void **)&s->status_dev, kStatusSlots * sizeof(uint32_t)), "status words");
   HIP_TRY(hipHostMalloc((void **)&s->status_host, kStatusSlots * sizeof(uint32_t), hipHostMallocDefault),
           "status words");
@@ -642,7 +644,6 @@ extern "C" void jxrhip_session_destroy(jxrhip_session *s) {
     if (s->streams[i]) (void)hipStreamSynchronize(s->streams[i]);
   for (auto &kv : s->coef_cache) s->pool.put(kv.second.buf);
   for (auto &kv : s->desc_cache) s->pool.put(kv.second.buf);
-  s->pool.put(s->counter);
   if (s->status_dev) (void)hipFree(s->status_dev);
   if (s->status_host) (void)hipHostFree(s->status_host);
   for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
@@ -919,10 +920,13 @@ static jxrhip_status submit_impl(jxrhip_session *s, const jxrhip_plan *p, void *
   args.tile_x = tx;
   args.tile_y = ty;
   args.lds_words = words;
+  args.storage = p->dev.storage;
   const uint32_t tiles = args.inline_slot.tiles_x * ((p->mby + ty - 1) / ty);
   args.item_count = tiles;
   HIP_TRY(hipMemsetAsync(status_word, 0, 4, sub->stream), "status reset");
-  HIP_TRY(launch_tiles(args, tiles, sub->stream), "reconstruction launch");
+  // one item per workgroup; a multiple of 8 workgroups lets the kernel give
+  // each XCD a contiguous share of the tiles
+  HIP_TRY(launch_tiles(args, (tiles + 7u) & ~7u, sub->stream), "reconstruction launch");
   HIP_TRY(hipMemcpyAsync(s->status_host + sub->slot, status_word, 4, hipMemcpyDeviceToHost, sub->stream),
           "status readback");
   HIP_TRY(hipEventRecord(sub->done, sub->stream), "hipEventRecord");
@@ -1153,7 +1157,7 @@ extern "C" jxrhip_status jxrhip_submit_batch(jxrhip_session *s, const jxrhip_pla
   std::vector<Slot> slots;
   struct Launch {
     size_t first;
-    uint32_t n, items, tx, ty, words;
+    uint32_t n, items, tx, ty, words, storage;
   };
   std::vector<Launch> launches;
   for (auto &kv : buckets) {
@@ -1166,7 +1170,7 @@ extern "C" jxrhip_status jxrhip_submit_batch(jxrhip_session *s, const jxrhip_pla
       words = std::max(words, w2);
     }
     if ((size_t)words * 4 > s->max_lds) return fail(JXRHIP_ERR_INVALID_PLAN, "tile does not fit in LDS");
-    Launch L{slots.size(), (uint32_t)kv.second.size(), 0, tx, ty, words};
+    Launch L{slots.size(), (uint32_t)kv.second.size(), 0, tx, ty, words, plans[kv.second[0]]->dev.storage};
     for (uint32_t i : kv.second) {
       Slot sl{};
       sl.img = dimgs[i];
@@ -1187,29 +1191,37 @@ extern "C" jxrhip_status jxrhip_submit_batch(jxrhip_session *s, const jxrhip_pla
   HIP_TRY(hipMemcpyAsync(slot_buf.ptr, slots.data(), slots.size() * sizeof(Slot), hipMemcpyHostToDevice, stream),
           "slot upload");
   HIP_TRY(hipStreamSynchronize(stream), "slot upload");
-  // counters: one per bucket
-  DeviceBuffer counters;
-  HIP_TRY(s->pool.get(std::max<size_t>(launches.size(), 1) * 64, counters, s->diag.scratch_hits, s->diag.scratch_misses),
-          "counters");
-  b->transient.push_back(counters);
+  // owning image of every work item, one table per launch
+  uint64_t total_items = 0;
+  for (const Launch &L : launches) total_items += L.items;
+  DeviceBuffer item_slots;
+  HIP_TRY(s->pool.get(std::max<uint64_t>(total_items, 1) * sizeof(uint32_t), item_slots, s->diag.scratch_hits,
+                      s->diag.scratch_misses),
+          "item table");
+  b->transient.push_back(item_slots);
   HIP_TRY(hipEventRecord(s->t0, stream), "hipEventRecord");
+  uint64_t item_base = 0;
   for (size_t li = 0; li < launches.size(); li++) {
     const Launch &L = launches[li];
     KernelArgs args{};
     args.slots = (const Slot *)slot_buf.ptr + L.first;
+    args.item_slot = (const uint32_t *)item_slots.ptr + item_base;
+    item_base += L.items;
     args.slot_count = L.n;
     args.item_count = L.items;
-    args.counter = (uint32_t *)((uint8_t *)counters.ptr + li * 64);
     args.persistent = 1;
     args.tile_x = L.tx;
     args.tile_y = L.ty;
     args.lds_words = L.words;
-    // workgroups a compute unit holds: 160 KiB of LDS, at most 8 groups
-    const uint32_t per_cu = s->groups_per_cu ? s->groups_per_cu
-                                              : std::max<uint32_t>(1, std::min<uint32_t>(8, (uint32_t)((160u * 1024u) / std::max<uint32_t>(L.words * 4, 1))));
+    args.storage = L.storage;
+    // workgroups a compute unit holds at once (LDS, registers), unless the
+    // session fixes it; a multiple of 8 workgroups for the XCD split
+    int resident = 0;
+    if (!s->groups_per_cu) HIP_TRY(kernel_occupancy(L.storage, (size_t)L.words * 4, &resident), "occupancy");
+    const uint32_t per_cu = s->groups_per_cu ? s->groups_per_cu : (uint32_t)std::max(resident, 1);
     uint32_t grid = std::min<uint32_t>(L.items, s->cus * per_cu);
-    grid = std::max<uint32_t>(grid, 1);
-    HIP_TRY(launch_reset(args.counter, stream), "counter reset");
+    grid = std::max<uint32_t>((grid + 7u) & ~7u, 8u);
+    HIP_TRY(launch_expand(args.slots, L.n, L.items, (uint32_t *)args.item_slot, stream), "item table");
     HIP_TRY(launch_tiles(args, grid, stream), "persistent launch");
     s->diag.kernel_launches++;
   }

@@ -96,3 +96,7 @@ hipError_t hipMemsetAsync(void *dst, int value, size_t bytes, hipStream_t) {
   return hipSuccess;
 }
 hipError_t hipFuncSetAttribute(const void *, hipFuncAttribute, int) { return hipSuccess; }
+hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *blocks, const void *, int, size_t lds) {
+  *blocks = (int)std::max<size_t>(1, std::min<size_t>(4, 65536 / (lds + 2048)));
+  return hipSuccess;
+}

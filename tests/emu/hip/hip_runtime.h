@@ -21,7 +21,7 @@
 #define __host__
 #define __forceinline__ inline
 #define __shared__ static
-#define __launch_bounds__(x)
+#define __launch_bounds__(...)
 #define JXR_GLOBAL
 
 struct dim3 {
@@ -120,6 +120,7 @@ hipError_t hipMemcpy(void *dst, const void *src, size_t bytes, hipMemcpyKind kin
 hipError_t hipMemcpyAsync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s);
 hipError_t hipMemsetAsync(void *dst, int value, size_t bytes, hipStream_t s);
 hipError_t hipFuncSetAttribute(const void *f, hipFuncAttribute attr, int value);
+hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *blocks, const void *f, int threads, size_t lds);
 
 void emu_begin_launch(size_t lds_bytes);
 void emu_end_launch();
