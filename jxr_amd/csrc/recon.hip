@@ -79,6 +79,7 @@ __device__ __forceinline__ MbRec load_mb(const MbRec *mbs, uint32_t i) {
 
 // ---- arithmetic policies ------------------------------------------------
 struct Fast {
+  static constexpr bool kFast = true;
   bool ovf = false;
   __device__ __forceinline__ int add(int a, int b) { return a + b; }
   __device__ __forceinline__ int sub(int a, int b) { return a - b; }
@@ -86,6 +87,7 @@ struct Fast {
   __device__ __forceinline__ int neg(int a) { return -a; }
 };
 struct Checked {
+  static constexpr bool kFast = false;
   bool ovf = false;
   __device__ __forceinline__ int add(int a, int b) {
     int r;
@@ -440,11 +442,13 @@ __device__ __forceinline__ void first_level_mb(const DevImage &im, const DevPlan
 // 4-tap edge strips across the split; slots split along both are four 2x2
 // corners (reconstruct/overlap.rs:124-323; device_plan/overlap.rs:125-178).
 // Only positions inside [0, width) x [0, height) are touched.
-__device__ __forceinline__ void overlap_slot(int *s, int rw, int ox, int oy, int kx, int ky, int width, int height,
-                             bool sx, bool sy, uint32_t *status) {
+// Returns the OR of magbits of the values written.
+__device__ __forceinline__ uint32_t overlap_slot(int *s, int rw, int ox, int oy, int kx, int ky, int width,
+                                                 int height, bool sx, bool sy, uint32_t *status) {
   const int x0 = 4 * kx + 2, y0 = 4 * ky + 2;
   int *base = s + (y0 - oy) * rw + (x0 - ox);
   bool ovf = false;
+  uint32_t omag = 0;
   if (!sx && !sy) {
     int v[16];
     uint32_t mag = 0;
@@ -461,6 +465,8 @@ __device__ __forceinline__ void overlap_slot(int *s, int rw, int ox, int oy, int
     for (int k = 0; k < 16; k++) mag |= magbits(v[k]);
     ovf = RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap4x4(v, a); });
 #pragma unroll
+    for (int k = 0; k < 16; k++) omag |= magbits(v[k]);
+#pragma unroll
     for (int r = 0; r < 4; r++) {
       *reinterpret_cast<int2 *>(base + r * rw) = make_int2(v[r * 4 + 0], v[r * 4 + 1]);
       *reinterpret_cast<int2 *>(base + r * rw + 2) = make_int2(v[r * 4 + 2], v[r * 4 + 3]);
@@ -475,6 +481,7 @@ __device__ __forceinline__ void overlap_slot(int *s, int rw, int ox, int oy, int
       int a0 = q[0], a1 = q[rw], a2 = q[2 * rw], a3 = q[3 * rw];
       const uint32_t mag = magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
       ovf |= RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap4(a0, a1, a2, a3, a); });
+      omag |= magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
       q[0] = a0;
       q[rw] = a1;
       q[2 * rw] = a2;
@@ -489,6 +496,7 @@ __device__ __forceinline__ void overlap_slot(int *s, int rw, int ox, int oy, int
       int a0 = q[0], a1 = q[1], a2 = q[2], a3 = q[3];
       const uint32_t mag = magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
       ovf |= RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap4(a0, a1, a2, a3, a); });
+      omag |= magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
       q[0] = a0;
       q[1] = a1;
       q[2] = a2;
@@ -505,6 +513,7 @@ __device__ __forceinline__ void overlap_slot(int *s, int rw, int ox, int oy, int
         int a0 = q[0], a1 = q[1], a2 = q[rw], a3 = q[rw + 1];
         const uint32_t mag = magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
         ovf |= RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap4(a0, a1, a2, a3, a); });
+        omag |= magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
         q[0] = a0;
         q[1] = a1;
         q[rw] = a2;
@@ -512,6 +521,7 @@ __device__ __forceinline__ void overlap_slot(int *s, int rw, int ox, int oy, int
       }
   }
   if (ovf) fail(status, JXRHIP_STAGE_OVERLAP);
+  return omag;
 }
 
 // ---- subsampled first-level overlap (subsampled_overlap.rs:110-283) -----
@@ -636,8 +646,10 @@ __device__ __forceinline__ BlockRange block_range(const DevPlane &p, const Plane
                     max(tmy * (int)p.br - 1, 0), min((tmy + ny) * (int)p.br + 1, (int)g.lh)};
 }
 
-__device__ __forceinline__ void second_level_block(const DevImage &im, const DevPlane &p, const PlaneGeo &g, const int *low,
-                                   int *smp, const BlockRange &br, DivMod t, uint32_t *status) {
+// Returns the OR of magbits of the block's sixteen samples.
+__device__ __forceinline__ uint32_t second_level_block(const DevImage &im, const DevPlane &p, const PlaneGeo &g,
+                                                       const int *low, int *smp, const BlockRange &br, DivMod t,
+                                                       uint32_t *status) {
   const int bx = br.bx0 + t.r, by = br.by0 + t.q;
   const int mx = bx >> g.bcs, my = by >> g.brs;
   const int cc = bx & ((int)g.bc - 1), rr = by & ((int)g.br - 1);
@@ -713,19 +725,37 @@ __device__ __forceinline__ void second_level_block(const DevImage &im, const Dev
   for (int y = 0; y < 4; y++)
     if ((uint32_t)(r0 + y) < g.srh)
       *reinterpret_cast<v4i *>(o + y * (int)g.srw) = v4i{v[y * 4], v[y * 4 + 1], v[y * 4 + 2], v[y * 4 + 3]};
+  uint32_t omag = 0;
+#pragma unroll
+  for (int j = 0; j < 16; j++) omag |= magbits(v[j]);
+  return omag;
 }
 
 // ---- output stage -------------------------------------------------------
+// Output arithmetic runs under the same policies as the transforms: Checked
+// follows the reference's checked i32 operations, Fast is plain i32 and
+// is chosen for a tile only when every sample of the tile is within 2^27
+// and the image's bias and rounding constants are below 2^29 in magnitude,
+// under which no conversion, scale or interpolation step leaves i32
+// (tests/test_bounds.py propagates the intervals).
+constexpr uint32_t kOutputSampleBound = 1u << 27;
+
+// (a*wa + b*wb + 4) >> 3 with wa + wb = 8 (sampling.rs:105-115). The
+// checked form splits off the low three bits so that no intermediate can
+// overflow for any i32 input; both forms are the same floor division.
+template <class A>
 __device__ __forceinline__ int weighted(int a, int wa, int b, int wb) {
-  // (a*wa + b*wb + 4) >> 3 exactly in i32 (sampling.rs:105-115)
-  return (a >> 3) * wa + (b >> 3) * wb + (((a & 7) * wa + (b & 7) * wb + 4) >> 3);
+  if constexpr (A::kFast) {
+    return (a * wa + b * wb + 4) >> 3;
+  } else {
+    return (a >> 3) * wa + (b >> 3) * wb + (((a & 7) * wa + (b & 7) * wb + 4) >> 3);
+  }
 }
-__device__ __forceinline__ int upsample_pick(int prev, int cur, int next, uint32_t centering,
-                                             int phase) {
-  // T.832 Table 181 weights [h0, h1, h2, h3]
-  const int h0 = 4 + (int)centering, h1 = 4 - (int)centering;
-  const int h2 = (int)centering, h3 = 8 - (int)centering;
-  return phase == 0 ? weighted(prev, h2, cur, h3) : weighted(cur, h0, next, h1);
+// T.832 Table 181 weights [h0, h1, h2, h3] = [4 + c, 4 - c, c, 8 - c]
+template <class A>
+__device__ __forceinline__ int upsample_pick(int prev, int cur, int next, int centering, int phase) {
+  return phase == 0 ? weighted<A>(prev, centering, cur, 8 - centering)
+                    : weighted<A>(cur, 4 + centering, next, 4 - centering);
 }
 
 // The output stage's view of a tile: the image descriptor and per-plane
@@ -737,57 +767,33 @@ struct OutCtx {
   const int *lds;
 };
 
-__device__ __forceinline__ int chroma_clamped(const OutCtx &o, int pl, int cx, int cy) {
-  const PlaneGeo &g = o.G[pl];
-  cx = min(max(cx, 0), (int)g.pw - 1);
-  cy = min(max(cy, 0), (int)g.ph - 1);
-  return o.lds[o.soff[pl] + (cy - g.soy) * (int)g.srw + (cx - g.sox)];
-}
-// scaling.rs:104-124; returns false on overflow
-__device__ __forceinline__ bool apply_scale(const ChanScale &s, int v, int &out) {
-  if (s.flags & 2u) {
-    out = 0;
-    return true;
-  }
-  if (s.flags & 1u) return false;
-  int b, r;
-  if (__builtin_add_overflow(v, s.shifted_bias, &b)) return false;
-  if (__builtin_add_overflow(b, s.rounding, &r)) return false;
-  int q = r >> s.shift;
-  if (s.flags & 4u) {
-    long long w = (long long)q * (1ll << (s.post_shift > 31 ? 31 : s.post_shift));
+// scaling.rs:104-124. The saturating post-shift is not an error path.
+template <class A>
+__device__ __forceinline__ int apply_scale(const ChanScale &s, int v, A &a) {
+  if (s.flags & 2u) return 0;
+  int q = a.add(a.add(v, s.shifted_bias), s.rounding) >> s.shift;
+  if ((s.flags & 4u) && s.post_shift) {
+    const long long w = (long long)q * (1ll << (s.post_shift > 31 ? 31 : s.post_shift));
     q = w > INT32_MAX ? INT32_MAX : (w < INT32_MIN ? INT32_MIN : (int)w);
   }
-  out = q;
-  return true;
+  return q;
 }
 
-__device__ __forceinline__ uint32_t f16_bits(int v, bool scaled, bool &ok) {
-  // F16 depth: no bias, rounding 3 when scaled (scaling.rs:233-240)
-  int q;
-  if (scaled) {
-    int r;
-    ok &= !__builtin_add_overflow(v, 3, &r);
-    q = r >> 3;
-  } else {
-    q = v;
-  }
-  uint32_t sign = q < 0 ? 0x8000u : 0u;
-  long long m = q < 0 ? -(long long)q : (long long)q;
-  if (m > 32767) m = 32767;
-  return sign | (uint32_t)m;
+// F16 depth: no bias, rounding 3 when scaled (scaling.rs:233-240)
+template <class A>
+__device__ __forceinline__ uint32_t f16_bits(int v, bool scaled, A &a) {
+  const int q = scaled ? a.add(v, 3) >> 3 : v;
+  const uint32_t sign = q < 0 ? 0x8000u : 0u;
+  uint32_t m = q < 0 ? 0u - (uint32_t)q : (uint32_t)q;
+  if (m > 32767u) m = 32767u;
+  return sign | m;
 }
 
-__device__ __forceinline__ uint32_t f32_bits(int v, bool scaled, uint32_t length, int bias, bool &ok) {
-  // scaling.rs:254-296, with the CPU's clip to f32::MAX on exponent overflow
-  int q;
-  if (scaled) {
-    int r;
-    ok &= !__builtin_add_overflow(v, 3, &r);
-    q = r >> 3;
-  } else {
-    q = v;
-  }
+// scaling.rs:254-296, with the CPU's clip to f32::MAX on exponent overflow;
+// `ok` is cleared when the mantissa does not fit the f32 field.
+template <class A>
+__device__ __forceinline__ uint32_t f32_bits(int v, bool scaled, uint32_t length, int bias, A &a, bool &ok) {
+  const int q = scaled ? a.add(v, 3) >> 3 : v;
   const uint32_t sign = q < 0 ? 0x80000000u : 0u;
   const unsigned long long mag = q < 0 ? (unsigned long long)(-(long long)q) : (unsigned long long)q;
   const unsigned long long implicit = 1ull << length;
@@ -841,9 +847,10 @@ __device__ __forceinline__ bool pm_signed(int v, int a, int &out) {
 }
 
 // Converted, ordered and formatted samples of one pixel (packing.rs:333-357,
-// premultiply.rs); returns false on arithmetic overflow.
-template <uint32_t STORE>
-__device__ __forceinline__ bool format_pixel(const DevImage &im, const int in[4], int alpha, uint32_t stored[6]) {
+// premultiply.rs). Arithmetic overflow is recorded in the policy; the return
+// value is false for the other failures (f32 mantissa, i32 premultiply).
+template <uint32_t STORE, class A>
+__device__ __forceinline__ bool format_pixel(const DevImage &im, const int in[4], int alpha, uint32_t stored[6], A &a) {
   bool ok = true;
   int conv[4] = {0, 0, 0, 0};
   switch (im.conv) {
@@ -851,28 +858,21 @@ __device__ __forceinline__ bool format_pixel(const DevImage &im, const int in[4]
     case kConvLumaRgb: conv[0] = conv[1] = conv[2] = in[0]; break;
     case kConvYuvRgb: {
       // output_format/color.rs:100-110
-      int t0, g, rr, b, s;
-      ok &= !__builtin_sub_overflow(0, in[1], &t0);
-      ok &= !__builtin_sub_overflow(in[0], t0 >> 1, &g);
-      ok &= !__builtin_add_overflow(t0, g, &s);
-      ok &= !__builtin_sub_overflow(s, (in[2] >> 1) + (in[2] & 1), &rr);
-      ok &= !__builtin_add_overflow(in[2], rr, &b);
+      const int t0 = a.neg(in[1]);
+      const int g = a.sub(in[0], t0 >> 1);
+      const int rr = a.sub(a.add(t0, g), (in[2] >> 1) + (in[2] & 1));
       conv[0] = rr;
       conv[1] = g;
-      conv[2] = b;
+      conv[2] = a.add(in[2], rr);
       break;
     }
     case kConvYuvkCmyk: {
-      int black, m1, magenta, c1, cyan, yellow;
-      ok &= !__builtin_add_overflow(in[3], in[0] >> 1, &black);
-      ok &= !__builtin_sub_overflow(black, in[0], &m1);
-      ok &= !__builtin_sub_overflow(m1, in[1] >> 1, &magenta);
-      ok &= !__builtin_add_overflow(in[1], magenta, &c1);
-      ok &= !__builtin_add_overflow(c1, in[2] >> 1, &cyan);
-      ok &= !__builtin_sub_overflow(cyan, in[2], &yellow);
+      const int black = a.add(in[3], in[0] >> 1);
+      const int magenta = a.sub(a.sub(black, in[0]), in[1] >> 1);
+      const int cyan = a.add(a.add(in[1], magenta), in[2] >> 1);
       conv[0] = cyan;
       conv[1] = magenta;
-      conv[2] = yellow;
+      conv[2] = a.sub(cyan, in[2]);
       conv[3] = black;
       break;
     }
@@ -911,15 +911,15 @@ __device__ __forceinline__ bool format_pixel(const DevImage &im, const int in[4]
     const bool is_alpha = im.has_alpha && c == pc;
     const ChanScale &cs = is_alpha ? im.alpha_scale : im.color_scale[c];
     if constexpr (STORE == JXRHIP_STORE_F16) {
-      stored[c] = (cs.flags & 2u) ? 0u : f16_bits(ord[c], is_alpha ? im.f_alpha_scaled : im.f_scaled, ok);
+      stored[c] = (cs.flags & 2u) ? 0u : f16_bits(ord[c], is_alpha ? im.f_alpha_scaled : im.f_scaled, a);
     } else if constexpr (STORE == JXRHIP_STORE_F32) {
       stored[c] = (cs.flags & 2u) ? 0u
                                   : f32_bits(ord[c], is_alpha ? im.f_alpha_scaled : im.f_scaled,
                                              is_alpha ? im.f_alpha_mant : im.f_mant,
-                                             is_alpha ? im.f_alpha_bias : im.f_bias, ok);
+                                             is_alpha ? im.f_alpha_bias : im.f_bias, a, ok);
     } else {
-      int v = 0;
-      ok &= apply_scale(cs, ord[c], v);
+      ok &= (cs.flags & 2u) || !(cs.flags & 1u);  // padding first, then an unresolvable scale
+      int v = apply_scale(cs, ord[c], a);
       if constexpr (STORE == JXRHIP_STORE_U8) v = min(max(v, 0), 255);
       else if constexpr (STORE == JXRHIP_STORE_U16) v = min(max(v, 0), im.u16_max);
       else if constexpr (STORE == JXRHIP_STORE_I16) v = min(max(v, -32768), 32767);
@@ -928,30 +928,30 @@ __device__ __forceinline__ bool format_pixel(const DevImage &im, const int in[4]
   }
   if (im.premultiply && im.has_alpha) {
     const uint32_t ai = im.channels - 1;
-    uint32_t a = 0;
+    uint32_t al = 0;
 #pragma unroll
     for (uint32_t c = 0; c < 6; c++)
-      if (c == ai) a = stored[c];
+      if (c == ai) al = stored[c];
 #pragma unroll
     for (uint32_t c = 0; c < 5; c++) {
       if (c >= ai) continue;
       if constexpr (STORE == JXRHIP_STORE_U8) {
-        stored[c] = pm_unsigned<255>(stored[c], a);
+        stored[c] = pm_unsigned<255>(stored[c], al);
       } else if constexpr (STORE == JXRHIP_STORE_U16) {
-        stored[c] = pm_unsigned<65535>(stored[c], a);
+        stored[c] = pm_unsigned<65535>(stored[c], al);
       } else if constexpr (STORE == JXRHIP_STORE_I16) {
         int r = 0;
-        ok &= pm_signed<32767>((int16_t)stored[c], (int16_t)a, r);
+        ok &= pm_signed<32767>((int16_t)stored[c], (int16_t)al, r);
         stored[c] = (uint32_t)r;
       } else if constexpr (STORE == JXRHIP_STORE_I32) {
         int r = 0;
-        ok &= pm_signed<INT32_MAX>((int)stored[c], (int)a, r);
+        ok &= pm_signed<INT32_MAX>((int)stored[c], (int)al, r);
         stored[c] = (uint32_t)r;
       } else if constexpr (STORE == JXRHIP_STORE_F16) {
-        const uint32_t am = (a & 0x8000u) ? 0u : (a & 0x7fffu);
+        const uint32_t am = (al & 0x8000u) ? 0u : (al & 0x7fffu);
         stored[c] = (stored[c] & 0x8000u) | pm_unsigned<0x7fff>(stored[c] & 0x7fffu, am);
       } else {
-        const float af = __uint_as_float(a);
+        const float af = __uint_as_float(al);
         const float ac = af < 0.0f ? 0.0f : (af > 1.0f ? 1.0f : af);
         stored[c] = __float_as_uint(__fmul_rn(__uint_as_float(stored[c]), ac));
       }
@@ -1050,47 +1050,59 @@ __device__ __forceinline__ v4i sample_quad(const OutCtx &o, int pl, int x, int y
 // One output group: pixels x = 4gx..4gx+3, y = 2gy..2gy+1 (window coords)
 // inside [x0, x1) x [y0, y1). 4:2:0 chroma is interpolated once for the
 // group: three chroma rows and four chroma columns give both rows' values.
-template <uint32_t STORE>
+// Chroma reads clamp to the plane edge (sampling.rs); only the outer column
+// and row of the group's chroma window can leave the plane, so four
+// coordinates are clamped per group.
+template <uint32_t STORE, class A>
 __device__ __forceinline__ void output_group(const OutCtx &o, JXR_GLOBAL uint8_t *out, int gx, int gy, int x0,
                                              int x1, int y0, int y1, bool &ok) {
   constexpr uint32_t B = StoreBytes<STORE>::value;
   const DevImage &im = *o.im;
   const int xb = 4 * gx, yb = 2 * gy;
   int chroma[2][2][4];  // [U/V][row][pixel]
-  if (im.upsample == 1) {
+  if (im.upsample) {
+    const PlaneGeo &gc = o.G[1];  // both chroma planes share the geometry
+    const int cx = 2 * gx - gc.sox;
+    const int col[4] = {max(2 * gx - 1, 0) - gc.sox, cx, cx + 1, min(2 * gx + 2, (int)gc.pw - 1) - gc.sox};
+    const int cx_c = (int)im.centering_x, cy_c = (int)im.centering_y;
+    if (im.upsample == 1) {
+      const int srw = (int)gc.srw;
+      const int row[3] = {(max(gy - 1, 0) - gc.soy) * srw, (gy - gc.soy) * srw,
+                          (min(gy + 1, (int)gc.ph - 1) - gc.soy) * srw};
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
-      int vert[2][4];
+      for (int q = 0; q < 2; q++) {
+        const int *base = o.lds + o.soff[1 + q];
+        int vert[2][4];
 #pragma unroll
-      for (int c = 0; c < 4; c++) {
-        const int cx = 2 * gx - 1 + c;
-        const int a = chroma_clamped(o, 1 + q, cx, gy - 1), b = chroma_clamped(o, 1 + q, cx, gy),
-                  d = chroma_clamped(o, 1 + q, cx, gy + 1);
-        vert[0][c] = upsample_pick(a, b, d, im.centering_y, 0);
-        vert[1][c] = upsample_pick(a, b, d, im.centering_y, 1);
+        for (int c = 0; c < 4; c++) {
+          const int a = base[row[0] + col[c]], b = base[row[1] + col[c]], d = base[row[2] + col[c]];
+          vert[0][c] = upsample_pick<A>(a, b, d, cy_c, 0);
+          vert[1][c] = upsample_pick<A>(a, b, d, cy_c, 1);
+        }
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const int k = i >> 1;
+            chroma[q][j][i] = upsample_pick<A>(vert[j][k], vert[j][k + 1], vert[j][k + 2], cx_c, i & 1);
+          }
       }
+    } else {
 #pragma unroll
-      for (int j = 0; j < 2; j++)
+      for (int q = 0; q < 2; q++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int k = i >> 1;
-          chroma[q][j][i] = upsample_pick(vert[j][k], vert[j][k + 1], vert[j][k + 2], im.centering_x, i & 1);
+        for (int j = 0; j < 2; j++) {
+          const int *base = o.lds + o.soff[1 + q] + (yb + j - gc.soy) * (int)gc.srw;
+          int h[4];
+#pragma unroll
+          for (int c = 0; c < 4; c++) h[c] = base[col[c]];
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const int k = i >> 1;
+            chroma[q][j][i] = upsample_pick<A>(h[k], h[k + 1], h[k + 2], cx_c, i & 1);
+          }
         }
     }
-  } else if (im.upsample == 2) {
-#pragma unroll
-    for (int q = 0; q < 2; q++)
-#pragma unroll
-      for (int j = 0; j < 2; j++) {
-        int h[4];
-#pragma unroll
-        for (int c = 0; c < 4; c++) h[c] = chroma_clamped(o, 1 + q, 2 * gx - 1 + c, yb + j);
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int k = i >> 1;
-          chroma[q][j][i] = upsample_pick(h[k], h[k + 1], h[k + 2], im.centering_x, i & 1);
-        }
-      }
   }
   uint32_t valid = 0;
 #pragma unroll
@@ -1121,8 +1133,9 @@ __device__ __forceinline__ void output_group(const OutCtx &o, JXR_GLOBAL uint8_t
     for (int i = 0; i < 4; i++) {
       const int in[4] = {l[i], u[i], v[i], k[i]};
       uint32_t px[6];
-      const bool good = format_pixel<STORE>(im, in, al[i], px);
-      ok &= good || !(valid & (1u << i));
+      A ar;  // pixels outside the crop are formatted but never fail
+      const bool good = format_pixel<STORE>(im, in, al[i], px, ar);
+      ok &= (good && !ar.ovf) || !(valid & (1u << i));
 #pragma unroll
       for (int c = 0; c < 6; c++) st[c][i] = px[c];
     }
@@ -1157,14 +1170,14 @@ __device__ __forceinline__ void output_group(const OutCtx &o, JXR_GLOBAL uint8_t
   }
 }
 
-template <uint32_t STORE>
+template <uint32_t STORE, class A>
 __device__ __forceinline__ void output_tile(const OutCtx &o, JXR_GLOBAL uint8_t *out, int gx0, int gy0, int ngx,
                                             int ngy, int x0, int x1, int y0, int y1, uint32_t *status) {
   const uint32_t m = div_magic((uint32_t)ngx);
   bool ok = true;
   for (int t = threadIdx.x; t < ngx * ngy; t += blockDim.x) {
     const DivMod u = divmod(t, (uint32_t)ngx, m);
-    output_group<STORE>(o, out, gx0 + u.r, gy0 + u.q, x0, x1, y0, y1, ok);
+    output_group<STORE, A>(o, out, gx0 + u.r, gy0 + u.q, x0, x1, y0, y1, ok);
   }
   if (!ok) fail(status, JXRHIP_STAGE_OUTPUT);
 }
@@ -1192,7 +1205,8 @@ struct TileShared {
   int sA[kMaxPlanes + 1], sB1[kMaxPlanes + 1], sB2[kMaxPlanes + 1], sB3[kMaxPlanes + 1];
   int sC[kMaxPlanes + 1], sD[kMaxPlanes + 1];
   int tmx, tmy, nx, ny;
-  int any_lp, any_sub, any_pot, pad;
+  int any_lp, any_sub, any_pot;
+  uint32_t out_mag;  // OR of magbits of every sample the tile's output stage can read
 };
 
 __device__ __forceinline__ int plane_of(const int *start, int np, int t) {
@@ -1284,6 +1298,7 @@ __device__ __forceinline__ void tile_finish(TileShared &ts) {
   ts.any_lp = any_lp;
   ts.any_sub = any_sub;
   ts.any_pot = any_pot;
+  ts.out_mag = 0;
   ts.sA[0] = ts.sB1[0] = ts.sB2[0] = ts.sB3[0] = ts.sC[0] = ts.sD[0] = 0;
   for (int pl = 0; pl < np; pl++) {
     ts.sA[pl + 1] += ts.sA[pl];
@@ -1363,11 +1378,14 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
   }
 
   // stage C: high-pass prediction and second level, tile blocks + halo
+  uint32_t omag = 0;  // magnitude of every sample this thread writes in stages C and D
   for (int t = threadIdx.x; t < ts.sC[np]; t += blockDim.x) {
     const int pl = plane_of(ts.sC, np, t);
     const DivMod u = divmod(t - ts.sC[pl], ts.w[pl].wC, ts.w[pl].mC);
-    second_level_block(im, im.planes[pl], ts.g[pl], lds + ts.loff[pl], lds + ts.soff[pl], ts.br[pl], u, status);
+    omag |= second_level_block(im, im.planes[pl], ts.g[pl], lds + ts.loff[pl], lds + ts.soff[pl], ts.br[pl], u,
+                               status);
   }
+  if (!ts.any_pot) atomicOr(&ts.out_mag, omag);
   __syncthreads();
 
   // stage D: sample-level overlap (modes 1, 2)
@@ -1381,8 +1399,10 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
       const int px = 4 * kx + 4, py = 4 * ky + 4;
       const bool xs = (px & ((int)g.mw - 1)) == 0 && col_boundary(p, im.mbx, (uint32_t)(px >> (g.bcs + 2)));
       const bool ys = (py & ((int)g.mh - 1)) == 0 && row_boundary(p, im.mby, (uint32_t)(py >> (g.brs + 2)));
-      overlap_slot(lds + ts.soff[pl], (int)g.srw, g.sox, g.soy, kx, ky, (int)g.pw, (int)g.ph, xs, ys, status);
+      omag |= overlap_slot(lds + ts.soff[pl], (int)g.srw, g.sox, g.soy, kx, ky, (int)g.pw, (int)g.ph, xs, ys,
+                           status);
     }
+    atomicOr(&ts.out_mag, omag);
     __syncthreads();
   }
 
@@ -1394,7 +1414,11 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
   const int gy0 = y0 >> 1, gy1 = ((y1 - 1) >> 1) + 1;
   const int ngx = gx1 - gx0, ngy = gy1 - gy0;
   const OutCtx o{&im, ts.g, ts.soff, lds};
-  output_tile<STORE>(o, gptr_mut(slot.out), gx0, gy0, ngx, ngy, x0, x1, y0, y1, status);
+  // plain arithmetic when no output step can overflow for this tile
+  if (im.out_fast && ts.out_mag < kOutputSampleBound)
+    output_tile<STORE, Fast>(o, gptr_mut(slot.out), gx0, gy0, ngx, ngy, x0, x1, y0, y1, status);
+  else
+    output_tile<STORE, Checked>(o, gptr_mut(slot.out), gx0, gy0, ngx, ngy, x0, x1, y0, y1, status);
 }
 
 // Work assignment: workgroup b takes an arithmetic sequence of work items.

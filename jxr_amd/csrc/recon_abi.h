@@ -77,7 +77,8 @@ struct alignas(64) DevImage {
   uint32_t f_scaled, f_alpha_scaled;       // F16/F32 scaled flags
   uint32_t f_mant, f_alpha_mant;           // F32 mantissa lengths
   int32_t f_bias, f_alpha_bias;            // F32 exponent biases
-  uint32_t pad0, pad1;
+  uint32_t out_fast;       // every channel's |bias| + rounding < 2^29: plain output arithmetic allowed
+  uint32_t pad0;
   uint64_t plane_offset[kMaxPlanes];       // output surface planes
   uint64_t row_stride[kMaxPlanes];
   ChanScale color_scale[8];

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstdlib>
 #include <cstring>
 #include <list>
 #include <map>
@@ -400,6 +401,17 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
     }
     dv.alpha_scale = make_scale(0, JXRHIP_COLOR_LUMA, adep, f.has_alpha_format ? f.alpha_shift_bits : f.shift_bits,
                                 f.has_alpha_format ? f.alpha_scaled : f.scaled);
+  }
+  {
+    // output arithmetic needs no overflow checks for samples within 2^27
+    // when every bias and rounding constant is below 2^29 in magnitude
+    bool small = true;
+    auto check = [&](const ChanScale &c) {
+      small &= (int64_t)std::llabs((long long)c.shifted_bias) + std::llabs((long long)c.rounding) < (1ll << 29);
+    };
+    for (const ChanScale &c : dv.color_scale) check(c);
+    check(dv.alpha_scale);
+    dv.out_fast = small ? 1 : 0;
   }
   dv.premultiply = f.premultiply_alpha ? 1 : 0;
   dv.centering_x = f.chroma_centering_x;

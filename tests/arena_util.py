@@ -63,6 +63,31 @@ class ManualArena:
         self.mbx, self.mby = mbx, mby
 
 
+class FullArena(ManualArena):
+    """Arena of full macroblocks (all bands, no prediction, steps 1):
+    planes is a list of arrays (mby, mbx, 256) holding each macroblock's
+    coefficients block by block (block k at 16k, its DC/LP at 16k)."""
+
+    def __init__(self, planes, overlap=0):
+        mby, mbx = planes[0].shape[:2]
+        n = len(planes)
+        super().__init__([np.zeros((mby, mbx), np.int64)] * n, overlap=overlap)
+        self.coef = np.concatenate([np.asarray(p, np.int64).reshape(-1) for p in planes]).astype(np.int32)
+        total = n * mbx * mby
+        self.offsets = (np.arange(total, dtype=np.uint64) * 256).astype(np.uint32)
+        self.bands = np.full(total, abi.BANDS_ALL, np.uint8)
+        for i in range(n):
+            cp = self.planes[i]
+            cp.coefficient_offset = i * mbx * mby * 256
+            cp.coefficient_count = mbx * mby * 256
+        a = self.arena
+        a.coefficients = self.coef.ctypes.data_as(C.POINTER(C.c_int32))
+        a.coefficient_count = self.coef.size
+        a.coefficient_offsets = self.offsets.ctypes.data_as(abi.u32p)
+        a.bands = self.bands.ctypes.data_as(C.POINTER(C.c_uint8))
+        a.identity = next(_ids)
+
+
 def desc_for(arena: ManualArena, color, components, spec: OutputSpec, crop, scaled=False,
              alpha_mode=abi.ALPHA_NONE, separate_alpha: ManualArena = None, alpha_scaled=None):
     d = abi.ImageDesc()

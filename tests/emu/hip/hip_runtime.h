@@ -57,6 +57,11 @@ inline unsigned atomicCAS(unsigned *p, unsigned cmp, unsigned val) {
   if (old == cmp) *p = val;
   return old;
 }
+inline unsigned atomicOr(unsigned *p, unsigned v) {
+  unsigned old = *p;
+  *p = old | v;
+  return old;
+}
 inline unsigned atomicAdd(unsigned *p, unsigned v) {
   unsigned old = *p;
   *p = old + v;

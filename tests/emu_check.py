@@ -39,6 +39,28 @@ def main():
                 print(f"MISMATCH {name} tile {tile}: {bad.size} bytes, first {bad[:6]}", flush=True)
                 failures += 1
     session.set_tile(0, 0)
+    # samples beyond the output stage's plain-arithmetic bound
+    import large_cases
+    for tile in tiles:
+        session.set_tile(*tile)
+        for name, desc, stage in large_cases.cases():
+            if only and name not in only:
+                continue
+            plan = HipDecodePlan(desc, keepalive=desc._keep)
+            if stage:
+                try:
+                    session.decode_to_host(plan)
+                    print(f"MISSING ERROR {name} tile {tile}", flush=True)
+                    failures += 1
+                except Exception as e:  # noqa: BLE001
+                    if getattr(e, "stage", None) != stage:
+                        print(f"WRONG ERROR {name} tile {tile}: {e}", flush=True)
+                        failures += 1
+                continue
+            if not np.array_equal(session.decode_to_host(plan), O.decode(desc)):
+                print(f"MISMATCH {name} tile {tile}", flush=True)
+                failures += 1
+    session.set_tile(0, 0)
     # persistent batch over every case at once
     encs, plans, wants = [], [], []
     for name in names:

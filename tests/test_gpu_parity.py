@@ -143,3 +143,19 @@ def test_batch_sibling_errors_are_indexed(session):
     want = _expected(enc_ok, spec_ok)
     assert np.array_equal(bufs[0].to_host(), want)
     assert np.array_equal(bufs[2].to_host(), want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("index", range(9))
+def test_large_samples_take_checked_output_path(session, index):
+    """Samples above 2^27 run the output stage with checked arithmetic;
+    results and the output-stage overflow match the oracle."""
+    import large_cases
+    name, desc, stage = large_cases.cases()[index]
+    plan = HipDecodePlan(desc, keepalive=desc._keep)
+    if stage:
+        with pytest.raises(JxrHipError) as err:
+            session.decode_to_host(plan)
+        assert err.value.code == abi.ERR_KERNEL_ARITHMETIC and err.value.stage == stage, name
+    else:
+        assert np.array_equal(session.decode_to_host(plan), O.decode(desc)), name

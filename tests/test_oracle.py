@@ -394,3 +394,16 @@ def test_invalid_requests_fail_before_decoding():
     d = desc_for(arena, abi.COLOR_LUMA, 1, LUMA8, (8, 8, 16, 16))
     code, _, _ = O.decode_status(d)  # crop outside the plane
     assert code == abi.ERR_INVALID_PLAN
+
+
+def test_large_sample_cases_reach_beyond_the_plain_output_bound():
+    """The hand-built large-sample arenas decode (or fail in the output
+    stage) as their GPU tests assume: samples above 2^27 and, for the last,
+    an overflow first met in the colour conversion."""
+    import large_cases
+    for name, desc, stage in large_cases.cases():
+        code, st, _ = O.decode_status(desc)
+        assert (code, st) == ((abi.ERR_KERNEL_ARITHMETIC, stage) if stage else (0, 0)), name
+        peak = max(int(np.abs(O.reconstruct_plane(desc, i).astype(np.int64)).max())
+                   for i in range(desc.component_count))
+        assert peak > 2 ** 27, name
