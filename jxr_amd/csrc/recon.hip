@@ -746,7 +746,8 @@ constexpr uint32_t kOutputSampleBound = 1u << 27;
 template <class A>
 __device__ __forceinline__ int weighted(int a, int wa, int b, int wb) {
   if constexpr (A::kFast) {
-    return (a * wa + b * wb + 4) >> 3;
+    // wb = 8 - wa: (8b + (a - b) * wa + 4) >> 3, one multiply
+    return b + (((a - b) * wa + 4) >> 3);
   } else {
     return (a >> 3) * wa + (b >> 3) * wb + (((a & 7) * wa + (b & 7) * wb + 4) >> 3);
   }
@@ -1047,6 +1048,84 @@ __device__ __forceinline__ v4i sample_quad(const OutCtx &o, int pl, int x, int y
   return *reinterpret_cast<const v4i *>(o.lds + o.soff[pl] + (y - g.soy) * (int)g.srw + (x - g.sox));
 }
 
+// Upsampled chroma of one output group: chroma[U/V][row][pixel] for pixels
+// 4gx..4gx+3 of rows 2gy, 2gy+1 (sampling.rs: vertical pass, then
+// horizontal, edge samples clamped). Only the outer column and row of the
+// group's chroma window can leave the plane, so four coordinates are clamped.
+// With centering 0 the weights are [4, 4, 0, 8]: even positions copy a
+// sample and odd ones average two, which plain arithmetic does exactly.
+template <class A>
+__device__ __forceinline__ void group_chroma(const OutCtx &o, int gx, int gy, int chroma[2][2][4]) {
+  const DevImage &im = *o.im;
+  const PlaneGeo &gc = o.G[1];  // both chroma planes share the geometry
+  const int cx = 2 * gx - gc.sox;
+  const int srw = (int)gc.srw;
+  const int cx_c = (int)im.centering_x, cy_c = (int)im.centering_y;
+  const int c3 = min(2 * gx + 2, (int)gc.pw - 1) - gc.sox;
+  if (im.upsample == 1) {
+    const int r1 = (gy - gc.soy) * srw, r2 = (min(gy + 1, (int)gc.ph - 1) - gc.soy) * srw;
+    if (A::kFast && cx_c == 0 && cy_c == 0) {
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        const int *base = o.lds + o.soff[1 + q];
+        int v0[3], v1[3];  // rows 2gy and 2gy+1 at chroma columns cx, cx + 1, cx + 2
+        const int cols[3] = {cx, cx + 1, c3};
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const int b = base[r1 + cols[c]], d = base[r2 + cols[c]];
+          v0[c] = b;
+          v1[c] = (b + d + 1) >> 1;
+        }
+        chroma[q][0][0] = v0[0];
+        chroma[q][0][1] = (v0[0] + v0[1] + 1) >> 1;
+        chroma[q][0][2] = v0[1];
+        chroma[q][0][3] = (v0[1] + v0[2] + 1) >> 1;
+        chroma[q][1][0] = v1[0];
+        chroma[q][1][1] = (v1[0] + v1[1] + 1) >> 1;
+        chroma[q][1][2] = v1[1];
+        chroma[q][1][3] = (v1[1] + v1[2] + 1) >> 1;
+      }
+      return;
+    }
+    const int col[4] = {max(2 * gx - 1, 0) - gc.sox, cx, cx + 1, c3};
+    const int r0 = (max(gy - 1, 0) - gc.soy) * srw;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int *base = o.lds + o.soff[1 + q];
+      int vert[2][4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const int a = base[r0 + col[c]], b = base[r1 + col[c]], d = base[r2 + col[c]];
+        vert[0][c] = upsample_pick<A>(a, b, d, cy_c, 0);
+        vert[1][c] = upsample_pick<A>(a, b, d, cy_c, 1);
+      }
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int k = i >> 1;
+          chroma[q][j][i] = upsample_pick<A>(vert[j][k], vert[j][k + 1], vert[j][k + 2], cx_c, i & 1);
+        }
+    }
+  } else {
+    const int col[4] = {max(2 * gx - 1, 0) - gc.sox, cx, cx + 1, c3};
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const int *base = o.lds + o.soff[1 + q] + (2 * gy + j - gc.soy) * srw;
+        int h[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) h[c] = base[col[c]];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int k = i >> 1;
+          chroma[q][j][i] = upsample_pick<A>(h[k], h[k + 1], h[k + 2], cx_c, i & 1);
+        }
+      }
+  }
+}
+
 // One output group: pixels x = 4gx..4gx+3, y = 2gy..2gy+1 (window coords)
 // inside [x0, x1) x [y0, y1). 4:2:0 chroma is interpolated once for the
 // group: three chroma rows and four chroma columns give both rows' values.
@@ -1060,50 +1139,7 @@ __device__ __forceinline__ void output_group(const OutCtx &o, JXR_GLOBAL uint8_t
   const DevImage &im = *o.im;
   const int xb = 4 * gx, yb = 2 * gy;
   int chroma[2][2][4];  // [U/V][row][pixel]
-  if (im.upsample) {
-    const PlaneGeo &gc = o.G[1];  // both chroma planes share the geometry
-    const int cx = 2 * gx - gc.sox;
-    const int col[4] = {max(2 * gx - 1, 0) - gc.sox, cx, cx + 1, min(2 * gx + 2, (int)gc.pw - 1) - gc.sox};
-    const int cx_c = (int)im.centering_x, cy_c = (int)im.centering_y;
-    if (im.upsample == 1) {
-      const int srw = (int)gc.srw;
-      const int row[3] = {(max(gy - 1, 0) - gc.soy) * srw, (gy - gc.soy) * srw,
-                          (min(gy + 1, (int)gc.ph - 1) - gc.soy) * srw};
-#pragma unroll
-      for (int q = 0; q < 2; q++) {
-        const int *base = o.lds + o.soff[1 + q];
-        int vert[2][4];
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-          const int a = base[row[0] + col[c]], b = base[row[1] + col[c]], d = base[row[2] + col[c]];
-          vert[0][c] = upsample_pick<A>(a, b, d, cy_c, 0);
-          vert[1][c] = upsample_pick<A>(a, b, d, cy_c, 1);
-        }
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-#pragma unroll
-          for (int i = 0; i < 4; i++) {
-            const int k = i >> 1;
-            chroma[q][j][i] = upsample_pick<A>(vert[j][k], vert[j][k + 1], vert[j][k + 2], cx_c, i & 1);
-          }
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < 2; q++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-          const int *base = o.lds + o.soff[1 + q] + (yb + j - gc.soy) * (int)gc.srw;
-          int h[4];
-#pragma unroll
-          for (int c = 0; c < 4; c++) h[c] = base[col[c]];
-#pragma unroll
-          for (int i = 0; i < 4; i++) {
-            const int k = i >> 1;
-            chroma[q][j][i] = upsample_pick<A>(h[k], h[k + 1], h[k + 2], cx_c, i & 1);
-          }
-        }
-    }
-  }
+  if (im.upsample) group_chroma<A>(o, gx, gy, chroma);
   uint32_t valid = 0;
 #pragma unroll
   for (int i = 0; i < 4; i++) valid |= (xb + i >= x0 && xb + i < x1) ? 1u << i : 0u;
@@ -1180,6 +1216,139 @@ __device__ __forceinline__ void output_tile(const OutCtx &o, JXR_GLOBAL uint8_t 
     output_group<STORE, A>(o, out, gx0 + u.r, gy0 + u.q, x0, x1, y0, y1, ok);
   }
   if (!ok) fail(status, JXRHIP_STAGE_OUTPUT);
+}
+
+// ---- specialised output recipes (plain-arithmetic tiles) -------------------
+// Every channel shares one bias + rounding, shift and clamp, so a stored
+// sample is clamp((v + add) >> shift): the generic path's operations in the
+// same order, without the per-channel dispatch.
+struct RecipeScale {
+  int add, lo, hi;
+  uint32_t shift;
+};
+__device__ __forceinline__ uint32_t recipe_store(const RecipeScale &r, int v) {
+  return (uint32_t)min(max((v + r.add) >> r.shift, r.lo), r.hi);
+}
+
+// Four pixels of three interleaved or planar channels (c[0..2] in stored
+// order) on output row oy.
+template <uint32_t B>
+__device__ __forceinline__ void store_rgb_row(const DevImage &im, JXR_GLOBAL uint8_t *out, uint32_t oy, int ox0,
+                                              uint32_t valid, const uint32_t c[3][4]) {
+  if (im.planar) {
+#pragma unroll
+    for (uint32_t k = 0; k < 3; k++) {
+      JXR_GLOBAL uint8_t *row = out + im.plane_offset[k] + (uint64_t)oy * im.row_stride[k];
+      store_run<B>(row + (int64_t)ox0 * B, c[k], valid);
+    }
+    return;
+  }
+  JXR_GLOBAL uint8_t *px0 = out + im.plane_offset[0] + (uint64_t)oy * im.row_stride[0] + (int64_t)ox0 * 3 * B;
+  if (valid == 0xFu && (reinterpret_cast<uintptr_t>(px0) & 3u) == 0) {
+    constexpr uint32_t W = 3 * B;
+    constexpr uint32_t mask = B == 4 ? 0xFFFFFFFFu : (1u << (8 * B)) - 1u;
+    uint32_t w[W];
+#pragma unroll
+    for (uint32_t k = 0; k < W; k++) w[k] = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++)
+#pragma unroll
+      for (uint32_t k = 0; k < 3; k++) {
+        const uint32_t byte = (i * 3 + k) * B;
+        w[byte / 4] |= (c[k][i] & mask) << (8 * (byte % 4));
+      }
+    store_dwords<W>(px0, w);
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    if (!(valid & (1u << i))) continue;
+#pragma unroll
+    for (uint32_t k = 0; k < 3; k++) store_sample<B>(px0 + (i * 3 + k) * B, c[k][i]);
+  }
+}
+
+// YUV -> RGB/BGR (output_format/color.rs:100-110), three stored channels.
+template <uint32_t STORE>
+__device__ __forceinline__ void rgb3_group(const OutCtx &o, JXR_GLOBAL uint8_t *out, int gx, int gy, int x0, int x1,
+                                           int y0, int y1, const RecipeScale &rs) {
+  constexpr uint32_t B = StoreBytes<STORE>::value;
+  const DevImage &im = *o.im;
+  const int xb = 4 * gx, yb = 2 * gy;
+  int chroma[2][2][4];
+  if (im.upsample) group_chroma<Fast>(o, gx, gy, chroma);
+  uint32_t valid = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) valid |= (xb + i >= x0 && xb + i < x1) ? 1u << i : 0u;
+  const int ox0 = xb - (int)im.crop_x;
+#pragma unroll 1
+  for (int j = 0; j < 2; j++) {
+    const int y = yb + j;
+    if (y < y0 || y >= y1) continue;
+    const v4i l = sample_quad(o, 0, xb, y);
+    v4i u, v;
+    if (im.upsample) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        u[i] = j ? chroma[0][1][i] : chroma[0][0][i];
+        v[i] = j ? chroma[1][1][i] : chroma[1][0][i];
+      }
+    } else {
+      u = sample_quad(o, 1, xb, y);
+      v = sample_quad(o, 2, xb, y);
+    }
+    uint32_t c[3][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int t0 = -u[i];
+      const int g = l[i] - (t0 >> 1);
+      const int r = t0 + g - ((v[i] >> 1) + (v[i] & 1));
+      const int b = v[i] + r;
+      c[0][i] = recipe_store(rs, im.reverse ? b : r);
+      c[1][i] = recipe_store(rs, g);
+      c[2][i] = recipe_store(rs, im.reverse ? r : b);
+    }
+    store_rgb_row<B>(im, out, (uint32_t)(y - (int)im.crop_y), ox0, valid, c);
+  }
+}
+
+// Luma -> Luma, one stored channel.
+template <uint32_t STORE>
+__device__ __forceinline__ void luma1_group(const OutCtx &o, JXR_GLOBAL uint8_t *out, int gx, int gy, int x0, int x1,
+                                            int y0, int y1, const RecipeScale &rs) {
+  constexpr uint32_t B = StoreBytes<STORE>::value;
+  const DevImage &im = *o.im;
+  const int xb = 4 * gx, yb = 2 * gy;
+  uint32_t valid = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) valid |= (xb + i >= x0 && xb + i < x1) ? 1u << i : 0u;
+  const int ox0 = xb - (int)im.crop_x;
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int y = yb + j;
+    if (y < y0 || y >= y1) continue;
+    const v4i l = sample_quad(o, 0, xb, y);
+    uint32_t c[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) c[i] = recipe_store(rs, l[i]);
+    JXR_GLOBAL uint8_t *row = out + im.plane_offset[0] + (uint64_t)(y - (int)im.crop_y) * im.row_stride[0];
+    store_run<B>(row + (int64_t)ox0 * B, c, valid);
+  }
+}
+
+template <uint32_t STORE, uint32_t RECIPE>
+__device__ __forceinline__ void recipe_tile(const OutCtx &o, JXR_GLOBAL uint8_t *out, int gx0, int gy0, int ngx,
+                                            int ngy, int x0, int x1, int y0, int y1) {
+  const DevImage &im = *o.im;
+  const RecipeScale rs{im.recipe_add, im.recipe_lo, im.recipe_hi, im.recipe_shift};
+  const uint32_t m = div_magic((uint32_t)ngx);
+  for (int t = threadIdx.x; t < ngx * ngy; t += blockDim.x) {
+    const DivMod u = divmod(t, (uint32_t)ngx, m);
+    if constexpr (RECIPE == kRecipeRgb3)
+      rgb3_group<STORE>(o, out, gx0 + u.r, gy0 + u.q, x0, x1, y0, y1, rs);
+    else
+      luma1_group<STORE>(o, out, gx0 + u.r, gy0 + u.q, x0, x1, y0, y1, rs);
+  }
 }
 
 // ---- one tile, every plane, then output -----------------------------------
@@ -1415,10 +1584,18 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
   const int ngx = gx1 - gx0, ngy = gy1 - gy0;
   const OutCtx o{&im, ts.g, ts.soff, lds};
   // plain arithmetic when no output step can overflow for this tile
-  if (im.out_fast && ts.out_mag < kOutputSampleBound)
-    output_tile<STORE, Fast>(o, gptr_mut(slot.out), gx0, gy0, ngx, ngy, x0, x1, y0, y1, status);
+  JXR_GLOBAL uint8_t *out = gptr_mut(slot.out);
+  const bool fast = im.out_fast && ts.out_mag < kOutputSampleBound;
+  constexpr bool integer = STORE == JXRHIP_STORE_U8 || STORE == JXRHIP_STORE_U16 || STORE == JXRHIP_STORE_I16 ||
+                           STORE == JXRHIP_STORE_I32;
+  if (integer && fast && im.recipe == kRecipeRgb3)
+    recipe_tile<STORE, kRecipeRgb3>(o, out, gx0, gy0, ngx, ngy, x0, x1, y0, y1);
+  else if (integer && fast && im.recipe == kRecipeLuma1)
+    recipe_tile<STORE, kRecipeLuma1>(o, out, gx0, gy0, ngx, ngy, x0, x1, y0, y1);
+  else if (fast)
+    output_tile<STORE, Fast>(o, out, gx0, gy0, ngx, ngy, x0, x1, y0, y1, status);
   else
-    output_tile<STORE, Checked>(o, gptr_mut(slot.out), gx0, gy0, ngx, ngy, x0, x1, y0, y1, status);
+    output_tile<STORE, Checked>(o, out, gx0, gy0, ngx, ngy, x0, x1, y0, y1, status);
 }
 
 // Work assignment: workgroup b takes an arithmetic sequence of work items.

@@ -45,6 +45,15 @@ struct ChanScale {
   uint32_t pad[3];
 };
 
+// Output paths specialised for common requests; applied to tiles whose
+// output arithmetic cannot overflow (every channel shares one scale with no
+// post-shift, no alpha, integer storage).
+enum OutRecipe : uint32_t {
+  kRecipeGeneric = 0,
+  kRecipeRgb3 = 1,   // YUV -> RGB or BGR, three stored channels
+  kRecipeLuma1 = 2,  // Luma -> Luma, one stored channel
+};
+
 enum Conversion : uint32_t {
   kConvLuma = 0,      // Luma -> Luma
   kConvLumaRgb = 1,   // Luma -> RGB (replicated)
@@ -78,7 +87,10 @@ struct alignas(64) DevImage {
   uint32_t f_mant, f_alpha_mant;           // F32 mantissa lengths
   int32_t f_bias, f_alpha_bias;            // F32 exponent biases
   uint32_t out_fast;       // every channel's |bias| + rounding < 2^29: plain output arithmetic allowed
-  uint32_t pad0;
+  uint32_t recipe;         // OutRecipe: a specialised output path for plain-arithmetic tiles
+  int32_t recipe_add;      // recipes: bias + rounding shared by every channel
+  uint32_t recipe_shift;   // recipes: shift shared by every channel
+  int32_t recipe_lo, recipe_hi;  // recipes: clamp of the stored value
   uint64_t plane_offset[kMaxPlanes];       // output surface planes
   uint64_t row_stride[kMaxPlanes];
   ChanScale color_scale[8];

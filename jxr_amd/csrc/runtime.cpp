@@ -413,6 +413,33 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
     check(dv.alpha_scale);
     dv.out_fast = small ? 1 : 0;
   }
+  {
+    // specialised output recipes (plain-arithmetic tiles only)
+    // a channel qualifies when it is neither padding nor invalid and its
+    // post-shift (if any) is zero
+    auto plain = [](const ChanScale &c) { return (c.flags & 3u) == 0 && (!(c.flags & 4u) || c.post_shift == 0); };
+    const ChanScale &c0 = dv.color_scale[0];
+    bool same = plain(c0);
+    for (uint32_t c = 1; c < dv.channels && c < 8; c++) {
+      const ChanScale &c1 = dv.color_scale[c];
+      same &= plain(c1) && c1.shifted_bias == c0.shifted_bias && c1.rounding == c0.rounding && c1.shift == c0.shift;
+    }
+    const bool integer = f.storage == JXRHIP_STORE_U8 || f.storage == JXRHIP_STORE_U16 ||
+                         f.storage == JXRHIP_STORE_I16 || f.storage == JXRHIP_STORE_I32;
+    dv.recipe = kRecipeGeneric;
+    if (dv.out_fast && same && integer && !dv.has_alpha && !f.premultiply_alpha) {
+      if (dv.conv == kConvYuvRgb && dv.channels == 3) dv.recipe = kRecipeRgb3;
+      else if (dv.conv == kConvLuma && dv.channels == 1) dv.recipe = kRecipeLuma1;
+    }
+    dv.recipe_add = c0.shifted_bias + c0.rounding;  // both below 2^29 when out_fast
+    dv.recipe_shift = c0.shift;
+    switch (f.storage) {
+      case JXRHIP_STORE_U8: dv.recipe_lo = 0; dv.recipe_hi = 255; break;
+      case JXRHIP_STORE_U16: dv.recipe_lo = 0; dv.recipe_hi = dv.u16_max; break;
+      case JXRHIP_STORE_I16: dv.recipe_lo = -32768; dv.recipe_hi = 32767; break;
+      default: dv.recipe_lo = INT32_MIN; dv.recipe_hi = INT32_MAX; break;
+    }
+  }
   dv.premultiply = f.premultiply_alpha ? 1 : 0;
   dv.centering_x = f.chroma_centering_x;
   dv.centering_y = f.chroma_centering_y;
