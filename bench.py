@@ -29,6 +29,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of the reconstruction kernel for `workload`, from
+    the committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, each
+    scaled by what the counter reads for a 256 MiB device copy), or None."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_counters.md"))):
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == workload:
+                    return int(parts[2])
+    return None
+
+
 def percentile(xs, p):
     xs = sorted(xs)
     if not xs:
@@ -184,7 +198,7 @@ def main():
                        "p99": round(percentile(batched["wall_ms"], 99), 4)},
         "device_ms_per_step": round(dev_ms, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
                      "kernel": "jxr_reconstruct_tiles", "avg_launch_ms": round(per_launch_ms, 4)},
     }
     if per_image is not None:

@@ -394,11 +394,16 @@ __device__ __forceinline__ void first_level_mb(const DevImage &im, const DevPlan
   const uint32_t blocks = p.bc * p.br;
   int v[16];
   bool ovf = false;
-  v[0] = dequant_checked(c[0], r.qdc, ovf);
   const bool lp = r.bands != JXRHIP_BANDS_DC_ONLY;
   const uint32_t stride = r.bands >= JXRHIP_BANDS_NO_FLEXBITS ? 16u : 1u;
+  // every load is issued before any is used (an absent coefficient reads the
+  // DC again), so the sixteen loads overlap instead of waiting one by one
+  int raw[16];
 #pragma unroll
-  for (uint32_t k = 1; k < 16; k++) v[k] = (lp && k < blocks) ? dequant_checked(c[k * stride], r.qlp, ovf) : 0;
+  for (uint32_t k = 0; k < 16; k++) raw[k] = c[(lp && k < blocks) ? k * stride : 0u];
+  v[0] = dequant_checked(raw[0], r.qdc, ovf);
+#pragma unroll
+  for (uint32_t k = 1; k < 16; k++) v[k] = (lp && k < blocks) ? dequant_checked(raw[k], r.qlp, ovf) : 0;
   uint32_t mag = 0;
 #pragma unroll
   for (int k = 0; k < 16; k++) mag |= magbits(v[k]);
@@ -673,32 +678,38 @@ __device__ __forceinline__ uint32_t second_level_block(const DevImage &im, const
 #pragma unroll
       for (int j = 1; j < 16; j++) v[j] = blk[j];
     }
-    // running sums of the residuals (prediction.rs:53-97)
-    if (r.pred == JXRHIP_PRED_FROM_LEFT && cc > 0) {
-      const JXR_GLOBAL int *row0 = c + (rr << (g.bcs + 4));
-      int s4 = row0[4], s8 = row0[8], s12 = row0[12];
-      for (int j = 1; j < cc; j++) {
-        const JXR_GLOBAL int *b = row0 + j * 16;
-        povf |= __builtin_add_overflow(b[4], s4, &s4);
-        povf |= __builtin_add_overflow(b[8], s8, &s8);
-        povf |= __builtin_add_overflow(b[12], s12, &s12);
+    // running sums of the residuals (prediction.rs:53-97): the residuals of
+    // the (at most three) earlier blocks of the row or column are loaded
+    // first, then summed in the reference's order
+    const bool left = r.pred == JXRHIP_PRED_FROM_LEFT && cc > 0;
+    const bool top = r.pred == JXRHIP_PRED_FROM_TOP && rr > 0;
+    if (left || top) {
+      const int n = left ? cc : rr;
+      const JXR_GLOBAL int *first = left ? c + (rr << (g.bcs + 4)) + 4 : c + cc * 16 + 1;
+      const int step = left ? 16 : 16 << g.bcs;   // next block along the row or column
+      const int lane = left ? 4 : 1;              // next position within a block
+      int t[3][3];
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) t[j][q] = first[(j < n ? j * step : 0) + q * lane];
+      int s0 = t[0][0], s1 = t[0][1], s2 = t[0][2];
+#pragma unroll
+      for (int j = 1; j < 3; j++)
+        if (j < n) {
+          povf |= __builtin_add_overflow(t[j][0], s0, &s0);
+          povf |= __builtin_add_overflow(t[j][1], s1, &s1);
+          povf |= __builtin_add_overflow(t[j][2], s2, &s2);
+        }
+      if (left) {
+        povf |= __builtin_add_overflow(v[4], s0, &v[4]);
+        povf |= __builtin_add_overflow(v[8], s1, &v[8]);
+        povf |= __builtin_add_overflow(v[12], s2, &v[12]);
+      } else {
+        povf |= __builtin_add_overflow(v[1], s0, &v[1]);
+        povf |= __builtin_add_overflow(v[2], s1, &v[2]);
+        povf |= __builtin_add_overflow(v[3], s2, &v[3]);
       }
-      povf |= __builtin_add_overflow(v[4], s4, &v[4]);
-      povf |= __builtin_add_overflow(v[8], s8, &v[8]);
-      povf |= __builtin_add_overflow(v[12], s12, &v[12]);
-    } else if (r.pred == JXRHIP_PRED_FROM_TOP && rr > 0) {
-      const JXR_GLOBAL int *col0 = c + cc * 16;
-      const int st = 16 << g.bcs;
-      int s1 = col0[1], s2 = col0[2], s3 = col0[3];
-      for (int j = 1; j < rr; j++) {
-        const JXR_GLOBAL int *b = col0 + j * st;
-        povf |= __builtin_add_overflow(b[1], s1, &s1);
-        povf |= __builtin_add_overflow(b[2], s2, &s2);
-        povf |= __builtin_add_overflow(b[3], s3, &s3);
-      }
-      povf |= __builtin_add_overflow(v[1], s1, &v[1]);
-      povf |= __builtin_add_overflow(v[2], s2, &v[2]);
-      povf |= __builtin_add_overflow(v[3], s3, &v[3]);
     }
     uint32_t mag = 0;
 #pragma unroll
