@@ -975,7 +975,9 @@ __device__ __forceinline__ bool format_pixel(const DevImage &im, const int in[4]
 template <uint32_t STORE>
 struct StoreBytes {
   static constexpr uint32_t value =
-      STORE == JXRHIP_STORE_U8 ? 1u : (STORE == JXRHIP_STORE_I32 || STORE == JXRHIP_STORE_F32) ? 4u : 2u;
+      (STORE == JXRHIP_STORE_U8 || STORE == JXRHIP_STORE_BITS) ? 1u
+      : (STORE == JXRHIP_STORE_I32 || STORE == JXRHIP_STORE_F32 || STORE == JXRHIP_STORE_PACKED32) ? 4u
+                                                                                                   : 2u;
 };
 
 // Dword vectors that need only 4-byte alignment (interleaved rows).
@@ -1225,6 +1227,200 @@ __device__ __forceinline__ void output_tile(const OutCtx &o, JXR_GLOBAL uint8_t 
   for (int t = threadIdx.x; t < ngx * ngy; t += blockDim.x) {
     const DivMod u = divmod(t, (uint32_t)ngx, m);
     output_group<STORE, A>(o, out, gx0 + u.r, gy0 + u.q, x0, x1, y0, y1, ok);
+  }
+  if (!ok) fail(status, JXRHIP_STAGE_OUTPUT);
+}
+
+// ---- packed words, one-bit rows and planar YUV ----------------------------
+__device__ __forceinline__ uint32_t clip_field(int v, uint32_t maximum) {
+  return v < 0 ? 0u : ((uint32_t)v > maximum ? maximum : (uint32_t)v);
+}
+
+// RGBE (jxr-math/src/rgbe.rs:6-100): each channel as (mantissa, exponent),
+// mantissas aligned to the largest exponent by a rounded shift; bytes R, G,
+// B, E from the lowest.
+__device__ __forceinline__ uint32_t rgbe_word(const int v[3]) {
+  int m[3], e[3], top = 0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if (v[c] <= 0) {
+      m[c] = 0;
+      e[c] = 0;
+    } else if ((v[c] >> 7) > 1) {
+      m[c] = (v[c] & 0x7f) + 128;
+      e[c] = v[c] >> 7;
+    } else {
+      m[c] = v[c];
+      e[c] = 1;
+    }
+    top = max(top, e[c]);
+  }
+  uint32_t word = (uint32_t)min(top, 255) << 24;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const int d = top - e[c];
+    const int a = d <= 0 ? m[c] : d >= 31 ? 0 : (int)((2ll * m[c] + 1) >> (d + 1));
+    word |= (uint32_t)min(max(a, 0), 255) << (8 * c);
+  }
+  return word;
+}
+
+// One packed word of scaled R, G, B (output_format/packed_color.rs:9-38,
+// 111-135): red in the highest field.
+__device__ __forceinline__ uint32_t packed_word(uint32_t kind, uint32_t depth, const int v[3]) {
+  if (kind == kOutPacked16) {
+    if (depth == JXRHIP_DEPTH_RGB565)
+      return clip_field(v[2], 31) | clip_field(v[1], 63) << 5 | clip_field(v[0], 31) << 11;
+    return clip_field(v[2], 31) | clip_field(v[1], 31) << 5 | clip_field(v[0], 31) << 10;
+  }
+  if (kind == kOutRgb101010) return clip_field(v[2], 1023) | clip_field(v[1], 1023) << 10 | clip_field(v[0], 1023) << 20;
+  return rgbe_word(v);
+}
+
+// One 4x2 group of packed RGB words (RGB555/565, RGB101010, RGBE).
+template <uint32_t STORE, class A>
+__device__ __forceinline__ void packed_group(const OutCtx &o, JXR_GLOBAL uint8_t *out, int gx, int gy, int x0,
+                                             int x1, int y0, int y1, bool &ok) {
+  constexpr uint32_t B = StoreBytes<STORE>::value;
+  const DevImage &im = *o.im;
+  const int xb = 4 * gx, yb = 2 * gy;
+  int chroma[2][2][4];
+  if (im.upsample) group_chroma<A>(o, gx, gy, chroma);
+  uint32_t valid = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) valid |= (xb + i >= x0 && xb + i < x1) ? 1u << i : 0u;
+  const int ox0 = xb - (int)im.crop_x;
+  const bool bad_scale = (im.color_scale[0].flags | im.color_scale[1].flags | im.color_scale[2].flags) & 1u;
+#pragma unroll 1
+  for (int j = 0; j < 2; j++) {
+    const int y = yb + j;
+    if (y < y0 || y >= y1) continue;
+    const v4i l = sample_quad(o, 0, xb, y);
+    v4i u = {0, 0, 0, 0}, v = {0, 0, 0, 0};
+    if (im.conv == kConvYuvRgb) {
+      if (im.upsample) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          u[i] = j ? chroma[0][1][i] : chroma[0][0][i];
+          v[i] = j ? chroma[1][1][i] : chroma[1][0][i];
+        }
+      } else {
+        u = sample_quad(o, 1, xb, y);
+        v = sample_quad(o, 2, xb, y);
+      }
+    }
+    uint32_t w[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      A ar;
+      int rgb[3];
+      if (im.conv == kConvYuvRgb) {
+        // output_format/color.rs:100-110
+        const int t0 = ar.neg(u[i]);
+        const int g = ar.sub(l[i], t0 >> 1);
+        const int r = ar.sub(ar.add(t0, g), (v[i] >> 1) + (v[i] & 1));
+        rgb[0] = r;
+        rgb[1] = g;
+        rgb[2] = ar.add(v[i], r);
+      } else {
+        rgb[0] = rgb[1] = rgb[2] = l[i];
+      }
+      int sc[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) sc[c] = apply_scale(im.color_scale[c], rgb[c], ar);
+      w[i] = packed_word(im.out_kind, im.out_depth, sc);
+      ok &= (!ar.ovf && !bad_scale) || !(valid & (1u << i));
+    }
+    JXR_GLOBAL uint8_t *row = out + im.plane_offset[0] + (uint64_t)(y - (int)im.crop_y) * im.row_stride[0];
+    store_run<B>(row + (int64_t)ox0 * B, w, valid);
+  }
+}
+
+// One-bit luma (packing.rs:364-386): byte k of an output row holds output
+// pixels 8k..8k+7, the first in the highest bit. A byte can hold pixels of two
+// tiles, so each tile ORs its bits into rows the runtime has cleared, a
+// 32-bit word at a time (device allocations start 4-byte aligned, so the
+// word containing a byte of the surface lies in its allocation).
+__device__ __forceinline__ void bits_tile(const OutCtx &o, JXR_GLOBAL uint8_t *out, int x0, int x1, int y0, int y1,
+                                          uint32_t *status) {
+  const DevImage &im = *o.im;
+  const PlaneGeo &g = o.G[0];
+  const int cx = (int)im.crop_x;
+  const int k0 = (x0 - cx) >> 3, k1 = ((x1 - 1 - cx) >> 3) + 1;
+  const int nk = k1 - k0, n = nk * (y1 - y0);
+  const uint32_t m = div_magic((uint32_t)nk);
+  const ChanScale cs = im.color_scale[0];
+  bool ok = true;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const DivMod u = divmod(t, (uint32_t)nk, m);
+    const int k = k0 + u.r, y = y0 + u.q;
+    const int a = max(8 * k + cx, x0), b = min(8 * k + 8 + cx, x1);
+    const int *row = o.lds + o.soff[0] + (y - g.soy) * (int)g.srw - g.sox;
+    uint32_t byte = 0;
+    for (int x = a; x < b; x++) {
+      Checked ar;
+      const int v = apply_scale(cs, row[x], ar);
+      ok &= !ar.ovf && !(cs.flags & 1u);
+      uint32_t bit = v < 0 ? 0u : v > 1 ? 1u : (uint32_t)v;
+      if (im.out_depth == JXRHIP_DEPTH_BIT1_BLACK) bit = 1u - bit;
+      byte |= bit << (7 - ((x - cx) & 7));
+    }
+    if (byte) {
+      const uintptr_t addr = reinterpret_cast<uintptr_t>(out + im.plane_offset[0] +
+                                                         (uint64_t)(y - (int)im.crop_y) * im.row_stride[0] + k);
+      atomicOr(reinterpret_cast<uint32_t *>(addr & ~(uintptr_t)3), byte << (8 * (addr & 3)));
+    }
+  }
+  if (!ok) fail(status, JXRHIP_STAGE_OUTPUT);
+}
+
+// Native-resolution planar YUV (planar.rs:77-230): luma and alpha over the
+// crop, each chroma plane over the crop at the coded sampling, every sample
+// scaled and clipped alone.
+template <uint32_t STORE>
+__device__ __forceinline__ void yuv_tile(const OutCtx &o, JXR_GLOBAL uint8_t *out, int tmx, int tmy, int nx, int ny,
+                                         uint32_t *status) {
+  constexpr uint32_t B = StoreBytes<STORE>::value;
+  const DevImage &im = *o.im;
+  bool ok = true;
+  const uint32_t nplanes = im.has_alpha ? 4u : 3u;
+  for (uint32_t k = 0; k < nplanes; k++) {
+    const int pl = k < 3 ? (int)k : im.alpha_plane;
+    const PlaneGeo &g = o.G[pl];
+    const bool chroma = k == 1 || k == 2;
+    const int sy = chroma ? (int)im.chroma_sub_y : 1;
+    const int cx0 = chroma ? (int)im.crop_x / 2 : (int)im.crop_x;
+    const int cy0 = (int)im.crop_y / sy;
+    const int cw = chroma ? (int)im.crop_w / 2 : (int)im.crop_w;
+    const int ch = (int)im.crop_h / sy;
+    const int ax0 = max(tmx * (int)g.mw, cx0), ax1 = min((tmx + nx) * (int)g.mw, cx0 + cw);
+    const int ay0 = max(tmy * (int)g.mh, cy0), ay1 = min((tmy + ny) * (int)g.mh, cy0 + ch);
+    if (ax1 <= ax0 || ay1 <= ay0) continue;
+    const int q0 = ax0 >> 2, nq = ((ax1 - 1) >> 2) + 1 - q0;
+    const int n = nq * (ay1 - ay0);
+    const uint32_t m = div_magic((uint32_t)nq);
+    const ChanScale cs = k == 3 ? im.alpha_scale : im.color_scale[k];
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      const DivMod u = divmod(t, (uint32_t)nq, m);
+      const int xq = 4 * (q0 + u.r), y = ay0 + u.q;
+      const v4i sv = sample_quad(o, pl, xq, y);
+      uint32_t st[4];
+      uint32_t valid = 0;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const bool in = xq + i >= ax0 && xq + i < ax1;
+        valid |= in ? 1u << i : 0u;
+        Checked ar;
+        int v = apply_scale(cs, sv[i], ar);
+        ok &= (!ar.ovf && !(cs.flags & 1u)) || !in;
+        if constexpr (STORE == JXRHIP_STORE_U8) v = min(max(v, 0), 255);
+        else if constexpr (STORE == JXRHIP_STORE_U16) v = min(max(v, 0), im.u16_max);
+        else v = min(max(v, -32768), 32767);
+        st[i] = (uint32_t)v;
+      }
+      JXR_GLOBAL uint8_t *row = out + im.plane_offset[k] + (uint64_t)(y - cy0) * im.row_stride[k];
+      store_run<B>(row + (int64_t)(xq - cx0) * B, st, valid);
+    }
   }
   if (!ok) fail(status, JXRHIP_STAGE_OUTPUT);
 }
@@ -1599,6 +1795,28 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
   const bool fast = im.out_fast && ts.out_mag < kOutputSampleBound;
   constexpr bool integer = STORE == JXRHIP_STORE_U8 || STORE == JXRHIP_STORE_U16 || STORE == JXRHIP_STORE_I16 ||
                            STORE == JXRHIP_STORE_I32;
+  if constexpr (STORE == JXRHIP_STORE_BITS) {
+    bits_tile(o, out, x0, x1, y0, y1, status);
+    return;
+  } else if constexpr (STORE == JXRHIP_STORE_PACKED16 || STORE == JXRHIP_STORE_PACKED32) {
+    const uint32_t mg = div_magic((uint32_t)ngx);
+    bool ok = true;
+    for (int t = threadIdx.x; t < ngx * ngy; t += blockDim.x) {
+      const DivMod u = divmod(t, (uint32_t)ngx, mg);
+      if (fast)
+        packed_group<STORE, Fast>(o, out, gx0 + u.r, gy0 + u.q, x0, x1, y0, y1, ok);
+      else
+        packed_group<STORE, Checked>(o, out, gx0 + u.r, gy0 + u.q, x0, x1, y0, y1, ok);
+    }
+    if (!ok) fail(status, JXRHIP_STAGE_OUTPUT);
+    return;
+  } else {
+  if constexpr (STORE == JXRHIP_STORE_U8 || STORE == JXRHIP_STORE_U16 || STORE == JXRHIP_STORE_I16) {
+    if (im.out_kind == kOutYuv) {
+      yuv_tile<STORE>(o, out, tmx, tmy, nx, ny, status);
+      return;
+    }
+  }
   if (integer && fast && im.recipe == kRecipeRgb3)
     recipe_tile<STORE, kRecipeRgb3>(o, out, gx0, gy0, ngx, ngy, x0, x1, y0, y1);
   else if (integer && fast && im.recipe == kRecipeLuma1)
@@ -1607,6 +1825,7 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
     output_tile<STORE, Fast>(o, out, gx0, gy0, ngx, ngy, x0, x1, y0, y1, status);
   else
     output_tile<STORE, Checked>(o, out, gx0, gy0, ngx, ngy, x0, x1, y0, y1, status);
+  }
 }
 
 // Work assignment: workgroup b takes an arithmetic sequence of work items.
@@ -1677,6 +1896,17 @@ hipError_t launch_tiles(const KernelArgs &args, uint32_t grid, hipStream_t strea
     case JXRHIP_STORE_F32:
       hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_F32>, dim3(grid), dim3(kThreads), lds, stream, args);
       break;
+    case JXRHIP_STORE_BITS:
+      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_BITS>, dim3(grid), dim3(kThreads), lds, stream, args);
+      break;
+    case JXRHIP_STORE_PACKED16:
+      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_PACKED16>, dim3(grid), dim3(kThreads), lds, stream,
+                         args);
+      break;
+    case JXRHIP_STORE_PACKED32:
+      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_PACKED32>, dim3(grid), dim3(kThreads), lds, stream,
+                         args);
+      break;
     default:
       return hipErrorInvalidValue;
   }
@@ -1692,6 +1922,13 @@ hipError_t kernel_occupancy(uint32_t storage, size_t dynamic_lds, int *groups_pe
     case JXRHIP_STORE_I32: k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_I32>); break;
     case JXRHIP_STORE_F16: k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_F16>); break;
     case JXRHIP_STORE_F32: k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_F32>); break;
+    case JXRHIP_STORE_BITS: k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_BITS>); break;
+    case JXRHIP_STORE_PACKED16:
+      k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_PACKED16>);
+      break;
+    case JXRHIP_STORE_PACKED32:
+      k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_PACKED32>);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu, k, (int)kThreads, dynamic_lds);
@@ -1704,7 +1941,10 @@ hipError_t set_kernel_lds_limit(size_t bytes) {
       reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_I16>),
       reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_I32>),
       reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_F16>),
-      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_F32>)};
+      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_F32>),
+      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_BITS>),
+      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_PACKED16>),
+      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_PACKED32>)};
   for (const void *k : kernels) {
     const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;

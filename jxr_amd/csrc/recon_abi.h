@@ -54,6 +54,16 @@ enum OutRecipe : uint32_t {
   kRecipeLuma1 = 2,  // Luma -> Luma, one stored channel
 };
 
+// What the output stage stores (output_format/{packing,packed_color,planar}.rs).
+enum OutKind : uint32_t {
+  kOutTyped = 0,      // one typed sample per channel (interleaved or planar)
+  kOutBits = 1,       // one-bit luma, eight pixels a byte, first pixel highest
+  kOutPacked16 = 2,   // RGB555 / RGB565 word
+  kOutRgb101010 = 3,  // RGB101010 word
+  kOutRgbe = 4,       // shared-exponent RGBE word
+  kOutYuv = 5,        // native-resolution planar YUV (+ alpha)
+};
+
 enum Conversion : uint32_t {
   kConvLuma = 0,      // Luma -> Luma
   kConvLumaRgb = 1,   // Luma -> RGB (replicated)
@@ -91,6 +101,9 @@ struct alignas(64) DevImage {
   int32_t recipe_add;      // recipes: bias + rounding shared by every channel
   uint32_t recipe_shift;   // recipes: shift shared by every channel
   int32_t recipe_lo, recipe_hi;  // recipes: clamp of the stored value
+  uint32_t out_kind;       // OutKind
+  uint32_t out_depth;      // JXRHIP_DEPTH_* (packed field widths, one-bit polarity)
+  uint32_t chroma_sub_y;   // planar YUV: 2 for 4:2:0 chroma rows, 1 otherwise
   uint64_t plane_offset[kMaxPlanes];       // output surface planes
   uint64_t row_stride[kMaxPlanes];
   ChanScale color_scale[8];

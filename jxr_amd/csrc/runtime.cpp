@@ -109,6 +109,7 @@ struct jxrhip_plan {
   DevImage dev{};  // device pointers filled per session at upload
   uint32_t mbx = 0, mby = 0;
   uint64_t output_bytes = 0, recon_coefs = 0, pixels = 0, alg_bytes = 0;
+  uint64_t content_bytes = 0;  // bytes of the surface the output stage writes
   uint64_t descriptor_bytes = 0;
   uint64_t coefficient_bytes = 0;
   uint64_t geometry_key = 0;  // bucket key: plane geometries
@@ -352,6 +353,8 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
     case JXRHIP_LAYOUT_CMYKA: primary = 4; break;
     case JXRHIP_LAYOUT_NCOMPONENT:
     case JXRHIP_LAYOUT_NCOMPONENT_ALPHA: primary = colors; break;
+    case JXRHIP_LAYOUT_YUV:
+    case JXRHIP_LAYOUT_YUVA: primary = 3; break;
     default: return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "channel layout not supported");
   }
   dv.primary_count = primary;
@@ -363,7 +366,47 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
     return fail(JXRHIP_ERR_INVALID_PLAN, "alpha output layout has no prepared alpha plane");
   const uint32_t dep = f.bit_depth;
   const uint32_t adep = f.has_alpha_format ? f.alpha_bit_depth : dep;
-  {
+  const uint32_t oc = f.output_color;
+  // storage, colour, layout and depth combinations of the packed, one-bit
+  // and planar YUV outputs (output_format/validate.rs:137-255)
+  dv.out_kind = kOutTyped;
+  dv.out_depth = dep;
+  if (f.storage == JXRHIP_STORE_BITS) {
+    if (dep != JXRHIP_DEPTH_BIT1_WHITE && dep != JXRHIP_DEPTH_BIT1_BLACK)
+      return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "bit depth does not match the storage kind");
+    if (l != JXRHIP_LAYOUT_LUMA || oc != JXRHIP_COLOR_LUMA)
+      return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "one-bit output stores luma only");
+    dv.out_kind = kOutBits;
+  } else if (f.storage == JXRHIP_STORE_PACKED16 || f.storage == JXRHIP_STORE_PACKED32) {
+    if (l != JXRHIP_LAYOUT_RGB) return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "packed output takes the RGB layout");
+    if (f.storage == JXRHIP_STORE_PACKED16 && (dep == JXRHIP_DEPTH_RGB555 || dep == JXRHIP_DEPTH_RGB565) &&
+        oc == JXRHIP_COLOR_RGB)
+      dv.out_kind = kOutPacked16;
+    else if (f.storage == JXRHIP_STORE_PACKED32 && dep == JXRHIP_DEPTH_RGB101010 && oc == JXRHIP_COLOR_RGB)
+      dv.out_kind = kOutRgb101010;
+    else if (f.storage == JXRHIP_STORE_PACKED32 && dep == JXRHIP_DEPTH_U8 && oc == JXRHIP_COLOR_RGBE)
+      dv.out_kind = kOutRgbe;
+    else
+      return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "bit depth does not match the storage kind");
+  } else if (l == JXRHIP_LAYOUT_YUV || l == JXRHIP_LAYOUT_YUVA) {
+    if (oc != cf || (oc != JXRHIP_COLOR_YUV420 && oc != JXRHIP_COLOR_YUV422) || f.internal_color != oc)
+      return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "planar YUV output keeps the coded 4:2:0 or 4:2:2 sampling");
+    if (f.premultiply_alpha) return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "premultiplied YUV output");
+    if (!d->surface.planar) return fail(JXRHIP_ERR_INVALID_PLAN, "YUV output is planar");
+    if (f.storage != JXRHIP_STORE_U8 && f.storage != JXRHIP_STORE_U16 && f.storage != JXRHIP_STORE_I16)
+      return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "T.832 output colour and bit depth matrix");
+    dv.out_kind = kOutYuv;
+    dv.chroma_sub_y = oc == JXRHIP_COLOR_YUV420 ? 2 : 1;
+    dv.upsample = 0;  // chroma is stored at its coded resolution
+  } else if (dep == JXRHIP_DEPTH_BIT1_WHITE || dep == JXRHIP_DEPTH_BIT1_BLACK || dep == JXRHIP_DEPTH_RGB555 ||
+             dep == JXRHIP_DEPTH_RGB565 || dep == JXRHIP_DEPTH_RGB101010) {
+    return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "bit depth does not match the storage kind");
+  }
+  if (dv.out_kind == kOutBits) {
+    dv.bytes = 1;
+  } else if (dv.out_kind == kOutPacked16 || dv.out_kind == kOutRgb101010 || dv.out_kind == kOutRgbe) {
+    dv.bytes = f.storage == JXRHIP_STORE_PACKED16 ? 2 : 4;
+  } else {
     bool depth_ok = true;
     switch (f.storage) {
       case JXRHIP_STORE_U8: dv.bytes = 1; depth_ok = dep == JXRHIP_DEPTH_U8; break;
@@ -384,9 +427,9 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
   dv.storage = f.storage;
   dv.u16_max = dep == JXRHIP_DEPTH_U10 ? 1023 : 65535;
   {
-    const uint32_t oc = f.output_color;
     const bool yuv = cf == JXRHIP_COLOR_YUV420 || cf == JXRHIP_COLOR_YUV422 || cf == JXRHIP_COLOR_YUV444;
-    if (oc == JXRHIP_COLOR_NCOMPONENT && cf == JXRHIP_COLOR_NCOMPONENT) dv.conv = kConvDirect;
+    if (dv.out_kind == kOutYuv) dv.conv = kConvDirect;  // planes stored as they are
+    else if (oc == JXRHIP_COLOR_NCOMPONENT && cf == JXRHIP_COLOR_NCOMPONENT) dv.conv = kConvDirect;
     else if (cf == JXRHIP_COLOR_LUMA && oc == JXRHIP_COLOR_LUMA) dv.conv = kConvLuma;
     else if (cf == JXRHIP_COLOR_LUMA && oc == JXRHIP_COLOR_RGB) dv.conv = kConvLumaRgb;
     else if (yuv && (oc == JXRHIP_COLOR_RGB || oc == JXRHIP_COLOR_RGBE)) dv.conv = kConvYuvRgb;
@@ -427,7 +470,7 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
     const bool integer = f.storage == JXRHIP_STORE_U8 || f.storage == JXRHIP_STORE_U16 ||
                          f.storage == JXRHIP_STORE_I16 || f.storage == JXRHIP_STORE_I32;
     dv.recipe = kRecipeGeneric;
-    if (dv.out_fast && same && integer && !dv.has_alpha && !f.premultiply_alpha) {
+    if (dv.out_kind == kOutTyped && dv.out_fast && same && integer && !dv.has_alpha && !f.premultiply_alpha) {
       if (dv.conv == kConvYuvRgb && dv.channels == 3) dv.recipe = kRecipeRgb3;
       else if (dv.conv == kConvLuma && dv.channels == 1) dv.recipe = kRecipeLuma1;
     }
@@ -462,15 +505,29 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
     dv.crop_y = (uint32_t)(f.crop_y - wy);
     dv.crop_w = f.crop_width;
     dv.crop_h = f.crop_height;
+    const bool word = dv.out_kind == kOutPacked16 || dv.out_kind == kOutRgb101010 || dv.out_kind == kOutRgbe;
+    if ((word || dv.out_kind == kOutBits) && s.planar)
+      return fail(JXRHIP_ERR_INVALID_PLAN, "packed and one-bit outputs are interleaved");
     const uint32_t need = s.planar ? dv.channels : 1;
     if (s.plane_count != need || need > kMaxPlanes) return fail(JXRHIP_ERR_INVALID_PLAN, "surface plane count");
     dv.planar = s.planar ? 1 : 0;
+    uint64_t content = 0;  // bytes the output stage writes
     for (uint32_t i = 0; i < need; i++) {
       const jxrhip_surface_plane &sp = s.planes[i];
       const uint32_t ch = s.planar ? 1 : dv.channels;
-      if (sp.width != s.width || sp.height != s.height || sp.channels != ch)
+      uint32_t pw = s.width, ph = s.height;
+      if (dv.out_kind == kOutYuv && (i == 1 || i == 2)) {
+        // chroma planes at the coded sampling (planar.rs:77-106)
+        pw = s.width / 2;
+        ph = s.height / dv.chroma_sub_y;
+        if (pw == 0 || ph == 0) return fail(JXRHIP_ERR_INVALID_PLAN, "planar chroma plane is empty");
+      }
+      if (sp.width != pw || sp.height != ph || sp.channels != ch)
         return fail(JXRHIP_ERR_INVALID_PLAN, "surface plane geometry");
-      const uint64_t row = (uint64_t)sp.width * ch * dv.bytes;
+      uint64_t row = (uint64_t)sp.width * ch * dv.bytes;
+      if (dv.out_kind == kOutBits) row = ((uint64_t)sp.width + 7) / 8;  // format.rs:216
+      if (word) row = (uint64_t)sp.width * dv.bytes;                     // one word a pixel
+      content += row * sp.height;
       if (sp.row_stride_bytes < row) return fail(JXRHIP_ERR_INVALID_PLAN, "surface row stride");
       if (sp.byte_offset + (uint64_t)(sp.height - 1) * sp.row_stride_bytes + row > s.byte_len)
         return fail(JXRHIP_ERR_INVALID_PLAN, "surface plane exceeds its byte length");
@@ -480,6 +537,7 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
       dv.row_stride[i] = sp.row_stride_bytes;
     }
     plan->output_bytes = s.byte_len;
+    plan->content_bytes = content;
   }
   // ---- work and footprint figures ----
   plan->pixels = (uint64_t)f.crop_width * f.crop_height;
@@ -497,7 +555,7 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
       for (const MbRec &r : p.recs)
         stored += r.bands == 0 ? 1 : r.bands == 1 ? p.bc * p.br : (uint64_t)p.bc * p.br * 16;
     }
-    const uint64_t out_px_bytes = plan->pixels * dv.channels * dv.bytes;
+    const uint64_t out_px_bytes = plan->content_bytes;
     plan->alg_bytes = stored * 4 + mbs * sizeof(MbRec) + out_px_bytes;
   }
   plan->descriptor_bytes += sizeof(DevImage);
@@ -916,6 +974,16 @@ static void choose_tile(const jxrhip_session *s, const jxrhip_plan *p, uint32_t 
   words = need(1, 1);
 }
 
+// One-bit output: the kernel ORs each tile's bits into cleared rows (a byte
+// can hold pixels of two tiles), so the rows' bytes are zeroed first; bytes
+// between rows are left as they are.
+static hipError_t clear_bit_rows(const jxrhip_plan *p, void *dst, hipStream_t stream) {
+  if (p->dev.out_kind != kOutBits) return hipSuccess;
+  const size_t row = ((size_t)p->dev.crop_w + 7) / 8;
+  return hipMemset2DAsync((uint8_t *)dst + p->dev.plane_offset[0], (size_t)p->dev.row_stride[0], 0, row,
+                          p->dev.crop_h, stream);
+}
+
 static jxrhip_status submit_impl(jxrhip_session *s, const jxrhip_plan *p, void *dst, uint64_t dst_bytes,
                                  jxrhip_submission **out) {
   if (!s || !p || !out) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
@@ -963,6 +1031,7 @@ static jxrhip_status submit_impl(jxrhip_session *s, const jxrhip_plan *p, void *
   const uint32_t tiles = args.inline_slot.tiles_x * ((p->mby + ty - 1) / ty);
   args.item_count = tiles;
   HIP_TRY(hipMemsetAsync(status_word, 0, 4, sub->stream), "status reset");
+  HIP_TRY(clear_bit_rows(p, dst, sub->stream), "one-bit rows");
   // one item per workgroup; a multiple of 8 workgroups lets the kernel give
   // each XCD a contiguous share of the tiles
   HIP_TRY(launch_tiles(args, (tiles + 7u) & ~7u, sub->stream), "reconstruction launch");
@@ -1192,6 +1261,7 @@ extern "C" jxrhip_status jxrhip_submit_batch(jxrhip_session *s, const jxrhip_pla
     if (st) return st;
   }
   HIP_TRY(hipMemsetAsync(b->statuses.ptr, 0, sizeof(uint32_t) * std::max<uint32_t>(count, 1), stream), "status reset");
+  for (uint32_t i = 0; i < count; i++) HIP_TRY(clear_bit_rows(plans[i], dsts[i], stream), "one-bit rows");
   // per-bucket slot arrays go into one device buffer
   std::vector<Slot> slots;
   struct Launch {

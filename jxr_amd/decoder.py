@@ -104,11 +104,14 @@ class EncodedImage:
 # ---------------------------------------------------------------------------
 # output policy + surface layout
 # ---------------------------------------------------------------------------
-_STORE_BYTES = {abi.STORE_U8: 1, abi.STORE_U16: 2, abi.STORE_I16: 2, abi.STORE_I32: 4,
-                abi.STORE_F16: 2, abi.STORE_F32: 4}
+# bytes of one stored element: a sample, a packed pixel word, or a byte of
+# eight one-bit samples
+_STORE_BYTES = {abi.STORE_BITS: 1, abi.STORE_U8: 1, abi.STORE_U16: 2, abi.STORE_I16: 2, abi.STORE_I32: 4,
+                abi.STORE_F16: 2, abi.STORE_F32: 4, abi.STORE_PACKED16: 2, abi.STORE_PACKED32: 4}
 _LAYOUT_CHANNELS = {abi.LAYOUT_LUMA: 1, abi.LAYOUT_LUMA_ALPHA: 2, abi.LAYOUT_RGB: 3,
                     abi.LAYOUT_BGR: 3, abi.LAYOUT_RGBA: 4, abi.LAYOUT_BGRA: 4,
-                    abi.LAYOUT_RGBX: 4, abi.LAYOUT_BGRX: 4}
+                    abi.LAYOUT_RGBX: 4, abi.LAYOUT_BGRX: 4, abi.LAYOUT_CMYK: 4, abi.LAYOUT_CMYKA: 5,
+                    abi.LAYOUT_YUV: 3, abi.LAYOUT_YUVA: 4}
 
 
 @dataclass
@@ -138,10 +141,41 @@ class OutputSpec:
 
 
 def surface_layout(width, height, spec: OutputSpec, base: int = 0) -> abi.SurfaceLayout:
+    """Tightly packed surface of a request (SurfaceLayout::for_output,
+    crates/jxr-core/src/surface.rs:29-140): one interleaved plane, one plane
+    per channel (NCHW), or native planar YUV with chroma planes at the coded
+    sampling; packed formats store one word per pixel and one-bit luma a
+    byte per eight pixels."""
     s = abi.SurfaceLayout()
     s.width, s.height = width, height
     b = spec.bytes_per_sample
     ch = spec.channels
+    if spec.layout in (abi.LAYOUT_YUV, abi.LAYOUT_YUVA):
+        s.planar = 1
+        sub_y = 2 if spec.output_color == abi.COLOR_YUV420 else 1
+        dims = [(width, height), (width // 2, height // sub_y), (width // 2, height // sub_y)]
+        if spec.layout == abi.LAYOUT_YUVA:
+            dims.append((width, height))
+        s.plane_count = len(dims)
+        off = base
+        for c, (w, h) in enumerate(dims):
+            pl = s.planes[c]
+            pl.byte_offset = off
+            pl.row_stride_bytes = w * b + spec.row_padding
+            pl.width, pl.height, pl.channels = w, h, 1
+            off += pl.row_stride_bytes * h
+        s.byte_len = off
+        return s
+    if spec.storage in (abi.STORE_BITS, abi.STORE_PACKED16, abi.STORE_PACKED32):
+        s.planar = 0
+        s.plane_count = 1
+        pl = s.planes[0]
+        pl.byte_offset = base
+        row = (width + 7) // 8 if spec.storage == abi.STORE_BITS else width * b
+        pl.row_stride_bytes = row + spec.row_padding
+        pl.width, pl.height, pl.channels = width, height, ch
+        s.byte_len = base + pl.row_stride_bytes * height
+        return s
     if spec.planar:
         s.planar = 1
         s.plane_count = ch
@@ -176,7 +210,8 @@ def image_desc(enc: EncodedImage, spec: OutputSpec) -> abi.ImageDesc:
         d.separate_alpha = enc.alpha
     o = d.output
     yuv = enc.color_format in (abi.COLOR_YUV420, abi.COLOR_YUV422, abi.COLOR_YUV444)
-    o.internal_color = abi.COLOR_YUV444 if yuv else enc.color_format
+    native = spec.output_color in (abi.COLOR_YUV420, abi.COLOR_YUV422)  # planar YUV keeps the sampling
+    o.internal_color = enc.color_format if native else (abi.COLOR_YUV444 if yuv else enc.color_format)
     o.output_color = spec.output_color
     o.bit_depth = spec.depth
     o.shift_bits = spec.shift_bits

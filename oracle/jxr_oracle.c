@@ -1146,7 +1146,13 @@ typedef struct {
   channel_scale alpha_scale;
   int32_t u16_max;
   int upsample; /* 0 none, 1 = 4:2:0, 2 = 4:2:2 */
+  int kind;     /* OUT_* */
+  uint32_t depth;
 } out_ctx;
+
+/* Output kinds: typed samples per channel, one-bit luma, packed RGB words,
+ * RGBE words and native-resolution planar YUV. */
+enum { OUT_TYPED = 0, OUT_BITS, OUT_PACKED16, OUT_RGB101010, OUT_RGBE, OUT_YUV };
 
 static int layout_has_alpha(uint32_t l) {
   return l == JXRHIP_LAYOUT_LUMA_ALPHA || l == JXRHIP_LAYOUT_YUVA || l == JXRHIP_LAYOUT_RGBA ||
@@ -1277,6 +1283,8 @@ static int output_setup(out_ctx *o, const jxrhip_image_desc *d, image_t *im, cha
     case JXRHIP_LAYOUT_CMYKA: o->primary_count = 4; break;
     case JXRHIP_LAYOUT_NCOMPONENT:
     case JXRHIP_LAYOUT_NCOMPONENT_ALPHA: o->primary_count = im->colors; break;
+    case JXRHIP_LAYOUT_YUV:
+    case JXRHIP_LAYOUT_YUVA: o->primary_count = 3; break;
     default: FAIL(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "channel layout not supported");
   }
   o->has_alpha = layout_has_alpha(l);
@@ -1287,7 +1295,41 @@ static int output_setup(out_ctx *o, const jxrhip_image_desc *d, image_t *im, cha
   uint32_t st = f->storage;
   uint32_t dep = f->bit_depth;
   uint32_t adep = f->has_alpha_format ? f->alpha_bit_depth : dep;
-  switch (st) {
+  uint32_t oc = f->output_color;
+  o->depth = dep;
+  o->kind = OUT_TYPED;
+  /* validate.rs:137-255: storage, colour, layout and depth combinations of
+   * the packed, one-bit and planar YUV outputs */
+  if (st == JXRHIP_STORE_BITS) {
+    if (dep != JXRHIP_DEPTH_BIT1_WHITE && dep != JXRHIP_DEPTH_BIT1_BLACK) goto bad_depth;
+    if (l != JXRHIP_LAYOUT_LUMA || oc != JXRHIP_COLOR_LUMA)
+      FAIL(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "one-bit output stores luma only");
+    o->kind = OUT_BITS;
+    o->bytes = 1;
+  } else if (st == JXRHIP_STORE_PACKED16 || st == JXRHIP_STORE_PACKED32) {
+    if (l != JXRHIP_LAYOUT_RGB) FAIL(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "packed output takes the RGB layout");
+    if (st == JXRHIP_STORE_PACKED16 && (dep == JXRHIP_DEPTH_RGB555 || dep == JXRHIP_DEPTH_RGB565) &&
+        oc == JXRHIP_COLOR_RGB)
+      o->kind = OUT_PACKED16;
+    else if (st == JXRHIP_STORE_PACKED32 && dep == JXRHIP_DEPTH_RGB101010 && oc == JXRHIP_COLOR_RGB)
+      o->kind = OUT_RGB101010;
+    else if (st == JXRHIP_STORE_PACKED32 && dep == JXRHIP_DEPTH_U8 && oc == JXRHIP_COLOR_RGBE)
+      o->kind = OUT_RGBE;
+    else
+      goto bad_depth;
+    o->bytes = st == JXRHIP_STORE_PACKED16 ? 2 : 4;
+  } else if (l == JXRHIP_LAYOUT_YUV || l == JXRHIP_LAYOUT_YUVA) {
+    if (oc != d->color_format || (oc != JXRHIP_COLOR_YUV420 && oc != JXRHIP_COLOR_YUV422) ||
+        f->internal_color != oc)
+      FAIL(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "planar YUV output keeps the coded 4:2:0 or 4:2:2 sampling");
+    if (f->premultiply_alpha) FAIL(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "premultiplied YUV output");
+    if (!d->surface.planar) FAIL(JXRHIP_ERR_INVALID_PLAN, "YUV output is planar");
+    o->kind = OUT_YUV;
+  } else if (dep == JXRHIP_DEPTH_BIT1_WHITE || dep == JXRHIP_DEPTH_BIT1_BLACK || dep == JXRHIP_DEPTH_RGB555 ||
+             dep == JXRHIP_DEPTH_RGB565 || dep == JXRHIP_DEPTH_RGB101010) {
+    goto bad_depth;
+  }
+  if (o->kind == OUT_TYPED || o->kind == OUT_YUV) switch (st) {
     case JXRHIP_STORE_U8: o->bytes = 1; if (dep != JXRHIP_DEPTH_U8) goto bad_depth; break;
     case JXRHIP_STORE_U16: o->bytes = 2; if (dep != JXRHIP_DEPTH_U16 && dep != JXRHIP_DEPTH_U10) goto bad_depth; break;
     case JXRHIP_STORE_I16: o->bytes = 2; if (dep != JXRHIP_DEPTH_I16) goto bad_depth; break;
@@ -1301,9 +1343,11 @@ static int output_setup(out_ctx *o, const jxrhip_image_desc *d, image_t *im, cha
       break;
     default: FAIL(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "storage kind not supported");
   }
+  if (o->kind == OUT_YUV && st != JXRHIP_STORE_U8 && st != JXRHIP_STORE_U16 && st != JXRHIP_STORE_I16)
+    FAIL(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "T.832 output colour and bit depth matrix");
   o->u16_max = dep == JXRHIP_DEPTH_U10 ? 1023 : 65535;
   uint32_t cf = d->color_format;
-  o->upsample = cf == JXRHIP_COLOR_YUV420 ? 1 : cf == JXRHIP_COLOR_YUV422 ? 2 : 0;
+  o->upsample = o->kind == OUT_YUV ? 0 : cf == JXRHIP_COLOR_YUV420 ? 1 : cf == JXRHIP_COLOR_YUV422 ? 2 : 0;
   if (o->upsample && (f->chroma_centering_x > 4 || (o->upsample == 1 && f->chroma_centering_y > 4)))
     FAIL(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "unknown chroma centering has no exact reconstruction filter");
   for (uint32_t c = 0; c < 8; c++) {
@@ -1318,13 +1362,26 @@ static int output_setup(out_ctx *o, const jxrhip_image_desc *d, image_t *im, cha
   if (s->width != f->crop_width || s->height != f->crop_height || f->crop_width == 0 || f->crop_height == 0)
     FAIL(JXRHIP_ERR_INVALID_PLAN, "output layout and prepared request differ");
   uint32_t need = s->planar ? o->channels : 1;
+  if ((o->kind == OUT_BITS || o->kind == OUT_PACKED16 || o->kind == OUT_RGB101010 || o->kind == OUT_RGBE) &&
+      s->planar)
+    FAIL(JXRHIP_ERR_INVALID_PLAN, "packed and one-bit outputs are interleaved");
   if (s->plane_count != need) FAIL(JXRHIP_ERR_INVALID_PLAN, "surface plane count");
   for (uint32_t i = 0; i < need; i++) {
     const jxrhip_surface_plane *sp = &s->planes[i];
     uint32_t ch = s->planar ? 1 : o->channels;
-    if (sp->width != s->width || sp->height != s->height || sp->channels != ch)
+    uint32_t pw = s->width, ph = s->height;
+    if (o->kind == OUT_YUV && (i == 1 || i == 2)) {
+      /* planar.rs:77-106: chroma planes at the coded sampling */
+      pw = s->width / 2;
+      ph = cf == JXRHIP_COLOR_YUV420 ? s->height / 2 : s->height;
+      if (pw == 0 || ph == 0) FAIL(JXRHIP_ERR_INVALID_PLAN, "planar chroma plane is empty");
+    }
+    if (sp->width != pw || sp->height != ph || sp->channels != ch)
       FAIL(JXRHIP_ERR_INVALID_PLAN, "surface plane geometry");
     uint64_t row = (uint64_t)sp->width * ch * o->bytes;
+    if (o->kind == OUT_BITS) row = ((uint64_t)sp->width + 7) / 8;              /* format.rs:216 */
+    if (o->kind == OUT_PACKED16 || o->kind == OUT_RGB101010 || o->kind == OUT_RGBE)
+      row = (uint64_t)sp->width * o->bytes;                                      /* one word a pixel */
     if (sp->row_stride_bytes < row) FAIL(JXRHIP_ERR_INVALID_PLAN, "surface row stride");
     if (sp->byte_offset + (uint64_t)(sp->height - 1) * sp->row_stride_bytes + row > s->byte_len)
       FAIL(JXRHIP_ERR_INVALID_PLAN, "surface plane exceeds its byte length");
@@ -1332,8 +1389,9 @@ static int output_setup(out_ctx *o, const jxrhip_image_desc *d, image_t *im, cha
   /* crop inside every plane's window (plan.rs:282-354) */
   for (uint32_t i = 0; i < im->count; i++) {
     const plane_t *p = &im->p[i];
-    int chroma = (int)i < (int)im->colors && (i == 1 || i == 2) && o->upsample;
-    uint32_t dx = chroma ? 2 : 1, dy = (chroma && o->upsample == 1) ? 2 : 1;
+    int sub = cf == JXRHIP_COLOR_YUV420 ? 1 : cf == JXRHIP_COLOR_YUV422 ? 2 : 0;
+    int chroma = (int)i < (int)im->colors && (i == 1 || i == 2) && sub;
+    uint32_t dx = chroma ? 2 : 1, dy = (chroma && sub == 1) ? 2 : 1;
     uint64_t left = f->crop_x / dx, top = f->crop_y / dy;
     uint64_t right = ((uint64_t)f->crop_x + f->crop_width - 1) / dx + 1;
     uint64_t bottom = ((uint64_t)f->crop_y + f->crop_height - 1) / dy + 1;
@@ -1346,6 +1404,116 @@ bad_depth:
   FAIL(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "bit depth does not match the storage kind");
 }
 
+/* ---- packed words (packed_color.rs:9-38, 100-135) ---------------------- */
+static uint32_t clip_unsigned(int32_t v, uint32_t maximum) {
+  return v < 0 ? 0u : ((uint32_t)v > maximum ? maximum : (uint32_t)v);
+}
+
+/* RGBE (rgbe.rs:6-100): each channel as a mantissa and exponent, aligned to
+ * the largest exponent with a rounded shift, bytes [R, G, B, E]. */
+static void rgbe_component(int32_t v, int32_t *mantissa, int32_t *exponent) {
+  if (v <= 0) {
+    *mantissa = 0;
+    *exponent = 0;
+  } else if ((v >> 7) > 1) {
+    *mantissa = (v & 0x7f) + 128;
+    *exponent = v >> 7;
+  } else {
+    *mantissa = v;
+    *exponent = 1;
+  }
+}
+static int32_t rgbe_rounded_shift(int32_t value, int32_t difference) {
+  if (difference >= 31) return 0;
+  return (int32_t)((2 * (int64_t)value + 1) >> (difference + 1));
+}
+uint32_t jxro_rgbe_word(const int32_t rgb[3]) {
+  int32_t m[3], e[3], top = 0;
+  for (int c = 0; c < 3; c++) {
+    rgbe_component(rgb[c], &m[c], &e[c]);
+    if (e[c] > top) top = e[c];
+  }
+  uint32_t word = 0;
+  for (int c = 0; c < 3; c++) {
+    int32_t v = top > e[c] ? rgbe_rounded_shift(m[c], top - e[c]) : m[c];
+    word |= (uint32_t)(v < 0 ? 0 : v > 255 ? 255 : v) << (8 * c);
+  }
+  return word | (uint32_t)(top > 255 ? 255 : top) << 24;
+}
+
+/* One packed word of converted RGB (packed_color.rs:48-98). */
+static int packed_word(const out_ctx *o, const int32_t rgb[3], uint32_t *word) {
+  int32_t v[3];
+  for (int c = 0; c < 3; c++)
+    if (apply_scale(&o->color_scale[c], rgb[c], &v[c])) return 1;
+  switch (o->kind) {
+    case OUT_PACKED16:
+      if (o->depth == JXRHIP_DEPTH_RGB565)
+        *word = clip_unsigned(v[2], 31) | clip_unsigned(v[1], 63) << 5 | clip_unsigned(v[0], 31) << 11;
+      else
+        *word = clip_unsigned(v[2], 31) | clip_unsigned(v[1], 31) << 5 | clip_unsigned(v[0], 31) << 10;
+      return 0;
+    case OUT_RGB101010:
+      *word = clip_unsigned(v[2], 1023) | clip_unsigned(v[1], 1023) << 10 | clip_unsigned(v[0], 1023) << 20;
+      return 0;
+    default:
+      *word = jxro_rgbe_word(v);
+      return 0;
+  }
+}
+
+/* One-bit luma rows (packing.rs:364-386): the first pixel of a row is the
+ * highest bit of its first byte; padding bits stay zero. */
+static int format_bits(const out_ctx *o, uint8_t *out) {
+  const jxrhip_output_format *f = &o->d->output;
+  const jxrhip_surface_plane *sp = &o->d->surface.planes[0];
+  uint32_t row_bytes = (f->crop_width + 7) / 8;
+  for (uint32_t y = 0; y < f->crop_height; y++) {
+    uint8_t *row = out + sp->byte_offset + (uint64_t)y * sp->row_stride_bytes;
+    memset(row, 0, row_bytes);
+    for (uint32_t x = 0; x < f->crop_width; x++) {
+      int32_t conv[4], v;
+      int r = color_at(o, f->crop_x + x, f->crop_y + y, conv);
+      if (r) return r;
+      if (apply_scale(&o->color_scale[0], conv[0], &v)) return 1;
+      uint32_t bit = v < 0 ? 0u : v > 1 ? 1u : (uint32_t)v;
+      if (o->depth == JXRHIP_DEPTH_BIT1_BLACK) bit = 1 - bit;
+      row[x / 8] |= (uint8_t)(bit << (7 - x % 8));
+    }
+  }
+  return 0;
+}
+
+/* Native-resolution planar YUV (planar.rs:77-230): luma and alpha over the
+ * crop, each chroma plane over the crop at its coded sampling. */
+static int format_yuv(const out_ctx *o, uint8_t *out) {
+  const jxrhip_output_format *f = &o->d->output;
+  const jxrhip_surface_layout *s = &o->d->surface;
+  const image_t *im = o->im;
+  int cs420 = o->d->color_format == JXRHIP_COLOR_YUV420;
+  for (uint32_t pl = 0; pl < s->plane_count; pl++) {
+    const jxrhip_surface_plane *sp = &s->planes[pl];
+    int chroma = pl == 1 || pl == 2, alpha = pl == 3;
+    const plane_t *src = alpha ? &im->p[im->alpha_index] : &im->p[pl];
+    const channel_scale *cs = alpha ? &o->alpha_scale : &o->color_scale[pl];
+    uint32_t x0 = chroma ? f->crop_x / 2 : f->crop_x;
+    uint32_t y0 = chroma && cs420 ? f->crop_y / 2 : f->crop_y;
+    for (uint32_t y = 0; y < sp->height; y++)
+      for (uint32_t x = 0; x < sp->width; x++) {
+        int32_t v;
+        if (apply_scale(cs, plane_at(src, x0 + x, y0 + y), &v)) return 1;
+        switch (f->storage) {
+          case JXRHIP_STORE_U8: v = v < 0 ? 0 : v > 255 ? 255 : v; break;
+          case JXRHIP_STORE_U16: v = v < 0 ? 0 : v > o->u16_max ? o->u16_max : v; break;
+          default: v = v < -32768 ? -32768 : v > 32767 ? 32767 : v; break;
+        }
+        store_le(out + sp->byte_offset + (uint64_t)y * sp->row_stride_bytes + (uint64_t)x * o->bytes,
+                 (uint32_t)v, o->bytes);
+      }
+  }
+  return 0;
+}
+
 /* Stores every cropped pixel (packing.rs:333-357 ordered stores, then
  * premultiply.rs on the stored values). */
 static int format_output(const out_ctx *o, uint8_t *out) {
@@ -1354,6 +1522,22 @@ static int format_output(const out_ctx *o, uint8_t *out) {
   const jxrhip_surface_layout *s = &d->surface;
   const image_t *im = o->im;
   uint32_t st = f->storage;
+  if (o->kind == OUT_BITS) return format_bits(o, out);
+  if (o->kind == OUT_YUV) return format_yuv(o, out);
+  if (o->kind != OUT_TYPED) {
+    const jxrhip_surface_plane *sp = &s->planes[0];
+    for (uint32_t y = 0; y < f->crop_height; y++)
+      for (uint32_t x = 0; x < f->crop_width; x++) {
+        int32_t conv[4];
+        uint32_t word;
+        int r = color_at(o, f->crop_x + x, f->crop_y + y, conv);
+        if (r) return r;
+        if (packed_word(o, conv, &word)) return 1;
+        store_le(out + sp->byte_offset + (uint64_t)y * sp->row_stride_bytes + (uint64_t)x * o->bytes, word,
+                 o->bytes);
+      }
+    return 0;
+  }
   int ncomp = d->output.output_color == JXRHIP_COLOR_NCOMPONENT;
   for (uint32_t y = 0; y < f->crop_height; y++)
     for (uint32_t x = 0; x < f->crop_width; x++) {

@@ -62,6 +62,8 @@ int jxro_predict_high_pass(int32_t coefficients[256], uint32_t direction,
                            uint32_t block_columns, uint32_t block_rows);
 int jxro_premultiply_unsigned(uint32_t value, uint32_t alpha, uint32_t maximum, uint32_t *out);
 int jxro_premultiply_signed(int32_t value, int32_t alpha, int32_t maximum, int32_t *out);
+/* RGBE word of three scaled channels (rgbe.rs:6-100): bytes R, G, B, E. */
+uint32_t jxro_rgbe_word(const int32_t rgb[3]);
 int jxro_float32_bits(int32_t sample, uint32_t scaled, uint32_t mantissa_length,
                       int32_t exponent_bias, uint32_t *bits);
 int jxro_float16_bits(int32_t sample, uint32_t scaled, uint16_t *bits);

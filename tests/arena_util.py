@@ -100,7 +100,8 @@ def desc_for(arena: ManualArena, color, components, spec: OutputSpec, crop, scal
     if separate_alpha is not None:
         d.separate_alpha = C.pointer(separate_alpha.arena)
     o = d.output
-    o.internal_color = abi.COLOR_YUV444 if color in (abi.COLOR_YUV420, abi.COLOR_YUV422) else color
+    native = spec.output_color in (abi.COLOR_YUV420, abi.COLOR_YUV422)  # planar YUV keeps the sampling
+    o.internal_color = abi.COLOR_YUV444 if color in (abi.COLOR_YUV420, abi.COLOR_YUV422) and not native else color
     o.output_color = spec.output_color
     o.bit_depth = spec.depth
     o.shift_bits = spec.shift_bits

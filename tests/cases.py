@@ -107,4 +107,40 @@ CASES = {
     "rgb8_420_nchw": dict(w=144, h=80, ov=2, quality=70, color=abi.COLOR_YUV420, out=dict(RGB8, planar=True)),
     "rgb8_region_crop": dict(w=160, h=128, ov=2, quality=75, color=abi.COLOR_YUV420, crop=(37, 21, 77, 58)),
     "rgb8_region_crop_444": dict(w=128, h=96, ov=1, quality=75, crop=(16, 32, 48, 40)),
+    # packed words, one-bit luma and native planar YUV
+    "rgb8_rgb555": dict(w=96, h=64, ov=1, quality=80,
+                        out=dict(storage=abi.STORE_PACKED16, depth=abi.DEPTH_RGB555, layout=abi.LAYOUT_RGB,
+                                 output_color=abi.COLOR_RGB)),
+    "rgb8_420_rgb565": dict(w=112, h=80, ov=2, quality=70, color=abi.COLOR_YUV420, crop=(5, 3, 97, 70),
+                            out=dict(storage=abi.STORE_PACKED16, depth=abi.DEPTH_RGB565, layout=abi.LAYOUT_RGB,
+                                     output_color=abi.COLOR_RGB)),
+    "rgb16_rgb101010": dict(w=80, h=64, ov=1, quality=75, image="rgb16",
+                            out=dict(storage=abi.STORE_PACKED32, depth=abi.DEPTH_RGB101010, layout=abi.LAYOUT_RGB,
+                                     output_color=abi.COLOR_RGB)),
+    "rgb16_rgbe": dict(w=80, h=48, ov=2, quality=85, image="rgb16",
+                       out=dict(storage=abi.STORE_PACKED32, depth=abi.DEPTH_U8, layout=abi.LAYOUT_RGB,
+                                output_color=abi.COLOR_RGBE)),
+    "gray8_to_rgb555": dict(w=64, h=48, ov=1, quality=80, image="gray8", color=abi.COLOR_LUMA,
+                            out=dict(storage=abi.STORE_PACKED16, depth=abi.DEPTH_RGB555, layout=abi.LAYOUT_RGB,
+                                     output_color=abi.COLOR_RGB)),
+    "gray8_bits_white": dict(w=160, h=48, ov=1, quality=70, image="gray8", color=abi.COLOR_LUMA,
+                             out=dict(storage=abi.STORE_BITS, depth=abi.DEPTH_BIT1_WHITE, layout=abi.LAYOUT_LUMA,
+                                      output_color=abi.COLOR_LUMA)),
+    "gray8_bits_black_crop": dict(w=176, h=64, ov=2, quality=60, image="gray8", color=abi.COLOR_LUMA,
+                                  crop=(13, 5, 150, 51),
+                                  out=dict(storage=abi.STORE_BITS, depth=abi.DEPTH_BIT1_BLACK,
+                                           layout=abi.LAYOUT_LUMA, output_color=abi.COLOR_LUMA)),
+    "yuv420_planar_u8": dict(w=144, h=96, ov=2, quality=70, color=abi.COLOR_YUV420,
+                             out=dict(storage=abi.STORE_U8, depth=abi.DEPTH_U8, layout=abi.LAYOUT_YUV,
+                                      output_color=abi.COLOR_YUV420)),
+    "yuv420_planar_crop": dict(w=128, h=80, ov=1, quality=80, color=abi.COLOR_YUV420, crop=(7, 9, 101, 63),
+                               out=dict(storage=abi.STORE_U8, depth=abi.DEPTH_U8, layout=abi.LAYOUT_YUV,
+                                        output_color=abi.COLOR_YUV420)),
+    "yuv422_planar_u16": dict(w=96, h=64, ov=2, quality=75, color=abi.COLOR_YUV422, image="rgb16",
+                              out=dict(storage=abi.STORE_U16, depth=abi.DEPTH_U16, layout=abi.LAYOUT_YUV,
+                                       output_color=abi.COLOR_YUV422)),
+    "yuv420a_planar_i16": dict(w=80, h=64, ov=1, quality=80, color=abi.COLOR_YUV420, image="rgba8",
+                               alpha=abi.ALPHA_SEPARATE,
+                               out=dict(storage=abi.STORE_I16, depth=abi.DEPTH_I16, layout=abi.LAYOUT_YUVA,
+                                        output_color=abi.COLOR_YUV420)),
 }

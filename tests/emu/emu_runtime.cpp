@@ -95,6 +95,10 @@ hipError_t hipMemsetAsync(void *dst, int value, size_t bytes, hipStream_t) {
   memset(dst, value, bytes);
   return hipSuccess;
 }
+hipError_t hipMemset2DAsync(void *dst, size_t pitch, int value, size_t width, size_t height, hipStream_t) {
+  for (size_t r = 0; r < height; r++) memset((char *)dst + r * pitch, value, width);
+  return hipSuccess;
+}
 hipError_t hipFuncSetAttribute(const void *, hipFuncAttribute, int) { return hipSuccess; }
 hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *blocks, const void *, int, size_t lds) {
   *blocks = (int)std::max<size_t>(1, std::min<size_t>(4, 65536 / (lds + 2048)));

@@ -124,6 +124,7 @@ hipError_t hipHostFree(void *p);
 hipError_t hipMemcpy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind);
 hipError_t hipMemcpyAsync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s);
 hipError_t hipMemsetAsync(void *dst, int value, size_t bytes, hipStream_t s);
+hipError_t hipMemset2DAsync(void *dst, size_t pitch, int value, size_t width, size_t height, hipStream_t s);
 hipError_t hipFuncSetAttribute(const void *f, hipFuncAttribute attr, int value);
 hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *blocks, const void *f, int threads, size_t lds);
 

@@ -407,3 +407,135 @@ def test_large_sample_cases_reach_beyond_the_plain_output_bound():
         peak = max(int(np.abs(O.reconstruct_plane(desc, i).astype(np.int64)).max())
                    for i in range(desc.component_count))
         assert peak > 2 ** 27, name
+
+
+# ---- packed, RGBE, one-bit and planar YUV outputs ---------------------------
+def _rgbe(r, g, b):
+    out = (C.c_int32 * 3)(r, g, b)
+    O.lib().jxro_rgbe_word.restype = C.c_uint32
+    w = O.lib().jxro_rgbe_word(out)
+    return [w & 0xFF, w >> 8 & 0xFF, w >> 16 & 0xFF, w >> 24]
+
+
+def test_rgbe_postscale_known_answers():
+    # jxr-math/src/rgbe.rs:129-147
+    assert _rgbe(128, 256, 64) == [64, 128, 32, 2]
+    assert _rgbe(-1, 0, 127) == [0, 0, 127, 1]
+    assert _rgbe(2**31 - 1, 0, 0)[3] == 255
+
+
+def _prescale(rgbe):
+    """rgbe.rs:27-52: the inverse the reference pins postscale against."""
+    out = []
+    for m in rgbe[:3]:
+        e = rgbe[3]
+        v = 0
+        if m and e:
+            while m < 128 and e > 1:
+                m, e = m << 1, e - 1
+            v = (e << 7) | (m & 0x7F) if e > 1 else m
+        out.append(v)
+    return out
+
+
+def test_rgbe_postscale_inverts_prescale_on_canonical_pixels():
+    # rgbe.rs:104-127
+    mantissas = [0, 1, 2, 63, 64, 127, 128, 129, 200, 255]
+    for e in (1, 2, 3, 100, 128, 254, 255):
+        for r in mantissas:
+            for g in mantissas:
+                for b in mantissas:
+                    top = max(r, g, b)
+                    if not (top >= 128 or (e == 1 and top != 0)):
+                        continue
+                    assert _rgbe(*_prescale([r, g, b, e])) == [r, g, b, e]
+    assert _rgbe(*_prescale([0, 0, 0, 0])) == [0, 0, 0, 0]
+
+
+RGB555 = OutputSpec(storage=abi.STORE_PACKED16, depth=abi.DEPTH_RGB555, layout=abi.LAYOUT_RGB,
+                    output_color=abi.COLOR_RGB)
+
+
+def test_packed_rgb555_bit_positions():
+    # output_format/tests.rs:277-293 (luma -17, 15 -> 0, 0x7fff)
+    assert _decode_row([[-17, 15]], abi.COLOR_LUMA, 1, RGB555).view(np.uint16).tolist() == [0, 0x7FFF]
+
+
+def test_packed_rgb555_places_blue_low_and_red_high():
+    # output_format/tests.rs:295-318 (Y 10, U 3, V 5 -> 0x5b9b)
+    got = _decode_row([[10, 10], [3, 3], [5, 5]], abi.COLOR_YUV444, 3, RGB555)
+    assert got.view(np.uint16).tolist() == [0x5B9B, 0x5B9B]
+
+
+def test_packed_rgb565_and_101010_fields():
+    # packed_color.rs:20-38 field positions (red highest; 565 green six bits,
+    # red and blue shifted one more, scaling.rs:87); biases 32 and 512
+    # (scaling.rs:299-318)
+    spec = OutputSpec(storage=abi.STORE_PACKED16, depth=abi.DEPTH_RGB565, layout=abi.LAYOUT_RGB,
+                      output_color=abi.COLOR_RGB)
+    assert _decode_row([[0, 31]], abi.COLOR_LUMA, 1, spec).view(np.uint16).tolist() == [
+        16 << 11 | 32 << 5 | 16, 0xFFFF]
+    spec = OutputSpec(storage=abi.STORE_PACKED32, depth=abi.DEPTH_RGB101010, layout=abi.LAYOUT_RGB,
+                      output_color=abi.COLOR_RGB)
+    assert _decode_row([[0, -600]], abi.COLOR_LUMA, 1, spec).view(np.uint32).tolist() == [
+        512 << 20 | 512 << 10 | 512, 0]
+
+
+def test_rgbe_output_through_decode():
+    spec = OutputSpec(storage=abi.STORE_PACKED32, depth=abi.DEPTH_U8, layout=abi.LAYOUT_RGB,
+                      output_color=abi.COLOR_RGBE)
+    # YUV (Y, 0, 0) is grey Y in every channel; RGBE takes no bias
+    got = _decode_row([[256, 100], [0, 0], [0, 0]], abi.COLOR_YUV444, 3, spec).view(np.uint32).tolist()
+    assert [[w & 0xFF, w >> 8 & 0xFF, w >> 16 & 0xFF, w >> 24] for w in got] == [_rgbe(256, 256, 256),
+                                                                                [100, 100, 100, 1]]
+
+
+def test_bit_packing_is_msb_first_and_black_inverts():
+    # output_format/tests.rs:664-679
+    black = OutputSpec(storage=abi.STORE_BITS, depth=abi.DEPTH_BIT1_BLACK, layout=abi.LAYOUT_LUMA,
+                       output_color=abi.COLOR_LUMA)
+    white = OutputSpec(storage=abi.STORE_BITS, depth=abi.DEPTH_BIT1_WHITE, layout=abi.LAYOUT_LUMA,
+                       output_color=abi.COLOR_LUMA)
+    assert _decode_row([[0, 1]], abi.COLOR_LUMA, 1, black).tolist() == [0x80]
+    assert _decode_row([[0, 1]], abi.COLOR_LUMA, 1, white).tolist() == [0x40]
+
+
+def test_bit_rows_pad_with_zero_bits():
+    # packed_color.rs:44-46: pixel x at byte x/8, bit 7 - x%8; rows are whole bytes
+    white = OutputSpec(storage=abi.STORE_BITS, depth=abi.DEPTH_BIT1_WHITE, layout=abi.LAYOUT_LUMA,
+                       output_color=abi.COLOR_LUMA)
+    arena = ManualArena([np.array([[1, 0]], np.int64)])
+    d = desc_for(arena, abi.COLOR_LUMA, 1, white, (3, 0, 21, 2))
+    got = O.decode(d).reshape(2, 3)
+    # columns 3..15 are 1 (13 pixels), 16..23 are 0
+    row = [0xFF, 0xF8, 0x00]
+    assert got.tolist() == [row, row]
+
+
+def test_planar_yuv420_keeps_native_chroma():
+    # planar.rs:77-123: Y over the crop, U and V at half resolution both ways
+    spec = OutputSpec(storage=abi.STORE_U8, depth=abi.DEPTH_U8, layout=abi.LAYOUT_YUV,
+                      output_color=abi.COLOR_YUV420)
+    arena = ManualArena([np.array([[10, -20]], np.int64), np.array([[3, 4]], np.int64),
+                         np.array([[-5, 6]], np.int64)], geometry=[(4, 4), (2, 2), (2, 2)])
+    d = desc_for(arena, abi.COLOR_YUV420, 3, spec, (0, 0, 32, 16))
+    got = O.decode(d)
+    y = got[:32 * 16].reshape(16, 32)
+    u = got[512:512 + 16 * 8].reshape(8, 16)
+    v = got[512 + 128:].reshape(8, 16)
+    assert (y[:, :16] == 138).all() and (y[:, 16:] == 108).all()
+    # each chroma plane is its reconstructed plane, unfiltered, biased by 128
+    for plane, index in ((u, 1), (v, 2)):
+        want = np.clip(O.reconstruct_plane(d, index).astype(np.int64) + 128, 0, 255)
+        assert want.shape == (8, 16) and (plane == want).all()
+    assert len(np.unique(u[:, :8])) == 1 and u[0, 0] != u[0, 8]
+
+
+def test_planar_yuv_rejects_premultiply_and_other_samplings():
+    # validate.rs:114-121, 205-215
+    spec = OutputSpec(storage=abi.STORE_U8, depth=abi.DEPTH_U8, layout=abi.LAYOUT_YUV,
+                      output_color=abi.COLOR_YUV422)
+    arena = ManualArena([np.zeros((1, 1), np.int64)] * 3, geometry=[(4, 4), (2, 2), (2, 2)])
+    d = desc_for(arena, abi.COLOR_YUV420, 3, spec, (0, 0, 16, 16))
+    code, _, _ = O.decode_status(d)
+    assert code == abi.ERR_UNSUPPORTED_OUTPUT
