@@ -371,6 +371,7 @@ struct PlaneGeo {
   int lox, loy;                // low-region origin
   uint32_t lrw, lrh;
   uint32_t lw, lh;             // low plane size
+  uint32_t pad[14];            // 128 bytes: per-plane indexing is a shift
 };
 
 __device__ __forceinline__ bool mask_bit(const uint32_t *m, uint32_t i) {
@@ -1569,6 +1570,7 @@ struct PlaneWork {
   uint32_t wB2, mB2; // stage B2: cell columns
   uint32_t wC, mC;   // stage C: blocks per row
   uint32_t wD, mD;   // stage D: overlap slots per row
+  uint32_t pad[6];   // 64 bytes: per-plane indexing is a shift
 };
 struct TileShared {
   DevImage im;
@@ -1688,9 +1690,13 @@ __device__ __forceinline__ void tile_finish(TileShared &ts) {
 
 // One tile of one image. `reload` is false when the previous tile of this
 // workgroup belonged to the same image, whose descriptor is still in LDS.
+// `claim_from` (thread 0 only; null for none) is a work counter to claim the
+// workgroup's next item from: the atomic is issued when stage A starts, so
+// its latency hides behind the stage's loads, and its result lands in
+// `claim`.
 template <uint32_t STORE>
 __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, const KernelArgs &args, int *lds,
-                                             TileShared &ts, bool reload) {
+                                             TileShared &ts, bool reload, uint32_t *claim_from, uint32_t &claim) {
   __syncthreads();  // the previous tile's output stage has read the tile state
   if (reload) {
     const JXR_GLOBAL v4u *src = (const JXR_GLOBAL v4u *)gptr(slot.img);
@@ -1707,6 +1713,8 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
   const int np = (int)im.plane_count;
   const int tmx = ts.tmx, tmy = ts.tmy, nx = ts.nx, ny = ts.ny;
   uint32_t *status = slot.status;
+
+  if (threadIdx.x == 0 && claim_from) claim = atomicAdd(claim_from, 1u);
 
   // stage A: first level of the tile's macroblocks and their halo
   for (int t = threadIdx.x; t < ts.sA[np]; t += blockDim.x) {
@@ -1828,49 +1836,77 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
   }
 }
 
-// Work assignment: workgroup b takes an arithmetic sequence of work items.
-// Workgroups are dispatched to the eight XCDs round-robin (b mod 8), and
-// each XCD's workgroups walk one contiguous eighth of the items, so tiles
-// processed at the same time on one XCD are neighbours whose halo reads meet
-// in that XCD's L2. Per-image launches run one item per workgroup (the grid
-// is rounded up to a multiple of 8); persistent launches size the grid to
-// the device and loop. Item i belongs to image item_slot[i].
+// Work assignment. Workgroups are dispatched to the eight XCDs round-robin
+// (b mod 8), and the items of a launch are split into eight contiguous
+// shares, one per XCD, so tiles processed at the same time on one XCD are
+// neighbours whose halo reads meet in its L2. Workgroup b takes item b/8 of
+// its XCD's share first; after that it claims items from the share's counter
+// (the first G/8 items of each share are the static ones), and once its share
+// is drained it moves on to the other shares, so XCDs that run slower or
+// finish later still end together. Per-image launches run one item per
+// workgroup (the grid is rounded up to a multiple of 8). Item i belongs to
+// image item_slot[i].
 //
 // One call site keeps process_tile inlined once, so its LDS pointers stay
 // LDS-addressed. One instantiation per storage kind keeps each kernel's
 // registers and code to what its output format needs; every image of a
 // launch shares the kind.
+struct Shares {
+  uint32_t n, count, per;  // items, shares, static items per share
+  __device__ __forceinline__ uint32_t lo(uint32_t r) const {
+    return count == 1 ? 0u : (uint32_t)(((uint64_t)n * r) >> 3);
+  }
+  __device__ __forceinline__ uint32_t hi(uint32_t r) const {
+    return count == 1 ? n : (uint32_t)(((uint64_t)n * (r + 1)) >> 3);
+  }
+};
+constexpr uint32_t kNoItem = 0xFFFFFFFFu;
+constexpr uint32_t kCounterStride = 32;  // words between share counters (own cache lines)
+
+// Thread 0: the next item from share r onwards, given a claimed index of r.
+__device__ __forceinline__ uint32_t next_item(const Shares &sh, uint32_t *counters, uint32_t &r, uint32_t &tries,
+                                              uint32_t claimed) {
+  for (;;) {
+    const uint32_t it = sh.lo(r) + sh.per + claimed;
+    if (claimed < sh.hi(r) - sh.lo(r) && it < sh.hi(r)) return it;
+    if (++tries >= sh.count) return kNoItem;
+    r = r + 1 == sh.count ? 0u : r + 1;
+    claimed = atomicAdd(counters + r * kCounterStride, 1u);
+  }
+}
+
 template <uint32_t STORE>
 __global__ void __launch_bounds__(kThreads, kMinWavesPerSimd) jxr_reconstruct_tiles(KernelArgs args) {
   HIP_DYNAMIC_SHARED(v4i, lds_quads)  // 16-byte aligned: regions are read and written in quads
   int *lds = reinterpret_cast<int *>(lds_quads);
   __shared__ TileShared ts;
-  const uint32_t G = gridDim.x, b = blockIdx.x, n = args.item_count;
-  uint32_t item, stride, end;
-  if ((G & 7u) == 0) {
-    const uint32_t x = b & 7u;
-    stride = G >> 3;
-    item = (uint32_t)(((uint64_t)n * x) >> 3) + (b >> 3);
-    end = (uint32_t)(((uint64_t)n * (x + 1)) >> 3);
-  } else {
-    item = b;
-    stride = G;
-    end = n;
+  __shared__ uint32_t shared_item;
+  const uint32_t G = gridDim.x, b = blockIdx.x;
+  const Shares sh{args.item_count, (G & 7u) == 0 ? 8u : 1u, (G & 7u) == 0 ? G >> 3 : G};
+  const uint32_t x = sh.count == 8 ? (b & 7u) : 0u;
+  uint32_t r = x, tries = 0;  // thread 0: share being drained, shares given up
+  if (threadIdx.x == 0) {
+    uint32_t first = sh.lo(x) + (sh.count == 8 ? b >> 3 : b);
+    if (first >= sh.hi(x))
+      first = args.persistent ? next_item(sh, args.counters, r, tries, atomicAdd(args.counters + r * kCounterStride, 1u))
+                              : kNoItem;
+    shared_item = first;
   }
-  if (item >= end) return;
+  __syncthreads();
+  uint32_t item = shared_item;
   const JXR_GLOBAL uint32_t *item_slot = gptr(args.item_slot);
-  uint32_t slot_index = args.persistent ? item_slot[item] : 0u;
-  uint32_t loaded = 0xFFFFFFFFu;
-  for (;;) {
-    const uint32_t next = item + stride;
-    // issued before the tile, used after it
-    const uint32_t next_slot = (args.persistent && next < end) ? item_slot[next] : 0u;
+  uint32_t loaded = kNoItem;
+  while (item != kNoItem) {
+    const uint32_t slot_index = args.persistent ? item_slot[item] : 0u;
     const Slot &slot = args.persistent ? args.slots[slot_index] : args.inline_slot;
-    process_tile<STORE>(slot, item - slot.tile_start, args, lds, ts, slot_index != loaded);
+    uint32_t claim = 0;
+    uint32_t *claim_from = args.persistent && tries < sh.count ? args.counters + r * kCounterStride : nullptr;
+    process_tile<STORE>(slot, item - slot.tile_start, args, lds, ts, slot_index != loaded, claim_from, claim);
     loaded = slot_index;
-    if (next >= end) break;
-    item = next;
-    slot_index = next_slot;
+    __syncthreads();  // every lane has finished with shared_item
+    if (threadIdx.x == 0) shared_item = claim_from ? next_item(sh, args.counters, r, tries, claim) : kNoItem;
+    __syncthreads();
+    item = shared_item;
   }
 }
 
@@ -1955,8 +1991,11 @@ hipError_t set_kernel_lds_limit(size_t bytes) {
 // item_slot[i] = index of the image that owns work item i: workgroup b
 // writes image b's items [tile_start_b, tile_start_{b+1}).
 __global__ void __launch_bounds__(kThreads) jxr_expand_slots(const Slot *slots, uint32_t slot_count,
-                                                             uint32_t item_count, uint32_t *item_slot) {
+                                                             uint32_t item_count, uint32_t *item_slot,
+                                                             uint32_t *counters) {
   const uint32_t b = blockIdx.x;
+  if (b == 0)  // the launch's share counters
+    for (uint32_t i = threadIdx.x; i < 8; i += blockDim.x) counters[i * kCounterStride] = 0;
   if (b >= slot_count) return;
   const uint32_t first = slots[b].tile_start;
   const uint32_t last = b + 1 < slot_count ? slots[b + 1].tile_start : item_count;
@@ -1964,9 +2003,9 @@ __global__ void __launch_bounds__(kThreads) jxr_expand_slots(const Slot *slots, 
 }
 
 hipError_t launch_expand(const Slot *slots, uint32_t slot_count, uint32_t item_count, uint32_t *item_slot,
-                         hipStream_t stream) {
+                         uint32_t *counters, hipStream_t stream) {
   hipLaunchKernelGGL(jxr_expand_slots, dim3(slot_count), dim3(kThreads), 0, stream, slots, slot_count, item_count,
-                     item_slot);
+                     item_slot, counters);
   return hipGetLastError();
 }
 

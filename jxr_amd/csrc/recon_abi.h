@@ -123,6 +123,7 @@ struct KernelArgs {
   Slot inline_slot;      // per-image launch: the one image
   const Slot *slots;     // persistent: images, ordered by tile_start
   const uint32_t *item_slot;  // persistent: owning image of each work item
+  uint32_t *counters;         // persistent: eight share counters, 128 bytes apart
   uint32_t slot_count;
   uint32_t item_count;   // work items (tiles) of the launch
   uint32_t persistent;

@@ -33,7 +33,7 @@
 namespace jxr {
 hipError_t launch_tiles(const KernelArgs &args, uint32_t grid, hipStream_t stream);
 hipError_t launch_expand(const Slot *slots, uint32_t slot_count, uint32_t item_count, uint32_t *item_slot,
-                         hipStream_t stream);
+                         uint32_t *counters, hipStream_t stream);
 hipError_t kernel_occupancy(uint32_t storage, size_t dynamic_lds, int *groups_per_cu);
 hipError_t set_kernel_lds_limit(size_t bytes);
 }  // namespace jxr
@@ -1308,6 +1308,12 @@ extern "C" jxrhip_status jxrhip_submit_batch(jxrhip_session *s, const jxrhip_pla
                       s->diag.scratch_misses),
           "item table");
   b->transient.push_back(item_slots);
+  // eight share counters per launch, each on its own 128-byte line
+  DeviceBuffer counters;
+  HIP_TRY(s->pool.get(std::max<size_t>(launches.size(), 1) * 8 * 128, counters, s->diag.scratch_hits,
+                      s->diag.scratch_misses),
+          "share counters");
+  b->transient.push_back(counters);
   HIP_TRY(hipEventRecord(s->t0, stream), "hipEventRecord");
   uint64_t item_base = 0;
   for (size_t li = 0; li < launches.size(); li++) {
@@ -1315,6 +1321,7 @@ extern "C" jxrhip_status jxrhip_submit_batch(jxrhip_session *s, const jxrhip_pla
     KernelArgs args{};
     args.slots = (const Slot *)slot_buf.ptr + L.first;
     args.item_slot = (const uint32_t *)item_slots.ptr + item_base;
+    args.counters = (uint32_t *)((uint8_t *)counters.ptr + li * 8 * 128);
     item_base += L.items;
     args.slot_count = L.n;
     args.item_count = L.items;
@@ -1330,7 +1337,7 @@ extern "C" jxrhip_status jxrhip_submit_batch(jxrhip_session *s, const jxrhip_pla
     const uint32_t per_cu = s->groups_per_cu ? s->groups_per_cu : (uint32_t)std::max(resident, 1);
     uint32_t grid = std::min<uint32_t>(L.items, s->cus * per_cu);
     grid = std::max<uint32_t>((grid + 7u) & ~7u, 8u);
-    HIP_TRY(launch_expand(args.slots, L.n, L.items, (uint32_t *)args.item_slot, stream), "item table");
+    HIP_TRY(launch_expand(args.slots, L.n, L.items, (uint32_t *)args.item_slot, args.counters, stream), "item table");
     HIP_TRY(launch_tiles(args, grid, stream), "persistent launch");
     s->diag.kernel_launches++;
   }

@@ -78,6 +78,7 @@ inline unsigned __float_as_uint(float f) {
   return u;
 }
 inline float __fmul_rn(float a, float b) { return a * b; }
+inline int __mul24(int a, int b) { return (int)((unsigned)(((a << 8) >> 8) * ((b << 8) >> 8))); }
 inline unsigned __umulhi(unsigned a, unsigned b) { return (unsigned)(((unsigned long long)a * b) >> 32); }
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
