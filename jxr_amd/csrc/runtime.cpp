@@ -1314,10 +1314,15 @@ extern "C" jxrhip_status jxrhip_submit_batch(jxrhip_session *s, const jxrhip_pla
                       s->diag.scratch_misses),
           "share counters");
   b->transient.push_back(counters);
+  // Buckets go round-robin over the session's streams, so that one bucket's
+  // workgroups fill the compute units another bucket's tail leaves idle.
   HIP_TRY(hipEventRecord(s->t0, stream), "hipEventRecord");
+  const uint32_t used_streams = (uint32_t)std::min<size_t>(launches.size(), kStreams);
+  for (uint32_t i = 1; i < used_streams; i++) HIP_TRY(hipStreamWaitEvent(s->streams[i], s->t0, 0), "hipStreamWaitEvent");
   uint64_t item_base = 0;
   for (size_t li = 0; li < launches.size(); li++) {
     const Launch &L = launches[li];
+    hipStream_t ls = s->streams[li % kStreams];
     KernelArgs args{};
     args.slots = (const Slot *)slot_buf.ptr + L.first;
     args.item_slot = (const uint32_t *)item_slots.ptr + item_base;
@@ -1337,9 +1342,16 @@ extern "C" jxrhip_status jxrhip_submit_batch(jxrhip_session *s, const jxrhip_pla
     const uint32_t per_cu = s->groups_per_cu ? s->groups_per_cu : (uint32_t)std::max(resident, 1);
     uint32_t grid = std::min<uint32_t>(L.items, s->cus * per_cu);
     grid = std::max<uint32_t>((grid + 7u) & ~7u, 8u);
-    HIP_TRY(launch_expand(args.slots, L.n, L.items, (uint32_t *)args.item_slot, args.counters, stream), "item table");
-    HIP_TRY(launch_tiles(args, grid, stream), "persistent launch");
+    HIP_TRY(launch_expand(args.slots, L.n, L.items, (uint32_t *)args.item_slot, args.counters, ls), "item table");
+    HIP_TRY(launch_tiles(args, grid, ls), "persistent launch");
     s->diag.kernel_launches++;
+  }
+  for (uint32_t i = 1; i < used_streams; i++) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    HIP_TRY(hipEventRecord(e, s->streams[i]), "hipEventRecord");
+    HIP_TRY(hipStreamWaitEvent(stream, e, 0), "hipStreamWaitEvent");
+    (void)hipEventDestroy(e);
   }
   HIP_TRY(hipEventRecord(s->t1, stream), "hipEventRecord");
   HIP_TRY(hipMemcpyAsync(b->host_statuses, b->statuses.ptr, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, stream),
