@@ -51,15 +51,16 @@ def percentile(xs, p):
     return xs[k]
 
 
-def time_path(session, plans, ptrs, mode, steps, warmup):
+def time_path(session, submit, steps, warmup):
+    """Times `steps` calls of submit().wait() after `warmup` untimed ones."""
     for _ in range(warmup):
-        session.submit_batch(plans, ptrs, mode).wait()
+        submit().wait()
     session.synchronize()
     walls, devs, launches = [], [], 0
     t_all = time.perf_counter()
     for _ in range(steps):
         t0 = time.perf_counter()
-        session.submit_batch(plans, ptrs, mode).wait()
+        submit().wait()
         walls.append((time.perf_counter() - t0) * 1e3)
         ms, launches = session.last_batch_timing()
         devs.append(ms)
@@ -113,6 +114,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-per-image", action="store_true")
     ap.add_argument("--tile", default=None, help="TXxTY macroblocks per work item")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shapes")
+    ap.add_argument("--no-graph", action="store_true", help="submit the batch eagerly every step")
     ap.add_argument("--calibrate", action="store_true",
                     help="also copy 256 MiB device-to-device (HBM counter calibration)")
     args = ap.parse_args()
@@ -157,13 +160,31 @@ def main():
         a.close()
         b.close()
 
-    batched = time_path(session, plans, ptrs, abi.BATCH_PERSISTENT, args.steps, args.warmup)
+    tuned = []
+    if not args.no_autotune and not args.tile:
+        t0 = time.perf_counter()
+        tuned = session.autotune(plans, ptrs, repeats=3)
+        for c in tuned:
+            log(f"[bench] autotune bucket {c['bucket_key']:#x}: {c['images']} images, tile "
+                f"{c['tile'][0]}x{c['tile'][1]} {c['best_ms']:.3f} ms (default {c['default_ms']:.3f} ms, "
+                f"{c['candidates']} shapes)")
+        log(f"[bench] autotune took {time.perf_counter() - t0:.1f} s")
+    graph = None
+    if args.no_graph:
+        def submit_batched():
+            return session.submit_batch(plans, ptrs, abi.BATCH_PERSISTENT)
+    else:
+        graph = session.capture_batch(plans, ptrs)
+        submit_batched = graph.launch
+    batched = time_path(session, submit_batched, args.steps, args.warmup)
     log(f"[bench] batched: median wall {np.median(batched['wall_ms']):.3f} ms, "
         f"device {np.median(batched['device_ms']):.3f} ms, launches/step {batched['launches']}")
+    if graph is not None:
+        graph.close()
     per_image = None
     if not args.no_per_image:
         k = max(3, args.steps // 2)
-        per_image = time_path(session, plans, ptrs, abi.BATCH_PER_IMAGE, k, 1)
+        per_image = time_path(session, lambda: session.submit_batch(plans, ptrs, abi.BATCH_PER_IMAGE), k, 1)
         log(f"[bench] per-image: median wall {np.median(per_image['wall_ms']):.3f} ms, "
             f"device {np.median(per_image['device_ms']):.3f} ms")
 
@@ -191,7 +212,10 @@ def main():
             "images": len(items),
             "megapixels_per_step": round(pixels / 1e6, 3),
             "algorithmic_gb_per_step": round(alg / 1e9, 4),
-            "path": "persistent batched kernel, one launch per shape bucket",
+            "path": "persistent batched kernel, one launch per shape bucket"
+                    + ("" if args.no_graph else ", replayed from a captured HIP graph")
+                    + (", autotuned tile shapes" if tuned else ""),
+            "tiles": {f"{c['bucket_key']:#x}": f"{c['tile'][0]}x{c['tile'][1]}" for c in tuned},
             "parallelism": "single-gpu",
         },
         "latency_ms": {"p50": round(percentile(batched["wall_ms"], 50), 4),

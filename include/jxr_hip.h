@@ -247,6 +247,7 @@ typedef struct jxrhip_plan jxrhip_plan;
 typedef struct jxrhip_submission jxrhip_submission;
 typedef struct jxrhip_image jxrhip_image;
 typedef struct jxrhip_batch jxrhip_batch;
+typedef struct jxrhip_batch_graph jxrhip_batch_graph;
 
 /* Counters of the session's pools and caches. */
 typedef struct jxrhip_diagnostics {
@@ -376,6 +377,57 @@ jxrhip_status jxrhip_session_set_tile(jxrhip_session *session, uint32_t tile_mbs
                                       uint32_t tile_mbs_y);
 /* Persistent grid: workgroups per compute unit (0 = automatic). */
 jxrhip_status jxrhip_session_set_occupancy(jxrhip_session *session, uint32_t groups_per_cu);
+
+/* Shape buckets. Plans with the same plane geometry (component count and
+ * block layout per plane) and storage kind share a kernel instantiation and
+ * a tile shape; jxrhip_plan_bucket_key names the bucket. The reference has
+ * no buckets (one launch sequence per image, crates/jxr-cuda/src/encode.rs:66-159). */
+uint64_t jxrhip_plan_bucket_key(const jxrhip_plan *plan);
+
+typedef struct jxrhip_tile_choice {
+  uint64_t bucket_key;
+  uint32_t tile_mbs_x, tile_mbs_y; /* the fastest shape, now in use          */
+  uint32_t images;                 /* images of the batch in this bucket     */
+  uint32_t candidates;             /* shapes timed                           */
+  double best_ms;                  /* median device time at that shape       */
+  double default_ms;               /* median device time at the untuned shape*/
+} jxrhip_tile_choice;
+
+/* Autotuner: times every candidate tile shape on each shape bucket of the
+ * batch (one persistent launch of the bucket alone, `repeats` timed runs
+ * after an untimed one) and keeps the fastest per bucket key for later
+ * submissions of this session, per-image and persistent. The destinations
+ * receive the images. Writes up to `capacity` choices and the bucket count.
+ * A fixed shape from jxrhip_session_set_tile still takes precedence. No
+ * counterpart in the reference. */
+jxrhip_status jxrhip_session_autotune(jxrhip_session *session, const jxrhip_plan *const *plans, uint32_t count,
+                                      void *const *device_dsts, uint32_t repeats, jxrhip_tile_choice *choices,
+                                      uint32_t capacity, uint32_t *bucket_count);
+/* Installs (or, with 0 x 0, removes) a bucket's tile shape, e.g. one saved
+ * from an earlier autotune; a shape whose LDS does not fit is ignored. */
+jxrhip_status jxrhip_session_set_bucket_tile(jxrhip_session *session, uint64_t bucket_key, uint32_t tile_mbs_x,
+                                             uint32_t tile_mbs_y);
+/* The bucket's installed shape, 0 x 0 when none. */
+jxrhip_status jxrhip_session_bucket_tile(jxrhip_session *session, uint64_t bucket_key, uint32_t *tile_mbs_x,
+                                         uint32_t *tile_mbs_y);
+
+/* ---- captured batches ---------------------------------------------------- */
+/* A persistent batch (JXRHIP_BATCH_PERSISTENT) captured once into a HIP graph:
+ * per shape bucket an item-table kernel and the persistent launch, buckets
+ * spread over the session's streams. jxrhip_batch_graph_launch replays it and
+ * returns a batch to wait on like jxrhip_submit_batch's. Every arena needs an
+ * identity: the graph keeps those uploads and the plans' descriptors resident
+ * until it is destroyed. Plans and destinations must outlive the graph. One
+ * replay at a time: launching again before the previous batch was waited for
+ * fails with JXRHIP_ERR_INVALID_ARGUMENT. Wait for the last replay before
+ * destroying the graph, and destroy graphs before their session. Replaces
+ * the reference's per-submission launch sequencing
+ * (crates/jxr-cuda/src/encode.rs:66-159, session.rs:103-211) for a batch
+ * decoded repeatedly into the same destinations. */
+jxrhip_status jxrhip_batch_graph_create(jxrhip_session *session, const jxrhip_plan *const *plans, uint32_t count,
+                                        void *const *device_dsts, jxrhip_batch_graph **out);
+jxrhip_status jxrhip_batch_graph_launch(jxrhip_batch_graph *graph, jxrhip_batch **out);
+void jxrhip_batch_graph_destroy(jxrhip_batch_graph *graph);
 
 #ifdef __cplusplus
 }

@@ -176,6 +176,18 @@ class Diagnostics(C.Structure):
     ]
 
 
+class TileChoice(C.Structure):
+    _fields_ = [
+        ("bucket_key", C.c_uint64),
+        ("tile_mbs_x", C.c_uint32),
+        ("tile_mbs_y", C.c_uint32),
+        ("images", C.c_uint32),
+        ("candidates", C.c_uint32),
+        ("best_ms", C.c_double),
+        ("default_ms", C.c_double),
+    ]
+
+
 class EncodeParams(C.Structure):
     _fields_ = [
         ("width", C.c_uint32),
@@ -250,6 +262,14 @@ SIGNATURES = {
     "jxrhip_last_batch_timing": (st, [vp, C.POINTER(C.c_double), u32p]),
     "jxrhip_session_set_tile": (st, [vp, C.c_uint32, C.c_uint32]),
     "jxrhip_session_set_occupancy": (st, [vp, C.c_uint32]),
+    "jxrhip_plan_bucket_key": (C.c_uint64, [vp]),
+    "jxrhip_session_autotune": (st, [vp, C.POINTER(vp), C.c_uint32, C.POINTER(vp), C.c_uint32,
+                                     C.POINTER(TileChoice), C.c_uint32, u32p]),
+    "jxrhip_session_set_bucket_tile": (st, [vp, C.c_uint64, C.c_uint32, C.c_uint32]),
+    "jxrhip_session_bucket_tile": (st, [vp, C.c_uint64, u32p, u32p]),
+    "jxrhip_batch_graph_create": (st, [vp, C.POINTER(vp), C.c_uint32, C.POINTER(vp), pp]),
+    "jxrhip_batch_graph_launch": (st, [vp, pp]),
+    "jxrhip_batch_graph_destroy": (None, [vp]),
     # jxr_hip_encode.h
     "jxrhip_encode": (st, [vp, C.c_uint64, C.POINTER(EncodeParams), pp]),
     "jxrhip_encoded_destroy": (None, [vp]),

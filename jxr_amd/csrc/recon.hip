@@ -1997,6 +1997,7 @@ __global__ void __launch_bounds__(kThreads) jxr_expand_slots(const Slot *slots, 
   if (b == 0)  // the launch's share counters
     for (uint32_t i = threadIdx.x; i < 8; i += blockDim.x) counters[i * kCounterStride] = 0;
   if (b >= slot_count) return;
+  if (threadIdx.x == 0) *slots[b].status = 0;  // the image's first failing stage
   const uint32_t first = slots[b].tile_start;
   const uint32_t last = b + 1 < slot_count ? slots[b + 1].tile_start : item_count;
   for (uint32_t i = first + threadIdx.x; i < last; i += blockDim.x) item_slot[i] = b;
@@ -2006,6 +2007,24 @@ hipError_t launch_expand(const Slot *slots, uint32_t slot_count, uint32_t item_c
                          uint32_t *counters, hipStream_t stream) {
   hipLaunchKernelGGL(jxr_expand_slots, dim3(slot_count), dim3(kThreads), 0, stream, slots, slot_count, item_count,
                      item_slot, counters);
+  return hipGetLastError();
+}
+
+// Zeroes `rows` rows of `row_bytes` bytes, `pitch` bytes apart (one-bit
+// output ORs its pixels into cleared rows). A kernel rather than a memset
+// node, so that it can sit in a captured graph.
+__global__ void __launch_bounds__(kThreads) jxr_clear_rows(uint8_t *base, uint64_t pitch, uint32_t row_bytes,
+                                                           uint32_t rows) {
+  for (uint32_t r = blockIdx.x; r < rows; r += gridDim.x) {
+    uint8_t *row = base + (uint64_t)r * pitch;
+    for (uint32_t i = threadIdx.x; i < row_bytes; i += blockDim.x) row[i] = 0;
+  }
+}
+
+hipError_t launch_clear_rows(uint8_t *base, uint64_t pitch, uint32_t row_bytes, uint32_t rows, hipStream_t stream) {
+  if (!rows || !row_bytes) return hipSuccess;
+  hipLaunchKernelGGL(jxr_clear_rows, dim3(rows < 4096u ? rows : 4096u), dim3(kThreads), 0, stream, base, pitch,
+                     row_bytes, rows);
   return hipGetLastError();
 }
 

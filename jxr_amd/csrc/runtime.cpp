@@ -32,6 +32,7 @@
 
 namespace jxr {
 hipError_t launch_tiles(const KernelArgs &args, uint32_t grid, hipStream_t stream);
+hipError_t launch_clear_rows(uint8_t *base, uint64_t pitch, uint32_t row_bytes, uint32_t rows, hipStream_t stream);
 hipError_t launch_expand(const Slot *slots, uint32_t slot_count, uint32_t item_count, uint32_t *item_slot,
                          uint32_t *counters, hipStream_t stream);
 hipError_t kernel_occupancy(uint32_t storage, size_t dynamic_lds, int *groups_per_cu);
@@ -646,6 +647,7 @@ struct CacheEntry {
   DeviceBuffer buf;
   uint64_t last_use = 0;
   const void *deps[2] = {nullptr, nullptr};  // coefficient buffers a descriptor points at
+  uint32_t pins = 0;  // prepared batches and graphs that read it; never evicted while pinned
 };
 
 struct jxrhip_session {
@@ -654,6 +656,7 @@ struct jxrhip_session {
   size_t max_lds = 65536;
   hipStream_t streams[kStreams] = {};
   hipEvent_t t0 = nullptr, t1 = nullptr;
+  hipEvent_t fork = nullptr, join[kStreams] = {};  // fan-out of a batch over the streams
   std::mutex lock;  // one submitter at a time (encode.rs:83-88)
   Pool pool;
   // coefficient uploads by arena identity; descriptor uploads by plan id
@@ -662,6 +665,8 @@ struct jxrhip_session {
   uint64_t cache_bytes = 0, cache_budget = uint64_t(64) << 30, tick = 0;
   uint32_t next_stream = 0;
   uint32_t tile_x = 0, tile_y = 0, groups_per_cu = 0;
+  std::map<uint64_t, std::pair<uint32_t, uint32_t>> tuned;  // bucket key -> tile shape
+  uint32_t live_graphs = 0;
   double last_ms = 0;
   uint32_t last_launches = 0;
   jxrhip_diagnostics diag{};
@@ -716,6 +721,9 @@ extern "C" jxrhip_status jxrhip_session_create(int32_t ordinal, jxrhip_session *
     HIP_TRY(hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking), "hipStreamCreate");
   HIP_TRY(hipEventCreate(&s->t0), "hipEventCreate");
   HIP_TRY(hipEventCreate(&s->t1), "hipEventCreate");
+  HIP_TRY(hipEventCreateWithFlags(&s->fork, hipEventDisableTiming), "hipEventCreate");
+  for (uint32_t i = 0; i < kStreams; i++)
+    HIP_TRY(hipEventCreateWithFlags(&s->join[i], hipEventDisableTiming), "hipEventCreate");
   size_t lds_limit = prop.sharedMemPerBlockOptin ? prop.sharedMemPerBlockOptin : prop.sharedMemPerBlock;
   s->max_lds = std::min<size_t>(lds_limit, 160 * 1024);
   if (s->max_lds > 65536 && set_kernel_lds_limit(s->max_lds) != hipSuccess) {
@@ -748,6 +756,9 @@ extern "C" void jxrhip_session_destroy(jxrhip_session *s) {
     if (s->streams[i]) (void)hipStreamDestroy(s->streams[i]);
   if (s->t0) (void)hipEventDestroy(s->t0);
   if (s->t1) (void)hipEventDestroy(s->t1);
+  if (s->fork) (void)hipEventDestroy(s->fork);
+  for (uint32_t i = 0; i < kStreams; i++)
+    if (s->join[i]) (void)hipEventDestroy(s->join[i]);
   delete s;
 }
 
@@ -790,7 +801,7 @@ static void evict_until(jxrhip_session *s, uint64_t need) {
     uint64_t victim = 0, oldest = UINT64_MAX;
     for (auto *m : {&s->coef_cache, &s->desc_cache})
       for (auto &kv : *m)
-        if (kv.second.last_use < oldest) {
+        if (!kv.second.pins && kv.second.last_use < oldest) {
           oldest = kv.second.last_use;
           victim = kv.first;
           victim_map = m;
@@ -898,12 +909,16 @@ static jxrhip_status device_image(jxrhip_session *s, const jxrhip_plan *p, hipSt
   // pageable copies are staged before the call returns, so `host` may go
   HIP_TRY(hipStreamSynchronize(stream), "descriptor upload");
   if (cacheable) {
+    // the eviction above may have removed the stale entry already
+    it = s->desc_cache.find(p->id);
+    uint32_t pins = 0;
     if (it != s->desc_cache.end()) {
       s->cache_bytes -= it->second.buf.bytes;
       s->pool.put(it->second.buf);
+      pins = it->second.pins;
     }
     s->cache_bytes += b.bytes;
-    CacheEntry ce{b, ++s->tick, {coef[0], coef[1]}};
+    CacheEntry ce{b, ++s->tick, {coef[0], coef[1]}, pins};
     s->desc_cache[p->id] = ce;
   } else {
     owner->transient.push_back(b);
@@ -931,47 +946,74 @@ extern "C" jxrhip_status jxrhip_session_clear_cache(jxrhip_session *s) {
   std::lock_guard<std::mutex> g(s->lock);
   make_current(s);
   for (uint32_t i = 0; i < kStreams; i++) (void)hipStreamSynchronize(s->streams[i]);
-  for (auto &kv : s->coef_cache) s->pool.put(kv.second.buf);
-  for (auto &kv : s->desc_cache) s->pool.put(kv.second.buf);
-  s->coef_cache.clear();
-  s->desc_cache.clear();
-  s->cache_bytes = 0;
+  // entries that a prepared batch or a graph still reads stay
+  for (auto *m : {&s->coef_cache, &s->desc_cache})
+    for (auto it = m->begin(); it != m->end();) {
+      if (it->second.pins) {
+        ++it;
+        continue;
+      }
+      s->cache_bytes -= it->second.buf.bytes;
+      s->pool.put(it->second.buf);
+      it = m->erase(it);
+    }
   s->pool.trim(0);
   return JXRHIP_OK;
 }
 
-// Tile shape of a plan set: the largest candidate whose LDS fits the budget.
-static void choose_tile(const jxrhip_session *s, const jxrhip_plan *p, uint32_t &tx, uint32_t &ty,
-                        uint32_t &words) {
-  // the kernel keeps a sample region and a low region for every plane
-  auto need = [&](uint32_t x, uint32_t y) {
-    uint32_t w = 0;
-    for (const PlanPlane &pp : p->planes)
-      w += sample_region_words(pp.bc, pp.br, x, y) + low_region_words(pp.bc, pp.br, x, y);
-    return w;
-  };
-  const uint32_t budget = 48 * 1024 / 4;
+// LDS words of a tile of tx x ty macroblocks: a sample region and a low
+// region for every plane.
+static uint32_t tile_words(const jxrhip_plan *p, uint32_t tx, uint32_t ty) {
+  uint32_t w = 0;
+  for (const PlanPlane &pp : p->planes)
+    w += sample_region_words(pp.bc, pp.br, tx, ty) + low_region_words(pp.bc, pp.br, tx, ty);
+  return w;
+}
+
+static constexpr uint32_t kDefaultTileBudgetWords = 48 * 1024 / 4;
+
+// Tile shape of a plan: `force` when given, else the session's fixed shape,
+// else the tuned shape of the plan's bucket, else the largest default whose
+// LDS fits 48 KiB (which keeps four workgroups on a compute unit). A fixed
+// shape that does not fit is halved along its longer side until it does.
+static void choose_tile(const jxrhip_session *s, const jxrhip_plan *p, uint32_t &tx, uint32_t &ty, uint32_t &words,
+                        const uint32_t *force = nullptr) {
+  if (force) {
+    tx = force[0];
+    ty = force[1];
+    words = tile_words(p, tx, ty);
+    return;
+  }
+  if (!s->tile_x) {
+    auto it = s->tuned.find(p->geometry_key);
+    if (it != s->tuned.end() && (size_t)tile_words(p, it->second.first, it->second.second) * 4 <= s->max_lds) {
+      tx = it->second.first;
+      ty = it->second.second;
+      words = tile_words(p, tx, ty);
+      return;
+    }
+  }
   if (s->tile_x) {
     tx = s->tile_x;
     ty = s->tile_y;
-    while (need(tx, ty) > budget && (tx > 1 || ty > 1)) {
+    while (tile_words(p, tx, ty) > kDefaultTileBudgetWords && (tx > 1 || ty > 1)) {
       if (tx >= ty) tx = (tx + 1) / 2;
       else ty = (ty + 1) / 2;
     }
-    words = need(tx, ty);
+    words = tile_words(p, tx, ty);
     return;
   }
   static const uint32_t cand[][2] = {{4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
   for (auto &c : cand) {
-    if (need(c[0], c[1]) <= budget) {
+    if (tile_words(p, c[0], c[1]) <= kDefaultTileBudgetWords) {
       tx = c[0];
       ty = c[1];
-      words = need(tx, ty);
+      words = tile_words(p, tx, ty);
       return;
     }
   }
   tx = ty = 1;
-  words = need(1, 1);
+  words = tile_words(p, 1, 1);
 }
 
 // One-bit output: the kernel ORs each tile's bits into cleared rows (a byte
@@ -979,9 +1021,8 @@ static void choose_tile(const jxrhip_session *s, const jxrhip_plan *p, uint32_t 
 // between rows are left as they are.
 static hipError_t clear_bit_rows(const jxrhip_plan *p, void *dst, hipStream_t stream) {
   if (p->dev.out_kind != kOutBits) return hipSuccess;
-  const size_t row = ((size_t)p->dev.crop_w + 7) / 8;
-  return hipMemset2DAsync((uint8_t *)dst + p->dev.plane_offset[0], (size_t)p->dev.row_stride[0], 0, row,
-                          p->dev.crop_h, stream);
+  const uint32_t row = (p->dev.crop_w + 7) / 8;
+  return launch_clear_rows((uint8_t *)dst + p->dev.plane_offset[0], p->dev.row_stride[0], row, p->dev.crop_h, stream);
 }
 
 static jxrhip_status submit_impl(jxrhip_session *s, const jxrhip_plan *p, void *dst, uint64_t dst_bytes,
@@ -1187,15 +1228,194 @@ extern "C" jxrhip_status jxrhip_decode_to_host(jxrhip_session *s, const jxrhip_p
   return st;
 }
 
+// ---- persistent batches --------------------------------------------------
+// One persistent launch per shape bucket. A prepared batch owns what those
+// launches read besides the images' descriptors and coefficients: the slot
+// records, the owning slot of every work item, the share counters and the
+// per-image status words.
+struct BucketLaunch {
+  size_t first;  // first slot of the bucket
+  uint32_t n, items, tx, ty, words, storage, grid;
+};
+
+struct PreparedBatch {
+  std::vector<BucketLaunch> launches;
+  std::vector<const jxrhip_plan *> plans;
+  std::vector<void *> dsts;
+  DeviceBuffer slots, item_slots, counters, statuses;
+  std::vector<DeviceBuffer> transient;  // descriptors and coefficients of arenas without identity
+  std::vector<uint64_t> coef_pins, desc_pins;  // cache entries held resident until release
+};
+
+static void pin_uploads(jxrhip_session *s, const jxrhip_plan *p, PreparedBatch &pb) {
+  for (uint32_t a = 0; a < p->arena_count; a++) {
+    auto it = s->coef_cache.find(p->arenas[a]->identity);
+    if (p->arenas[a]->identity && it != s->coef_cache.end()) {
+      it->second.pins++;
+      pb.coef_pins.push_back(it->first);
+    }
+  }
+  auto it = s->desc_cache.find(p->id);
+  if (it != s->desc_cache.end()) {
+    it->second.pins++;
+    pb.desc_pins.push_back(it->first);
+  }
+}
+
+static void release_prepared(jxrhip_session *s, PreparedBatch &pb) {
+  for (uint64_t k : pb.coef_pins) {
+    auto it = s->coef_cache.find(k);
+    if (it != s->coef_cache.end() && it->second.pins) it->second.pins--;
+  }
+  for (uint64_t k : pb.desc_pins) {
+    auto it = s->desc_cache.find(k);
+    if (it != s->desc_cache.end() && it->second.pins) it->second.pins--;
+  }
+  pb.coef_pins.clear();
+  pb.desc_pins.clear();
+  for (auto &t : pb.transient) s->pool.put(t);
+  pb.transient.clear();
+  s->pool.put(pb.slots);
+  s->pool.put(pb.item_slots);
+  s->pool.put(pb.counters);
+  s->pool.put(pb.statuses);
+}
+
+// Groups the plans into shape buckets, uploads what is missing and builds the
+// slot records. `force` (tile_x, tile_y) overrides every other tile choice.
+// Called with the session lock held; on failure the caller releases `pb`.
+static jxrhip_status prepare_persistent(jxrhip_session *s, const jxrhip_plan *const *plans, uint32_t count,
+                                        void *const *dsts, const uint32_t *force, PreparedBatch &pb) {
+  hipStream_t stream = s->streams[0];
+  pb.plans.assign(plans, plans + count);
+  pb.dsts.assign(dsts, dsts + count);
+  HIP_TRY(s->pool.get(sizeof(uint32_t) * std::max<uint32_t>(count, 1), pb.statuses, s->diag.scratch_hits,
+                      s->diag.scratch_misses),
+          "status words");
+  // shape buckets: plans with the same plane geometry and storage share a
+  // kernel instantiation and a tile shape
+  std::map<uint64_t, std::vector<uint32_t>> buckets;
+  for (uint32_t i = 0; i < count; i++) buckets[plans[i]->geometry_key].push_back(i);
+  std::vector<const DevImage *> dimgs(count);
+  for (uint32_t i = 0; i < count; i++) {
+    jxrhip_submission tmp;
+    tmp.session = s;
+    jxrhip_status st = device_image(s, plans[i], stream, &dimgs[i], &tmp);
+    for (auto &t : tmp.transient) pb.transient.push_back(t);
+    if (st) return st;
+    // later uploads of this batch must not evict what it already placed
+    pin_uploads(s, plans[i], pb);
+  }
+  std::vector<Slot> slots;
+  for (auto &kv : buckets) {
+    uint32_t tx, ty, words;
+    choose_tile(s, plans[kv.second[0]], tx, ty, words, force);
+    if ((size_t)words * 4 > s->max_lds) return fail(JXRHIP_ERR_INVALID_PLAN, "tile does not fit in LDS");
+    BucketLaunch L{slots.size(), (uint32_t)kv.second.size(), 0, tx, ty, words, plans[kv.second[0]]->dev.storage, 0};
+    for (uint32_t i : kv.second) {
+      Slot sl{};
+      sl.img = dimgs[i];
+      sl.out = (uint8_t *)dsts[i];
+      sl.status = (uint32_t *)pb.statuses.ptr + i;
+      sl.tiles_x = (plans[i]->mbx + tx - 1) / tx;
+      sl.tile_start = L.items;
+      const uint64_t items = (uint64_t)L.items + (uint64_t)sl.tiles_x * ((plans[i]->mby + ty - 1) / ty);
+      if (items > UINT32_MAX / 2) return fail(JXRHIP_ERR_BUDGET, "too many work items in one shape bucket");
+      L.items = (uint32_t)items;
+      slots.push_back(sl);
+    }
+    // workgroups a compute unit holds at once (LDS, registers), unless the
+    // session fixes it; a multiple of 8 workgroups for the XCD split
+    int resident = 0;
+    if (!s->groups_per_cu) HIP_TRY(kernel_occupancy(L.storage, (size_t)L.words * 4, &resident), "occupancy");
+    const uint32_t per_cu = s->groups_per_cu ? s->groups_per_cu : (uint32_t)std::max(resident, 1);
+    uint32_t grid = std::min<uint32_t>(L.items, s->cus * per_cu);
+    L.grid = std::max<uint32_t>((grid + 7u) & ~7u, 8u);
+    pb.launches.push_back(L);
+  }
+  HIP_TRY(s->pool.get(std::max<size_t>(slots.size(), 1) * sizeof(Slot), pb.slots, s->diag.scratch_hits,
+                      s->diag.scratch_misses),
+          "slot buffer");
+  HIP_TRY(hipMemcpyAsync(pb.slots.ptr, slots.data(), slots.size() * sizeof(Slot), hipMemcpyHostToDevice, stream),
+          "slot upload");
+  // the host records are staged before this returns
+  HIP_TRY(hipStreamSynchronize(stream), "slot upload");
+  uint64_t total_items = 0;
+  for (const BucketLaunch &L : pb.launches) total_items += L.items;
+  HIP_TRY(s->pool.get(std::max<uint64_t>(total_items, 1) * sizeof(uint32_t), pb.item_slots, s->diag.scratch_hits,
+                      s->diag.scratch_misses),
+          "item table");
+  // eight share counters per launch, each on its own 128-byte line
+  HIP_TRY(s->pool.get(std::max<size_t>(pb.launches.size(), 1) * 8 * 128, pb.counters, s->diag.scratch_hits,
+                      s->diag.scratch_misses),
+          "share counters");
+  return JXRHIP_OK;
+}
+
+// Enqueues a prepared batch behind the work already on streams[0] and joins
+// it back there: one-bit rows are cleared, then each bucket runs as an
+// item-table kernel (which also zeroes the share counters and the bucket's
+// status words) and a persistent launch. When every bucket holds at least two
+// items per workgroup, each launch fills the device and the buckets run one
+// after another on streams[0]; otherwise they go round-robin over the
+// session's streams, so that small buckets run side by side. Only kernel
+// launches and event dependencies, so that the sequence can be captured into
+// a graph.
+static jxrhip_status enqueue_persistent(jxrhip_session *s, const PreparedBatch &pb) {
+  hipStream_t s0 = s->streams[0];
+  for (size_t i = 0; i < pb.plans.size(); i++) HIP_TRY(clear_bit_rows(pb.plans[i], pb.dsts[i], s0), "one-bit rows");
+  bool spread = false;
+  for (const BucketLaunch &L : pb.launches) spread |= L.items < 2 * L.grid;
+  const uint32_t used = spread ? (uint32_t)std::min<size_t>(pb.launches.size(), kStreams) : 1;
+  if (used > 1) HIP_TRY(hipEventRecord(s->fork, s0), "hipEventRecord");
+  for (uint32_t i = 1; i < used; i++) HIP_TRY(hipStreamWaitEvent(s->streams[i], s->fork, 0), "hipStreamWaitEvent");
+  uint64_t item_base = 0;
+  for (size_t li = 0; li < pb.launches.size(); li++) {
+    const BucketLaunch &L = pb.launches[li];
+    hipStream_t ls = s->streams[li % used];
+    KernelArgs args{};
+    args.slots = (const Slot *)pb.slots.ptr + L.first;
+    args.item_slot = (const uint32_t *)pb.item_slots.ptr + item_base;
+    args.counters = (uint32_t *)((uint8_t *)pb.counters.ptr + li * 8 * 128);
+    item_base += L.items;
+    args.slot_count = L.n;
+    args.item_count = L.items;
+    args.persistent = 1;
+    args.tile_x = L.tx;
+    args.tile_y = L.ty;
+    args.lds_words = L.words;
+    args.storage = L.storage;
+    HIP_TRY(launch_expand(args.slots, L.n, L.items, (uint32_t *)args.item_slot, args.counters, ls), "item table");
+    HIP_TRY(launch_tiles(args, L.grid, ls), "persistent launch");
+    s->diag.kernel_launches++;
+  }
+  for (uint32_t i = 1; i < used; i++) {
+    HIP_TRY(hipEventRecord(s->join[i], s->streams[i]), "hipEventRecord");
+    HIP_TRY(hipStreamWaitEvent(s0, s->join[i], 0), "hipStreamWaitEvent");
+  }
+  return JXRHIP_OK;
+}
+
+struct jxrhip_batch_graph {
+  jxrhip_session *session = nullptr;
+  PreparedBatch pb;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t *host_statuses = nullptr;
+  hipEvent_t done = nullptr;
+  std::vector<uint64_t> coef_pins, desc_pins;  // cache entries kept resident
+  bool in_flight = false;
+};
+
 // ---- batches ------------------------------------------------------------
 struct jxrhip_batch {
   jxrhip_session *session = nullptr;
   uint32_t count = 0;
   std::vector<jxrhip_submission *> subs;  // per-image mode
-  DeviceBuffer statuses;                  // persistent mode: one word per image
+  PreparedBatch pb;                       // persistent mode
   uint32_t *host_statuses = nullptr;
-  std::vector<DeviceBuffer> transient;
   hipEvent_t done = nullptr;
+  jxrhip_batch_graph *graph = nullptr;  // replay of a captured batch: owns nothing
   bool timed = false;
 };
 
@@ -1244,122 +1464,28 @@ extern "C" jxrhip_status jxrhip_submit_batch(jxrhip_session *s, const jxrhip_pla
   if (mode != JXRHIP_BATCH_PERSISTENT) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "batch mode");
   std::lock_guard<std::mutex> g(s->lock);
   hipStream_t stream = s->streams[0];
-  HIP_TRY(hipHostMalloc((void **)&b->host_statuses, sizeof(uint32_t) * std::max<uint32_t>(count, 1), hipHostMallocDefault),
-          "hipHostMalloc");
-  HIP_TRY(s->pool.get(sizeof(uint32_t) * std::max<uint32_t>(count, 1), b->statuses, s->diag.scratch_hits,
-                      s->diag.scratch_misses),
-          "status words");
-  // shape buckets: plans with the same plane geometry share a tile shape
-  std::map<uint64_t, std::vector<uint32_t>> buckets;
-  for (uint32_t i = 0; i < count; i++) buckets[plans[i]->geometry_key].push_back(i);
-  std::vector<const DevImage *> dimgs(count);
-  for (uint32_t i = 0; i < count; i++) {
-    jxrhip_submission tmp;
-    tmp.session = s;
-    jxrhip_status st = device_image(s, plans[i], stream, &dimgs[i], &tmp);
-    for (auto &t : tmp.transient) b->transient.push_back(t);
-    if (st) return st;
-  }
-  HIP_TRY(hipMemsetAsync(b->statuses.ptr, 0, sizeof(uint32_t) * std::max<uint32_t>(count, 1), stream), "status reset");
-  for (uint32_t i = 0; i < count; i++) HIP_TRY(clear_bit_rows(plans[i], dsts[i], stream), "one-bit rows");
-  // per-bucket slot arrays go into one device buffer
-  std::vector<Slot> slots;
-  struct Launch {
-    size_t first;
-    uint32_t n, items, tx, ty, words, storage;
+  auto undo = [&](jxrhip_status st) {
+    (void)hipStreamSynchronize(stream);
+    release_prepared(s, b->pb);
+    if (b->host_statuses) (void)hipHostFree(b->host_statuses);
+    b->host_statuses = nullptr;
+    return st;
   };
-  std::vector<Launch> launches;
-  for (auto &kv : buckets) {
-    uint32_t tx, ty, words;
-    choose_tile(s, plans[kv.second[0]], tx, ty, words);
-    for (uint32_t i : kv.second) {
-      uint32_t w2;
-      uint32_t ttx, tty;
-      choose_tile(s, plans[i], ttx, tty, w2);
-      words = std::max(words, w2);
-    }
-    if ((size_t)words * 4 > s->max_lds) return fail(JXRHIP_ERR_INVALID_PLAN, "tile does not fit in LDS");
-    Launch L{slots.size(), (uint32_t)kv.second.size(), 0, tx, ty, words, plans[kv.second[0]]->dev.storage};
-    for (uint32_t i : kv.second) {
-      Slot sl{};
-      sl.img = dimgs[i];
-      sl.out = (uint8_t *)dsts[i];
-      sl.status = (uint32_t *)b->statuses.ptr + i;
-      sl.tiles_x = (plans[i]->mbx + tx - 1) / tx;
-      sl.tile_start = L.items;
-      L.items += sl.tiles_x * ((plans[i]->mby + ty - 1) / ty);
-      slots.push_back(sl);
-    }
-    launches.push_back(L);
-  }
-  DeviceBuffer slot_buf;
-  HIP_TRY(s->pool.get(std::max<size_t>(slots.size(), 1) * sizeof(Slot), slot_buf, s->diag.scratch_hits,
-                      s->diag.scratch_misses),
-          "slot buffer");
-  b->transient.push_back(slot_buf);
-  HIP_TRY(hipMemcpyAsync(slot_buf.ptr, slots.data(), slots.size() * sizeof(Slot), hipMemcpyHostToDevice, stream),
-          "slot upload");
-  HIP_TRY(hipStreamSynchronize(stream), "slot upload");
-  // owning image of every work item, one table per launch
-  uint64_t total_items = 0;
-  for (const Launch &L : launches) total_items += L.items;
-  DeviceBuffer item_slots;
-  HIP_TRY(s->pool.get(std::max<uint64_t>(total_items, 1) * sizeof(uint32_t), item_slots, s->diag.scratch_hits,
-                      s->diag.scratch_misses),
-          "item table");
-  b->transient.push_back(item_slots);
-  // eight share counters per launch, each on its own 128-byte line
-  DeviceBuffer counters;
-  HIP_TRY(s->pool.get(std::max<size_t>(launches.size(), 1) * 8 * 128, counters, s->diag.scratch_hits,
-                      s->diag.scratch_misses),
-          "share counters");
-  b->transient.push_back(counters);
-  // Buckets go round-robin over the session's streams, so that one bucket's
-  // workgroups fill the compute units another bucket's tail leaves idle.
-  HIP_TRY(hipEventRecord(s->t0, stream), "hipEventRecord");
-  const uint32_t used_streams = (uint32_t)std::min<size_t>(launches.size(), kStreams);
-  for (uint32_t i = 1; i < used_streams; i++) HIP_TRY(hipStreamWaitEvent(s->streams[i], s->t0, 0), "hipStreamWaitEvent");
-  uint64_t item_base = 0;
-  for (size_t li = 0; li < launches.size(); li++) {
-    const Launch &L = launches[li];
-    hipStream_t ls = s->streams[li % kStreams];
-    KernelArgs args{};
-    args.slots = (const Slot *)slot_buf.ptr + L.first;
-    args.item_slot = (const uint32_t *)item_slots.ptr + item_base;
-    args.counters = (uint32_t *)((uint8_t *)counters.ptr + li * 8 * 128);
-    item_base += L.items;
-    args.slot_count = L.n;
-    args.item_count = L.items;
-    args.persistent = 1;
-    args.tile_x = L.tx;
-    args.tile_y = L.ty;
-    args.lds_words = L.words;
-    args.storage = L.storage;
-    // workgroups a compute unit holds at once (LDS, registers), unless the
-    // session fixes it; a multiple of 8 workgroups for the XCD split
-    int resident = 0;
-    if (!s->groups_per_cu) HIP_TRY(kernel_occupancy(L.storage, (size_t)L.words * 4, &resident), "occupancy");
-    const uint32_t per_cu = s->groups_per_cu ? s->groups_per_cu : (uint32_t)std::max(resident, 1);
-    uint32_t grid = std::min<uint32_t>(L.items, s->cus * per_cu);
-    grid = std::max<uint32_t>((grid + 7u) & ~7u, 8u);
-    HIP_TRY(launch_expand(args.slots, L.n, L.items, (uint32_t *)args.item_slot, args.counters, ls), "item table");
-    HIP_TRY(launch_tiles(args, grid, ls), "persistent launch");
-    s->diag.kernel_launches++;
-  }
-  for (uint32_t i = 1; i < used_streams; i++) {
-    hipEvent_t e;
-    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-    HIP_TRY(hipEventRecord(e, s->streams[i]), "hipEventRecord");
-    HIP_TRY(hipStreamWaitEvent(stream, e, 0), "hipStreamWaitEvent");
-    (void)hipEventDestroy(e);
-  }
+  if (hipHostMalloc((void **)&b->host_statuses, sizeof(uint32_t) * std::max<uint32_t>(count, 1),
+                    hipHostMallocDefault) != hipSuccess)
+    return undo(fail(JXRHIP_ERR_DEVICE, "hipHostMalloc"));
+  jxrhip_status st = prepare_persistent(s, plans, count, dsts, nullptr, b->pb);
+  if (st) return undo(st);
+  if (hipEventRecord(s->t0, stream) != hipSuccess) return undo(fail(JXRHIP_ERR_DEVICE, "hipEventRecord"));
+  st = enqueue_persistent(s, b->pb);
+  if (st) return undo(st);
   HIP_TRY(hipEventRecord(s->t1, stream), "hipEventRecord");
-  HIP_TRY(hipMemcpyAsync(b->host_statuses, b->statuses.ptr, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, stream),
+  HIP_TRY(hipMemcpyAsync(b->host_statuses, b->pb.statuses.ptr, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, stream),
           "status readback");
   HIP_TRY(hipEventCreateWithFlags(&b->done, hipEventDisableTiming), "hipEventCreate");
   HIP_TRY(hipEventRecord(b->done, stream), "hipEventRecord");
   b->timed = true;
-  s->last_launches = (uint32_t)launches.size();
+  s->last_launches = (uint32_t)b->pb.launches.size();
   s->diag.submissions += count;
   *out = b.release();
   return JXRHIP_OK;
@@ -1407,13 +1533,16 @@ extern "C" jxrhip_status jxrhip_batch_wait(jxrhip_batch *b, uint32_t *stages) {
     (void)hipEventSynchronize(s->t1);
     if (hipEventElapsedTime(&ms, s->t0, s->t1) == hipSuccess) s->last_ms = ms;
   }
-  {
-    std::lock_guard<std::mutex> g(s->lock);
-    for (auto &t : b->transient) s->pool.put(t);
-    s->pool.put(b->statuses);
+  if (b->graph) {
+    b->graph->in_flight = false;
+  } else {
+    {
+      std::lock_guard<std::mutex> g(s->lock);
+      release_prepared(s, b->pb);
+    }
+    if (b->host_statuses) (void)hipHostFree(b->host_statuses);
+    if (b->done) (void)hipEventDestroy(b->done);
   }
-  if (b->host_statuses) (void)hipHostFree(b->host_statuses);
-  if (b->done) (void)hipEventDestroy(b->done);
   delete b;
   if (result) g_error = first_error.empty() ? g_error : first_error;
   return result;
@@ -1423,5 +1552,206 @@ extern "C" jxrhip_status jxrhip_last_batch_timing(jxrhip_session *s, double *ms,
   if (!s) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null session");
   if (ms) *ms = s->last_ms;
   if (launches) *launches = s->last_launches;
+  return JXRHIP_OK;
+}
+
+// ---- captured batches -----------------------------------------------------
+extern "C" jxrhip_status jxrhip_batch_graph_create(jxrhip_session *s, const jxrhip_plan *const *plans, uint32_t count,
+                                                   void *const *dsts, jxrhip_batch_graph **out) {
+  if (!s || !plans || !dsts || !out) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (count == 0) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "empty batch");
+  for (uint32_t i = 0; i < count; i++) {
+    if (!plans[i] || !dsts[i]) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null plan or destination");
+    for (uint32_t a = 0; a < plans[i]->arena_count; a++)
+      if (!plans[i]->arenas[a]->identity)
+        return fail(JXRHIP_ERR_INVALID_ARGUMENT, "a captured batch needs arenas with identity (cached uploads)");
+  }
+  std::unique_ptr<jxrhip_batch_graph> gr(new (std::nothrow) jxrhip_batch_graph());
+  if (!gr) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "out of host memory");
+  gr->session = s;
+  std::lock_guard<std::mutex> g(s->lock);
+  make_current(s);
+  hipStream_t s0 = s->streams[0];
+  auto undo = [&](jxrhip_status st) {
+    (void)hipStreamSynchronize(s0);
+    release_prepared(s, gr->pb);
+    if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+    if (gr->graph) (void)hipGraphDestroy(gr->graph);
+    if (gr->host_statuses) (void)hipHostFree(gr->host_statuses);
+    if (gr->done) (void)hipEventDestroy(gr->done);
+    return st;
+  };
+  if (hipHostMalloc((void **)&gr->host_statuses, sizeof(uint32_t) * count, hipHostMallocDefault) != hipSuccess)
+    return undo(fail(JXRHIP_ERR_DEVICE, "hipHostMalloc"));
+  if (hipEventCreateWithFlags(&gr->done, hipEventDisableTiming) != hipSuccess)
+    return undo(fail(JXRHIP_ERR_DEVICE, "hipEventCreate"));
+  jxrhip_status st = prepare_persistent(s, plans, count, dsts, nullptr, gr->pb);
+  if (st) return undo(st);
+  const uint64_t launches_before = s->diag.kernel_launches;
+  hipError_t e = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return undo(hip_fail(e, "hipStreamBeginCapture"));
+  st = enqueue_persistent(s, gr->pb);
+  std::string reason = st ? g_error : std::string();
+  e = hipStreamEndCapture(s0, &gr->graph);
+  s->diag.kernel_launches = launches_before;
+  if (st) return undo(fail(st, reason));
+  if (e != hipSuccess) return undo(hip_fail(e, "hipStreamEndCapture"));
+  e = hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) return undo(hip_fail(e, "hipGraphInstantiate"));
+  s->live_graphs++;
+  *out = gr.release();
+  return JXRHIP_OK;
+}
+
+extern "C" jxrhip_status jxrhip_batch_graph_launch(jxrhip_batch_graph *gr, jxrhip_batch **out) {
+  if (!gr || !out) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  jxrhip_session *s = gr->session;
+  std::lock_guard<std::mutex> g(s->lock);
+  if (gr->in_flight) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "the previous replay of this graph has not been waited for");
+  make_current(s);
+  std::unique_ptr<jxrhip_batch> b(new (std::nothrow) jxrhip_batch());
+  if (!b) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "out of host memory");
+  const uint32_t count = (uint32_t)gr->pb.plans.size();
+  b->session = s;
+  b->count = count;
+  b->graph = gr;
+  b->host_statuses = gr->host_statuses;
+  b->done = gr->done;
+  hipStream_t s0 = s->streams[0];
+  HIP_TRY(hipEventRecord(s->t0, s0), "hipEventRecord");
+  HIP_TRY(hipGraphLaunch(gr->exec, s0), "hipGraphLaunch");
+  HIP_TRY(hipEventRecord(s->t1, s0), "hipEventRecord");
+  HIP_TRY(hipMemcpyAsync(gr->host_statuses, gr->pb.statuses.ptr, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, s0),
+          "status readback");
+  HIP_TRY(hipEventRecord(gr->done, s0), "hipEventRecord");
+  gr->in_flight = true;
+  b->timed = true;
+  s->last_launches = (uint32_t)gr->pb.launches.size();
+  s->diag.kernel_launches += gr->pb.launches.size();
+  s->diag.submissions += count;
+  *out = b.release();
+  return JXRHIP_OK;
+}
+
+extern "C" void jxrhip_batch_graph_destroy(jxrhip_batch_graph *gr) {
+  if (!gr) return;
+  jxrhip_session *s = gr->session;
+  make_current(s);
+  (void)hipEventSynchronize(gr->done);
+  std::lock_guard<std::mutex> g(s->lock);
+  (void)hipStreamSynchronize(s->streams[0]);
+  release_prepared(s, gr->pb);
+  if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+  if (gr->graph) (void)hipGraphDestroy(gr->graph);
+  if (gr->host_statuses) (void)hipHostFree(gr->host_statuses);
+  if (gr->done) (void)hipEventDestroy(gr->done);
+  s->live_graphs--;
+  delete gr;
+}
+
+// ---- tile autotuner ---------------------------------------------------------
+extern "C" uint64_t jxrhip_plan_bucket_key(const jxrhip_plan *p) { return p ? p->geometry_key : 0; }
+
+extern "C" jxrhip_status jxrhip_session_set_bucket_tile(jxrhip_session *s, uint64_t key, uint32_t tx, uint32_t ty) {
+  if (!s) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null session");
+  if ((tx == 0) != (ty == 0) || tx > 16 || ty > 16) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "tile shape");
+  std::lock_guard<std::mutex> g(s->lock);
+  if (tx == 0) s->tuned.erase(key);
+  else s->tuned[key] = {tx, ty};
+  return JXRHIP_OK;
+}
+
+extern "C" jxrhip_status jxrhip_session_bucket_tile(jxrhip_session *s, uint64_t key, uint32_t *tx, uint32_t *ty) {
+  if (!s || !tx || !ty) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> g(s->lock);
+  auto it = s->tuned.find(key);
+  *tx = it == s->tuned.end() ? 0 : it->second.first;
+  *ty = it == s->tuned.end() ? 0 : it->second.second;
+  return JXRHIP_OK;
+}
+
+// Times every candidate tile shape on each shape bucket of the batch (a
+// persistent launch of that bucket alone, `repeats` times after one untimed
+// run) and keeps the fastest for the bucket key. The destinations receive
+// the images, as a batch would.
+extern "C" jxrhip_status jxrhip_session_autotune(jxrhip_session *s, const jxrhip_plan *const *plans, uint32_t count,
+                                                 void *const *dsts, uint32_t repeats, jxrhip_tile_choice *choices,
+                                                 uint32_t capacity, uint32_t *bucket_count) {
+  if (!s || !plans || !dsts) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  if (bucket_count) *bucket_count = 0;
+  for (uint32_t i = 0; i < count; i++)
+    if (!plans[i] || !dsts[i]) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null plan or destination");
+  repeats = std::min<uint32_t>(std::max<uint32_t>(repeats, 1), 64);
+  std::map<uint64_t, std::vector<uint32_t>> buckets;
+  for (uint32_t i = 0; i < count; i++) buckets[plans[i]->geometry_key].push_back(i);
+  if (bucket_count) *bucket_count = (uint32_t)buckets.size();
+  static const uint32_t cand[][2] = {{4, 4}, {8, 4}, {4, 8}, {8, 2}, {2, 8}, {4, 2}, {2, 4},
+                                     {8, 8}, {16, 2}, {16, 4}, {2, 2}, {4, 1}};
+  std::lock_guard<std::mutex> g(s->lock);
+  make_current(s);
+  hipStream_t s0 = s->streams[0];
+  uint32_t k = 0;
+  for (auto &kv : buckets) {
+    std::vector<const jxrhip_plan *> bp;
+    std::vector<void *> bd;
+    for (uint32_t i : kv.second) {
+      bp.push_back(plans[i]);
+      bd.push_back(dsts[i]);
+    }
+    const jxrhip_plan *p0 = bp[0];
+    // the shape the bucket gets without tuning
+    const auto saved = s->tuned.find(kv.first) != s->tuned.end() ? s->tuned[kv.first] : std::make_pair(0u, 0u);
+    s->tuned.erase(kv.first);
+    uint32_t dx, dy, dw;
+    choose_tile(s, p0, dx, dy, dw);
+    if (saved.first) s->tuned[kv.first] = saved;
+    std::vector<std::pair<uint32_t, uint32_t>> shapes{{dx, dy}};
+    for (auto &c : cand) {
+      if (c[0] == dx && c[1] == dy) continue;
+      if ((size_t)tile_words(p0, c[0], c[1]) * 4 > std::min<size_t>(s->max_lds, 64 * 1024)) continue;
+      shapes.push_back({c[0], c[1]});
+    }
+    double best_ms = 1e30, default_ms = 0;
+    std::pair<uint32_t, uint32_t> best{dx, dy};
+    for (auto &sh : shapes) {
+      const uint32_t force[2] = {sh.first, sh.second};
+      PreparedBatch pb;
+      jxrhip_status st = prepare_persistent(s, bp.data(), (uint32_t)bp.size(), bd.data(), force, pb);
+      std::vector<double> ms;
+      for (uint32_t r = 0; st == JXRHIP_OK && r <= repeats; r++) {
+        if (hipEventRecord(s->t0, s0) != hipSuccess) st = fail(JXRHIP_ERR_DEVICE, "hipEventRecord");
+        if (!st) st = enqueue_persistent(s, pb);
+        if (!st && hipEventRecord(s->t1, s0) != hipSuccess) st = fail(JXRHIP_ERR_DEVICE, "hipEventRecord");
+        if (!st && hipEventSynchronize(s->t1) != hipSuccess) st = fail(JXRHIP_ERR_DEVICE, "hipEventSynchronize");
+        float t = 0;
+        if (!st && hipEventElapsedTime(&t, s->t0, s->t1) != hipSuccess) st = fail(JXRHIP_ERR_DEVICE, "hipEventElapsedTime");
+        if (!st && r > 0) ms.push_back(t);
+      }
+      (void)hipStreamSynchronize(s0);
+      release_prepared(s, pb);
+      if (st) return st;
+      std::sort(ms.begin(), ms.end());
+      const double med = ms[ms.size() / 2];
+      if (sh.first == dx && sh.second == dy) default_ms = med;
+      if (med < best_ms) {
+        best_ms = med;
+        best = sh;
+      }
+    }
+    s->tuned[kv.first] = best;
+    if (choices && k < capacity) {
+      jxrhip_tile_choice &c = choices[k];
+      c.bucket_key = kv.first;
+      c.tile_mbs_x = best.first;
+      c.tile_mbs_y = best.second;
+      c.images = (uint32_t)bp.size();
+      c.candidates = (uint32_t)shapes.size();
+      c.best_ms = best_ms;
+      c.default_ms = default_ms;
+    }
+    k++;
+  }
   return JXRHIP_OK;
 }

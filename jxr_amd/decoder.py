@@ -267,6 +267,11 @@ class HipDecodePlan:
         return load().jxrhip_plan_output_bytes(self._h)
 
     @property
+    def bucket_key(self) -> int:
+        """Shape bucket: plans with equal keys share a persistent launch."""
+        return load().jxrhip_plan_bucket_key(self._h)
+
+    @property
     def output_pixels(self) -> int:
         return load().jxrhip_plan_output_pixels(self._h)
 
@@ -375,6 +380,38 @@ class HipBatch:
         if code not in (abi.OK, abi.ERR_KERNEL_ARITHMETIC) or (code != abi.OK and raise_on_error):
             raise JxrHipError(code, lib.jxrhip_last_error().decode(errors="replace"))
         return stages[: self.count]
+
+
+class HipBatchGraph:
+    """A captured persistent batch. launch() replays it and returns a
+    HipBatch; one replay at a time. Keeps its plans alive."""
+
+    def __init__(self, session, plans, device_ptrs):
+        self._s = session
+        self._plans = list(plans)
+        n = len(plans)
+        parr = (C.c_void_p * max(n, 1))(*[p.handle.value for p in plans])
+        darr = (C.c_void_p * max(n, 1))(*[int(d) for d in device_ptrs])
+        h = C.c_void_p()
+        check(load().jxrhip_batch_graph_create(session._h, parr, n, darr, C.byref(h)), "batch_graph_create")
+        self._h = h
+        self.count = n
+
+    def launch(self) -> HipBatch:
+        b = C.c_void_p()
+        check(load().jxrhip_batch_graph_launch(self._h, C.byref(b)), "batch_graph_launch")
+        return HipBatch(self._s, b, self.count)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().jxrhip_batch_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
 
 
 class DeviceBuffer:
@@ -490,6 +527,34 @@ class HipDecoderSession:
         h = C.c_void_p()
         check(load().jxrhip_submit_batch(self._h, parr, n, darr, mode, C.byref(h)), "submit_batch")
         return HipBatch(self, h, n)
+
+    def autotune(self, plans: Sequence[HipDecodePlan], device_ptrs: Sequence[int], repeats: int = 3):
+        """Times the candidate tile shapes of every shape bucket of the batch
+        and keeps the fastest per bucket (jxrhip_session_autotune). Returns
+        one dict per bucket."""
+        n = len(plans)
+        parr = (C.c_void_p * max(n, 1))(*[p.handle.value for p in plans])
+        darr = (C.c_void_p * max(n, 1))(*[int(d) for d in device_ptrs])
+        cap = max(n, 1)
+        out = (abi.TileChoice * cap)()
+        nb = C.c_uint32()
+        check(load().jxrhip_session_autotune(self._h, parr, n, darr, repeats, out, cap, C.byref(nb)), "autotune")
+        return [{"bucket_key": c.bucket_key, "tile": (c.tile_mbs_x, c.tile_mbs_y), "images": c.images,
+                 "candidates": c.candidates, "best_ms": c.best_ms, "default_ms": c.default_ms}
+                for c in out[: min(nb.value, cap)]]
+
+    def set_bucket_tile(self, bucket_key: int, tx: int, ty: int):
+        check(load().jxrhip_session_set_bucket_tile(self._h, bucket_key, tx, ty), "set_bucket_tile")
+
+    def bucket_tile(self, bucket_key: int):
+        tx, ty = C.c_uint32(), C.c_uint32()
+        check(load().jxrhip_session_bucket_tile(self._h, bucket_key, C.byref(tx), C.byref(ty)), "bucket_tile")
+        return tx.value, ty.value
+
+    def capture_batch(self, plans: Sequence[HipDecodePlan], device_ptrs: Sequence[int]) -> "HipBatchGraph":
+        """A persistent batch captured into a HIP graph for repeated replay
+        (jxrhip_batch_graph_create)."""
+        return HipBatchGraph(self, plans, device_ptrs)
 
     def allocate_destination(self, nbytes: int) -> "DeviceBuffer":
         return DeviceBuffer(self, nbytes)

@@ -2,6 +2,7 @@
 #include <stdio.h>
 
 #include <chrono>
+#include <vector>
 
 #include "hip/hip_runtime.h"
 
@@ -28,6 +29,44 @@ void emu_begin_launch(size_t lds_bytes) {
 void emu_end_launch() {
   free(emu_dynamic_lds);
   emu_dynamic_lds = nullptr;
+}
+
+struct emu_graph {
+  std::vector<std::function<void()>> launches;
+};
+static emu_graph *g_capture = nullptr;
+
+bool emu_capture(const std::function<void()> &launch) {
+  if (!g_capture) return false;
+  g_capture->launches.push_back(launch);
+  return true;
+}
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+  if (g_capture) return hipErrorInvalidValue;
+  g_capture = new emu_graph();
+  return hipSuccess;
+}
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *graph) {
+  if (!g_capture) return hipErrorInvalidValue;
+  *graph = g_capture;
+  g_capture = nullptr;
+  return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t *exec, hipGraph_t graph, void *, char *, size_t) {
+  *exec = new emu_graph(*graph);
+  return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t exec, hipStream_t) {
+  for (auto &f : exec->launches) f();
+  return hipSuccess;
+}
+hipError_t hipGraphExecDestroy(hipGraphExec_t exec) {
+  delete exec;
+  return hipSuccess;
+}
+hipError_t hipGraphDestroy(hipGraph_t graph) {
+  delete graph;
+  return hipSuccess;
 }
 
 hipError_t hipGetDeviceCount(int *n) {

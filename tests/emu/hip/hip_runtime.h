@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 
 #define __global__
 #define __device__
@@ -129,21 +130,39 @@ hipError_t hipMemset2DAsync(void *dst, size_t pitch, int value, size_t width, si
 hipError_t hipFuncSetAttribute(const void *f, hipFuncAttribute attr, int value);
 hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *blocks, const void *f, int threads, size_t lds);
 
+// Graph capture: while a capture is open, launches (on any stream, as the
+// streams joined to the capture by events would be) are recorded and run
+// when the instantiated graph is launched.
+typedef struct emu_graph *hipGraph_t;
+typedef struct emu_graph *hipGraphExec_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1,
+                            hipStreamCaptureModeRelaxed = 2 };
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode mode);
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t *graph);
+hipError_t hipGraphInstantiate(hipGraphExec_t *exec, hipGraph_t graph, void *, char *, size_t);
+hipError_t hipGraphLaunch(hipGraphExec_t exec, hipStream_t s);
+hipError_t hipGraphExecDestroy(hipGraphExec_t exec);
+hipError_t hipGraphDestroy(hipGraph_t graph);
+
 void emu_begin_launch(size_t lds_bytes);
 void emu_end_launch();
+bool emu_capture(const std::function<void()> &launch);  // true when recorded
 
 template <class... A, class... B>
 inline void emu_launch(void (*kernel)(A...), dim3 grid, dim3 block, size_t lds, hipStream_t, B... args) {
   (void)block;
-  emu_begin_launch(lds);
-  gridDim.x = grid.x;
-  for (unsigned b = 0; b < grid.x; b++) {
-    blockIdx.x = b;
-    threadIdx.x = 0;
-    blockDim.x = 1;
-    kernel(args...);
-  }
-  emu_end_launch();
+  auto run = [=]() {
+    emu_begin_launch(lds);
+    gridDim.x = grid.x;
+    for (unsigned b = 0; b < grid.x; b++) {
+      blockIdx.x = b;
+      threadIdx.x = 0;
+      blockDim.x = 1;
+      kernel(args...);
+    }
+    emu_end_launch();
+  };
+  if (!emu_capture(run)) run();
 }
 #define hipLaunchKernelGGL(kernel, grid, block, lds, stream, ...) \
   emu_launch(kernel, grid, block, lds, stream, __VA_ARGS__)

@@ -75,6 +75,28 @@ def main():
             if st or not np.array_equal(b.to_host(), want):
                 print(f"MISMATCH batch mode {mode} {name} stage {st}", flush=True)
                 failures += 1
+    # captured batch, replayed with the destinations scribbled between replays
+    bufs = [session.allocate_destination(p.output_bytes) for p in plans]
+    graph = session.capture_batch(plans, [b.ptr for b in bufs])
+    for replay in range(2):
+        for b in bufs:
+            b.fill(0x5A + replay)
+        stages = graph.launch().wait()
+        for name, b, want, st in zip(names, bufs, wants, stages):
+            if st or not np.array_equal(b.to_host(), want):
+                print(f"MISMATCH graph replay {replay} {name} stage {st}", flush=True)
+                failures += 1
+    graph.close()
+    # autotuned shapes, then a batch at those shapes
+    choices = session.autotune(plans, [b.ptr for b in bufs], repeats=1)
+    if len(choices) != len({p.bucket_key for p in plans}):
+        print(f"AUTOTUNE returned {len(choices)} buckets", flush=True)
+        failures += 1
+    stages = session.submit_batch(plans, [b.ptr for b in bufs], abi.BATCH_PERSISTENT).wait()
+    for name, b, want, st in zip(names, bufs, wants, stages):
+        if st or not np.array_equal(b.to_host(), want):
+            print(f"MISMATCH tuned batch {name} stage {st}", flush=True)
+            failures += 1
     print(f"emulated parity: {failures} failures over {len(names)} cases", flush=True)
     sys.exit(1 if failures else 0)
 
