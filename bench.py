@@ -105,6 +105,66 @@ def cpu_baseline(items, plans, dense, offsets, budget_s=15.0):
                 parity_checked_images=checked)
 
 
+def serve(session, seconds, cache, budget_gb):
+    """The serving stream (Poisson arrivals at 2000/s, the c5 mix) through
+    the native serving loop; returns its summary. With `cache` the pool's
+    coefficients stay resident on the device (uploaded once before the
+    stream); without, every request uploads its own."""
+    from jxr_amd import HipDecodePlan, HipServer, workloads
+    from jxr_amd.serving import clock_ns
+    t0 = time.perf_counter()
+    pool = workloads.c5_pool(cache=cache)
+    plans = {k: [HipDecodePlan.from_encoded(e, sp) for e, sp in pool[k]] for k in ("small", "gray", "region")}
+    log(f"[serve] pool of {sum(len(v) for v in plans.values())} images in {time.perf_counter() - t0:.1f} s")
+    if cache:
+        for v in plans.values():
+            for p in v:
+                session.upload(p)
+        session.synchronize()
+    sched = workloads.c5_schedule(seconds)
+    n = len(sched)
+    offered_px = sum(plans[kind][k].output_pixels for _, kind, k in sched)
+    server = HipServer(session, hbm_budget_bytes=int(budget_gb * (1 << 30)), max_wave_images=256, max_waves=4)
+    start = clock_ns() + 20_000_000
+    arrivals = [start + int(t * 1e9) for t, _, _ in sched]
+    i = done = failed = 0
+    next_log = time.perf_counter() + 5.0
+    while done < n:
+        now = clock_ns()
+        while i < n and arrivals[i] <= now:
+            _, kind, k = sched[i]
+            server.submit(plans[kind][k], tag=i, arrival_ns=arrivals[i])
+            i += 1
+        wait_us = 2000 if i >= n else int(max(0, min(2000, (arrivals[i] - clock_ns()) // 1000)))
+        for r in server.poll(4096, timeout_us=wait_us):
+            if r.image is not None:
+                r.image.close()
+            failed += r.status != 0
+            done += 1
+        if time.perf_counter() > next_log:
+            log(f"[serve] {i}/{n} submitted, {done} done")
+            next_log += 5.0
+    end = clock_ns()
+    st = server.stats()
+    server.close()
+    wall = (end - start) / 1e9
+    return {
+        "workload": "c5: Poisson 2000 req/s; 70% RGB u8 256..1024 q75 4:4:4 ov1, 20% 4096x4096 gray u16 8x8 "
+                    "tiles ov2 q75, 10% 512x512 regions of 8192x8192 RGB u8 (512-px tiles)",
+        "seconds_offered": seconds,
+        "uploads": "resident (pool uploaded once)" if cache else "per request (PCIe inside the latency)",
+        "requests": n, "completed": st["completed"], "failed": failed, "rejected": st["rejected"],
+        "offered_mpx_per_s": round(offered_px / seconds / 1e6, 1),
+        "mpx_per_s": round(st["pixels_completed"] / wall / 1e6, 1),
+        "latency_ms": {"p50": round(st["p50_ms"], 3), "p99": round(st["p99_ms"], 3)},
+        "queue_ms": {"p50": round(st["queue_p50_ms"], 3), "p99": round(st["queue_p99_ms"], 3)},
+        "waves": st["waves"], "mean_wave_images": round(st["mean_wave_images"], 2),
+        "max_wave_images": st["max_wave_images"],
+        "inflight_high_water_gb": round(st["inflight_high_water_bytes"] / 1e9, 3),
+        "hbm_high_water_gb": round(st["hbm_high_water_bytes"] / 1e9, 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +176,10 @@ def main():
     ap.add_argument("--tile", default=None, help="TXxTY macroblocks per work item")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--no-graph", action="store_true", help="submit the batch eagerly every step")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="seconds of the c5 serving stream to run after the batch (0: none)")
+    ap.add_argument("--serve-uploads", choices=["resident", "per-request"], default="resident")
+    ap.add_argument("--serve-budget-gb", type=float, default=0.0, help="admission budget (0: 3/4 of free HBM)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also copy 256 MiB device-to-device (HBM counter calibration)")
     args = ap.parse_args()
@@ -233,6 +297,8 @@ def main():
                             "latency_ms": {"p50": round(percentile(per_image["wall_ms"], 50), 4),
                                            "p99": round(percentile(per_image["wall_ms"], 99), 4)}}
         out["batched_over_per_image"] = round(value / pi_value, 3)
+    if args.serve_seconds > 0:
+        out["serving"] = serve(session, args.serve_seconds, args.serve_uploads == "resident", args.serve_budget_gb)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(items, plans, dense, offsets)
     print(json.dumps(out), flush=True)

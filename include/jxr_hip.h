@@ -429,6 +429,75 @@ jxrhip_status jxrhip_batch_graph_create(jxrhip_session *session, const jxrhip_pl
 jxrhip_status jxrhip_batch_graph_launch(jxrhip_batch_graph *graph, jxrhip_batch **out);
 void jxrhip_batch_graph_destroy(jxrhip_batch_graph *graph);
 
+/* ---- serving loop -------------------------------------------------------- */
+/* A request loop over one session: requests are admitted first come, first
+ * served while the device footprint of the admitted ones (uploaded
+ * coefficients and descriptors, plus the output when the server allocates
+ * it) fits the budget, and run as waves, each one persistent batch on one of
+ * four compute streams, with uploads on a stream of their own. A wave starts
+ * whenever a compute stream is free, so requests arriving while others run
+ * are admitted as memory frees up and their waves overlap on the device
+ * (continuous batching). Results carry each request's status, its
+ * arrival-to-resident latency and, when the request gave no destination, the
+ * output image. The nearest callers in the reference are the batch decoder's
+ * grouped submissions (crates/jxr/src/batch/cuda.rs:20-156, indexed
+ * per-image errors); admission and continuous batching have no counterpart
+ * there. Plans and destinations must outlive their requests; submit and poll
+ * may be called from any thread, destroy from one thread after the last
+ * submit. */
+typedef struct jxrhip_server jxrhip_server;
+
+typedef struct jxrhip_server_config {
+  uint64_t hbm_budget_bytes; /* admitted footprint limit (0: 3/4 of free HBM) */
+  uint32_t max_inflight;     /* admitted requests at once (0: 4096)           */
+  uint32_t max_wave_images;  /* images per wave, up to 65536 (0: 256)         */
+  uint32_t max_waves;        /* waves on the device at once, 1..4 (0: 4)      */
+  uint32_t reserved0;
+} jxrhip_server_config;
+
+typedef struct jxrhip_server_result {
+  uint64_t ticket;
+  uint64_t tag;
+  jxrhip_status status; /* OK; KERNEL_ARITHMETIC with `stage`; BUDGET when the
+                           request alone exceeds the budget; DEVICE          */
+  uint32_t stage;
+  jxrhip_image *image;  /* output when the request had no destination; the
+                           caller destroys it                                */
+  double latency_ms;    /* arrival -> output resident on the device          */
+  double queue_ms;      /* arrival -> admission                              */
+} jxrhip_server_result;
+
+typedef struct jxrhip_server_stats {
+  uint64_t submitted, admitted, completed, failed, rejected;
+  uint64_t waves, max_wave_images;
+  uint64_t pixels_completed;
+  uint64_t inflight_high_water_bytes; /* admitted footprint                 */
+  uint64_t hbm_high_water_bytes;      /* the session's device pool          */
+  double p50_ms, p99_ms;              /* latency of completed requests,
+                                         nearest rank                        */
+  double queue_p50_ms, queue_p99_ms;
+  double mean_wave_images;
+} jxrhip_server_stats;
+
+jxrhip_status jxrhip_server_create(jxrhip_session *session, const jxrhip_server_config *config,
+                                   jxrhip_server **out);
+/* The steady clock that arrival times are measured on, in nanoseconds. */
+uint64_t jxrhip_server_clock_ns(void);
+/* Queues a request. `arrival_ns` (0: now; later than now counts as now) is
+ * when the request arrived, for its latency. */
+jxrhip_status jxrhip_server_submit(jxrhip_server *server, const jxrhip_plan *plan, void *device_dst,
+                                   uint64_t dst_bytes, uint64_t tag, uint64_t arrival_ns, uint64_t *ticket);
+/* Moves up to `capacity` finished results to `out`, waiting up to
+ * `timeout_us` for the first; returns how many. */
+uint32_t jxrhip_server_poll(jxrhip_server *server, jxrhip_server_result *out, uint32_t capacity,
+                            uint32_t timeout_us);
+/* Waits until every submitted request has a result. */
+jxrhip_status jxrhip_server_drain(jxrhip_server *server);
+jxrhip_status jxrhip_server_get_stats(jxrhip_server *server, jxrhip_server_stats *out);
+/* Finishes every queued request, then stops; images of results never
+ * polled are released. */
+void jxrhip_server_destroy(jxrhip_server *server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,9 +20,10 @@ from .decoder import (
     quality_qp,
     surface_layout,
 )
+from .serving import HipServer, ServerResult
 
 __all__ = [
     "abi", "JxrHipError", "load", "EncodedImage", "HipBatch", "HipDecodePlan", "HipDecoderSession",
     "HipSubmission", "OutputSpec", "ResidentHipImage", "image_desc", "plan_hip_route", "quality_qp",
-    "surface_layout",
+    "surface_layout", "HipServer", "ServerResult",
 ]

@@ -188,6 +188,48 @@ class TileChoice(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_uint64),
+        ("max_inflight", C.c_uint32),
+        ("max_wave_images", C.c_uint32),
+        ("max_waves", C.c_uint32),
+        ("reserved0", C.c_uint32),
+    ]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [
+        ("ticket", C.c_uint64),
+        ("tag", C.c_uint64),
+        ("status", C.c_int32),
+        ("stage", C.c_uint32),
+        ("image", C.c_void_p),
+        ("latency_ms", C.c_double),
+        ("queue_ms", C.c_double),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64),
+        ("admitted", C.c_uint64),
+        ("completed", C.c_uint64),
+        ("failed", C.c_uint64),
+        ("rejected", C.c_uint64),
+        ("waves", C.c_uint64),
+        ("max_wave_images", C.c_uint64),
+        ("pixels_completed", C.c_uint64),
+        ("inflight_high_water_bytes", C.c_uint64),
+        ("hbm_high_water_bytes", C.c_uint64),
+        ("p50_ms", C.c_double),
+        ("p99_ms", C.c_double),
+        ("queue_p50_ms", C.c_double),
+        ("queue_p99_ms", C.c_double),
+        ("mean_wave_images", C.c_double),
+    ]
+
+
 class EncodeParams(C.Structure):
     _fields_ = [
         ("width", C.c_uint32),
@@ -270,6 +312,13 @@ SIGNATURES = {
     "jxrhip_batch_graph_create": (st, [vp, C.POINTER(vp), C.c_uint32, C.POINTER(vp), pp]),
     "jxrhip_batch_graph_launch": (st, [vp, pp]),
     "jxrhip_batch_graph_destroy": (None, [vp]),
+    "jxrhip_server_create": (st, [vp, C.POINTER(ServerConfig), pp]),
+    "jxrhip_server_clock_ns": (C.c_uint64, []),
+    "jxrhip_server_submit": (st, [vp, vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "jxrhip_server_poll": (C.c_uint32, [vp, C.POINTER(ServerResult), C.c_uint32, C.c_uint32]),
+    "jxrhip_server_drain": (st, [vp]),
+    "jxrhip_server_get_stats": (st, [vp, C.POINTER(ServerStats)]),
+    "jxrhip_server_destroy": (None, [vp]),
     # jxr_hip_encode.h
     "jxrhip_encode": (st, [vp, C.c_uint64, C.POINTER(EncodeParams), pp]),
     "jxrhip_encoded_destroy": (None, [vp]),

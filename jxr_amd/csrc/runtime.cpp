@@ -16,14 +16,19 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <list>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -596,6 +601,7 @@ class Pool {
       it->second.pop_back();
       out.bytes = bytes;
       retained_ -= bytes;
+      live_ += bytes;
       hits++;
       return hipSuccess;
     }
@@ -1285,8 +1291,9 @@ static void release_prepared(jxrhip_session *s, PreparedBatch &pb) {
 // slot records. `force` (tile_x, tile_y) overrides every other tile choice.
 // Called with the session lock held; on failure the caller releases `pb`.
 static jxrhip_status prepare_persistent(jxrhip_session *s, const jxrhip_plan *const *plans, uint32_t count,
-                                        void *const *dsts, const uint32_t *force, PreparedBatch &pb) {
-  hipStream_t stream = s->streams[0];
+                                        void *const *dsts, const uint32_t *force, PreparedBatch &pb,
+                                        hipStream_t upload_stream = nullptr) {
+  hipStream_t stream = upload_stream ? upload_stream : s->streams[0];
   pb.plans.assign(plans, plans + count);
   pb.dsts.assign(dsts, dsts + count);
   HIP_TRY(s->pool.get(sizeof(uint32_t) * std::max<uint32_t>(count, 1), pb.statuses, s->diag.scratch_hits,
@@ -1360,19 +1367,19 @@ static jxrhip_status prepare_persistent(jxrhip_session *s, const jxrhip_plan *co
 // after another on streams[0]; otherwise they go round-robin over the
 // session's streams, so that small buckets run side by side. Only kernel
 // launches and event dependencies, so that the sequence can be captured into
-// a graph.
-static jxrhip_status enqueue_persistent(jxrhip_session *s, const PreparedBatch &pb) {
-  hipStream_t s0 = s->streams[0];
+// a graph. With `only`, everything goes on that one stream instead.
+static jxrhip_status enqueue_persistent(jxrhip_session *s, const PreparedBatch &pb, hipStream_t only = nullptr) {
+  hipStream_t s0 = only ? only : s->streams[0];
   for (size_t i = 0; i < pb.plans.size(); i++) HIP_TRY(clear_bit_rows(pb.plans[i], pb.dsts[i], s0), "one-bit rows");
   bool spread = false;
   for (const BucketLaunch &L : pb.launches) spread |= L.items < 2 * L.grid;
-  const uint32_t used = spread ? (uint32_t)std::min<size_t>(pb.launches.size(), kStreams) : 1;
+  const uint32_t used = spread && !only ? (uint32_t)std::min<size_t>(pb.launches.size(), kStreams) : 1;
   if (used > 1) HIP_TRY(hipEventRecord(s->fork, s0), "hipEventRecord");
   for (uint32_t i = 1; i < used; i++) HIP_TRY(hipStreamWaitEvent(s->streams[i], s->fork, 0), "hipStreamWaitEvent");
   uint64_t item_base = 0;
   for (size_t li = 0; li < pb.launches.size(); li++) {
     const BucketLaunch &L = pb.launches[li];
-    hipStream_t ls = s->streams[li % used];
+    hipStream_t ls = used == 1 ? s0 : s->streams[li % used];
     KernelArgs args{};
     args.slots = (const Slot *)pb.slots.ptr + L.first;
     args.item_slot = (const uint32_t *)pb.item_slots.ptr + item_base;
@@ -1403,7 +1410,6 @@ struct jxrhip_batch_graph {
   hipGraphExec_t exec = nullptr;
   uint32_t *host_statuses = nullptr;
   hipEvent_t done = nullptr;
-  std::vector<uint64_t> coef_pins, desc_pins;  // cache entries kept resident
   bool in_flight = false;
 };
 
@@ -1754,4 +1760,385 @@ extern "C" jxrhip_status jxrhip_session_autotune(jxrhip_session *s, const jxrhip
     k++;
   }
   return JXRHIP_OK;
+}
+
+// ---- serving loop -----------------------------------------------------------
+// A worker thread admits requests first come, first served while their device
+// footprint fits the budget, and launches the admitted ones as waves: one
+// prepared persistent batch per wave, on one of four compute streams, with
+// uploads on a stream of their own. A wave starts whenever a compute stream is
+// free, so requests that arrive while others run are admitted as memory frees
+// up and their wave overlaps the running ones on the device.
+static constexpr uint32_t kServerStreams = 4;
+
+struct ServerRequest {
+  uint64_t ticket = 0, tag = 0;
+  const jxrhip_plan *plan = nullptr;
+  void *dst = nullptr;
+  uint64_t footprint = 0;
+  uint64_t arrival_ns = 0, admitted_ns = 0;
+  DeviceBuffer output;  // when the request brought no destination
+};
+
+struct ServerWave {
+  std::vector<ServerRequest *> reqs;
+  PreparedBatch pb;
+  uint32_t slot = 0;  // compute stream, completion event and status mirror
+};
+
+struct jxrhip_server {
+  jxrhip_session *s = nullptr;
+  jxrhip_server_config cfg{};
+  hipStream_t upload = nullptr;
+  hipStream_t compute[kServerStreams] = {};
+  hipEvent_t done[kServerStreams] = {};
+  uint32_t *host_status[kServerStreams] = {};
+  bool slot_busy[kServerStreams] = {};
+  std::thread worker;
+  std::mutex m;                   // arrivals, results, outstanding, stop, statistics
+  std::condition_variable wake;   // the worker: arrivals or stop
+  std::condition_variable ready;  // pollers and drain: results
+  std::deque<ServerRequest *> arrivals;
+  std::deque<jxrhip_server_result> results;
+  uint64_t next_ticket = 1;
+  uint64_t outstanding = 0;
+  bool stop = false;
+  jxrhip_server_stats stats{};
+  std::vector<double> latencies, queue_waits;
+  uint64_t wave_images = 0;
+  // the worker's own
+  std::deque<ServerRequest *> pending;
+  std::vector<ServerRequest *> admitted;
+  std::deque<ServerWave *> waves;
+  uint64_t inflight_bytes = 0;
+  uint32_t inflight_count = 0;
+};
+
+extern "C" uint64_t jxrhip_server_clock_ns(void) {
+  return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+static void server_finish(jxrhip_server *sv, ServerRequest *r, jxrhip_status status, uint32_t stage, uint64_t t) {
+  jxrhip_session *s = sv->s;
+  jxrhip_server_result res{};
+  res.ticket = r->ticket;
+  res.tag = r->tag;
+  res.status = status;
+  res.stage = stage;
+  res.latency_ms = (double)(t - r->arrival_ns) / 1e6;
+  res.queue_ms = r->admitted_ns ? (double)(r->admitted_ns - r->arrival_ns) / 1e6 : 0.0;
+  if (status == JXRHIP_OK && r->output.ptr) {
+    jxrhip_image *im = new (std::nothrow) jxrhip_image();
+    if (im) {
+      im->session = s;
+      im->buf = r->output;
+      im->bytes = r->plan->output_bytes;
+      r->output = DeviceBuffer{};
+      res.image = im;
+    } else {
+      res.status = JXRHIP_ERR_INVALID_ARGUMENT;
+    }
+  }
+  if (r->output.ptr) {
+    std::lock_guard<std::mutex> g(s->lock);
+    s->pool.put(r->output);
+  }
+  {
+    std::lock_guard<std::mutex> g(sv->m);
+    if (res.status == JXRHIP_OK) {
+      sv->stats.completed++;
+      sv->stats.pixels_completed += r->plan->pixels;
+      sv->latencies.push_back(res.latency_ms);
+      sv->queue_waits.push_back(res.queue_ms);
+    } else if (res.status == JXRHIP_ERR_BUDGET) {
+      sv->stats.rejected++;
+    } else {
+      sv->stats.failed++;
+    }
+    sv->results.push_back(res);
+    sv->outstanding--;
+  }
+  sv->ready.notify_all();
+  delete r;
+}
+
+static void server_launch_wave(jxrhip_server *sv, uint32_t slot) {
+  jxrhip_session *s = sv->s;
+  const size_t n = std::min<size_t>(sv->admitted.size(), sv->cfg.max_wave_images);
+  ServerWave *w = new ServerWave();
+  w->slot = slot;
+  w->reqs.assign(sv->admitted.begin(), sv->admitted.begin() + (long)n);
+  sv->admitted.erase(sv->admitted.begin(), sv->admitted.begin() + (long)n);
+  std::vector<const jxrhip_plan *> plans;
+  std::vector<void *> dsts;
+  jxrhip_status st = JXRHIP_OK;
+  {
+    std::lock_guard<std::mutex> g(s->lock);
+    make_current(s);
+    for (ServerRequest *r : w->reqs) {
+      if (!r->dst && !r->output.ptr &&
+          s->pool.get(r->plan->output_bytes, r->output, s->diag.scratch_hits, s->diag.scratch_misses) != hipSuccess) {
+        st = fail(JXRHIP_ERR_DEVICE, "output buffer");
+        break;
+      }
+      plans.push_back(r->plan);
+      dsts.push_back(r->dst ? r->dst : r->output.ptr);
+    }
+    if (!st) st = prepare_persistent(s, plans.data(), (uint32_t)n, dsts.data(), nullptr, w->pb, sv->upload);
+    if (!st) st = enqueue_persistent(s, w->pb, sv->compute[slot]);
+    if (!st && hipMemcpyAsync(sv->host_status[slot], w->pb.statuses.ptr, sizeof(uint32_t) * n,
+                              hipMemcpyDeviceToHost, sv->compute[slot]) != hipSuccess)
+      st = fail(JXRHIP_ERR_DEVICE, "status readback");
+    if (!st && hipEventRecord(sv->done[slot], sv->compute[slot]) != hipSuccess)
+      st = fail(JXRHIP_ERR_DEVICE, "hipEventRecord");
+    if (st) {
+      (void)hipStreamSynchronize(sv->compute[slot]);
+      release_prepared(s, w->pb);
+    }
+  }
+  if (st) {
+    const uint64_t t = jxrhip_server_clock_ns();
+    for (ServerRequest *r : w->reqs) {
+      sv->inflight_bytes -= r->footprint;
+      sv->inflight_count--;
+      server_finish(sv, r, st, 0, t);
+    }
+    delete w;
+    return;
+  }
+  {
+    std::lock_guard<std::mutex> g(sv->m);
+    sv->stats.waves++;
+    sv->wave_images += n;
+    sv->stats.max_wave_images = std::max<uint64_t>(sv->stats.max_wave_images, n);
+  }
+  sv->slot_busy[slot] = true;
+  sv->waves.push_back(w);
+}
+
+static void server_loop(jxrhip_server *sv) {
+  jxrhip_session *s = sv->s;
+  (void)hipSetDevice(s->device);
+  for (;;) {
+    bool progressed = false;
+    {
+      std::lock_guard<std::mutex> g(sv->m);
+      while (!sv->arrivals.empty()) {
+        sv->pending.push_back(sv->arrivals.front());
+        sv->arrivals.pop_front();
+        progressed = true;
+      }
+      if (sv->stop && sv->pending.empty() && sv->admitted.empty() && sv->waves.empty()) break;
+    }
+    // completed waves release their requests' footprint
+    for (auto it = sv->waves.begin(); it != sv->waves.end();) {
+      ServerWave *w = *it;
+      const hipError_t q = hipEventQuery(sv->done[w->slot]);
+      if (q == hipErrorNotReady) {
+        ++it;
+        continue;
+      }
+      const uint64_t t = jxrhip_server_clock_ns();
+      for (size_t i = 0; i < w->reqs.size(); i++) {
+        const uint32_t stage = q == hipSuccess ? sv->host_status[w->slot][i] : 0;
+        const jxrhip_status st =
+            q != hipSuccess ? JXRHIP_ERR_DEVICE : stage ? JXRHIP_ERR_KERNEL_ARITHMETIC : JXRHIP_OK;
+        sv->inflight_bytes -= w->reqs[i]->footprint;
+        sv->inflight_count--;
+        server_finish(sv, w->reqs[i], st, stage, t);
+      }
+      {
+        std::lock_guard<std::mutex> g(s->lock);
+        release_prepared(s, w->pb);
+      }
+      sv->slot_busy[w->slot] = false;
+      delete w;
+      it = sv->waves.erase(it);
+      progressed = true;
+    }
+    // admission, first come first served: a request waits for the ones
+    // before it, and one larger than the whole budget is refused
+    while (!sv->pending.empty()) {
+      ServerRequest *r = sv->pending.front();
+      if (r->footprint > sv->cfg.hbm_budget_bytes) {
+        sv->pending.pop_front();
+        server_finish(sv, r, JXRHIP_ERR_BUDGET, 0, jxrhip_server_clock_ns());
+        progressed = true;
+        continue;
+      }
+      if (sv->inflight_count >= sv->cfg.max_inflight || sv->inflight_bytes + r->footprint > sv->cfg.hbm_budget_bytes)
+        break;
+      sv->pending.pop_front();
+      r->admitted_ns = jxrhip_server_clock_ns();
+      sv->inflight_bytes += r->footprint;
+      sv->inflight_count++;
+      {
+        std::lock_guard<std::mutex> g(sv->m);
+        sv->stats.admitted++;
+        sv->stats.inflight_high_water_bytes = std::max(sv->stats.inflight_high_water_bytes, sv->inflight_bytes);
+      }
+      sv->admitted.push_back(r);
+      progressed = true;
+    }
+    if (!sv->admitted.empty()) {
+      for (uint32_t i = 0; i < sv->cfg.max_waves; i++)
+        if (!sv->slot_busy[i]) {
+          server_launch_wave(sv, i);
+          progressed = true;
+          break;
+        }
+    }
+    if (!progressed) {
+      std::unique_lock<std::mutex> g(sv->m);
+      if (sv->arrivals.empty() && !sv->stop)
+        sv->wake.wait_for(g, std::chrono::microseconds(sv->waves.empty() ? 1000 : 20));
+    }
+  }
+}
+
+extern "C" jxrhip_status jxrhip_server_create(jxrhip_session *s, const jxrhip_server_config *cfg,
+                                              jxrhip_server **out) {
+  if (!s || !out) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  std::unique_ptr<jxrhip_server> sv(new (std::nothrow) jxrhip_server());
+  if (!sv) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "out of host memory");
+  sv->s = s;
+  if (cfg) sv->cfg = *cfg;
+  make_current(s);
+  if (!sv->cfg.hbm_budget_bytes) {
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    sv->cfg.hbm_budget_bytes = free_b / 4 * 3;
+  }
+  if (!sv->cfg.max_inflight) sv->cfg.max_inflight = 4096;
+  if (!sv->cfg.max_wave_images) sv->cfg.max_wave_images = 256;
+  if (!sv->cfg.max_waves) sv->cfg.max_waves = kServerStreams;
+  if (sv->cfg.max_waves > kServerStreams) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "max_waves above 4");
+  if (sv->cfg.max_wave_images > 65536) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "max_wave_images above 65536");
+  auto undo = [&](jxrhip_status st) {
+    for (uint32_t i = 0; i < kServerStreams; i++) {
+      if (sv->compute[i]) (void)hipStreamDestroy(sv->compute[i]);
+      if (sv->done[i]) (void)hipEventDestroy(sv->done[i]);
+      if (sv->host_status[i]) (void)hipHostFree(sv->host_status[i]);
+    }
+    if (sv->upload) (void)hipStreamDestroy(sv->upload);
+    return st;
+  };
+  if (hipStreamCreateWithFlags(&sv->upload, hipStreamNonBlocking) != hipSuccess)
+    return undo(fail(JXRHIP_ERR_DEVICE, "hipStreamCreate"));
+  for (uint32_t i = 0; i < kServerStreams; i++) {
+    if (hipStreamCreateWithFlags(&sv->compute[i], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&sv->done[i], hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc((void **)&sv->host_status[i], sizeof(uint32_t) * sv->cfg.max_wave_images,
+                      hipHostMallocDefault) != hipSuccess)
+      return undo(fail(JXRHIP_ERR_DEVICE, "server streams"));
+  }
+  jxrhip_server *raw = sv.get();
+  try {
+    sv->worker = std::thread(server_loop, raw);
+  } catch (...) {
+    return undo(fail(JXRHIP_ERR_INVALID_ARGUMENT, "cannot start the server thread"));
+  }
+  *out = sv.release();
+  return JXRHIP_OK;
+}
+
+extern "C" jxrhip_status jxrhip_server_submit(jxrhip_server *sv, const jxrhip_plan *p, void *dst, uint64_t dst_bytes,
+                                              uint64_t tag, uint64_t arrival_ns, uint64_t *ticket) {
+  if (!sv || !p) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  if (dst && dst_bytes < p->output_bytes) return fail(JXRHIP_ERR_BUFFER_TOO_SMALL, "destination smaller than the plan's output");
+  ServerRequest *r = new (std::nothrow) ServerRequest();
+  if (!r) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "out of host memory");
+  r->plan = p;
+  r->dst = dst;
+  r->tag = tag;
+  r->footprint = p->coefficient_bytes + p->descriptor_bytes + (dst ? 0 : p->output_bytes);
+  const uint64_t now = jxrhip_server_clock_ns();
+  r->arrival_ns = arrival_ns && arrival_ns <= now ? arrival_ns : now;
+  {
+    std::lock_guard<std::mutex> g(sv->m);
+    if (sv->stop) {
+      delete r;
+      return fail(JXRHIP_ERR_INVALID_ARGUMENT, "server is stopping");
+    }
+    r->ticket = sv->next_ticket++;
+    sv->outstanding++;
+    sv->stats.submitted++;
+    sv->arrivals.push_back(r);
+    if (ticket) *ticket = r->ticket;
+  }
+  sv->wake.notify_one();
+  return JXRHIP_OK;
+}
+
+extern "C" uint32_t jxrhip_server_poll(jxrhip_server *sv, jxrhip_server_result *out, uint32_t capacity,
+                                       uint32_t timeout_us) {
+  if (!sv || !out || !capacity) return 0;
+  std::unique_lock<std::mutex> g(sv->m);
+  if (sv->results.empty() && timeout_us)
+    sv->ready.wait_for(g, std::chrono::microseconds(timeout_us), [&] { return !sv->results.empty(); });
+  uint32_t n = 0;
+  while (n < capacity && !sv->results.empty()) {
+    out[n++] = sv->results.front();
+    sv->results.pop_front();
+  }
+  return n;
+}
+
+extern "C" jxrhip_status jxrhip_server_drain(jxrhip_server *sv) {
+  if (!sv) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null server");
+  std::unique_lock<std::mutex> g(sv->m);
+  sv->ready.wait(g, [&] { return sv->outstanding == 0; });
+  return JXRHIP_OK;
+}
+
+static double nearest_rank(std::vector<double> v, double p) {
+  if (v.empty()) return 0.0;
+  std::sort(v.begin(), v.end());
+  size_t k = (size_t)std::ceil(p / 100.0 * (double)v.size());
+  k = std::min(std::max<size_t>(k, 1), v.size());
+  return v[k - 1];
+}
+
+extern "C" jxrhip_status jxrhip_server_get_stats(jxrhip_server *sv, jxrhip_server_stats *out) {
+  if (!sv || !out) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  std::vector<double> lat, queue;
+  {
+    std::lock_guard<std::mutex> g(sv->m);
+    *out = sv->stats;
+    lat = sv->latencies;
+    queue = sv->queue_waits;
+    out->mean_wave_images = sv->stats.waves ? (double)sv->wave_images / (double)sv->stats.waves : 0.0;
+  }
+  {
+    std::lock_guard<std::mutex> g(sv->s->lock);
+    out->hbm_high_water_bytes = sv->s->pool.high_water();
+  }
+  out->p50_ms = nearest_rank(lat, 50);
+  out->p99_ms = nearest_rank(lat, 99);
+  out->queue_p50_ms = nearest_rank(queue, 50);
+  out->queue_p99_ms = nearest_rank(queue, 99);
+  return JXRHIP_OK;
+}
+
+extern "C" void jxrhip_server_destroy(jxrhip_server *sv) {
+  if (!sv) return;
+  {
+    std::lock_guard<std::mutex> g(sv->m);
+    sv->stop = true;
+  }
+  sv->wake.notify_all();
+  if (sv->worker.joinable()) sv->worker.join();
+  for (auto &r : sv->results)
+    if (r.image) jxrhip_image_destroy(r.image);
+  make_current(sv->s);
+  for (uint32_t i = 0; i < kServerStreams; i++) {
+    if (sv->compute[i]) (void)hipStreamDestroy(sv->compute[i]);
+    if (sv->done[i]) (void)hipEventDestroy(sv->done[i]);
+    if (sv->host_status[i]) (void)hipHostFree(sv->host_status[i]);
+  }
+  if (sv->upload) (void)hipStreamDestroy(sv->upload);
+  delete sv;
 }

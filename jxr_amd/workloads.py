@@ -7,6 +7,9 @@
       4:2:0, overlap 1 or 2, quality 50..100 -> device-resident NCHW
   c4  128 x 1024x1024 RGBA f32 lognormal HDR + uniform alpha, lossless,
       separate alpha, overlap 1 -> planar f32 + alpha
+  c5  serving stream: Poisson arrivals at 2000/s, 70% RGB u8 256..1024 px
+      q75, 20% 4096x4096 gray u16 8x8 tiles, 10% 512x512 regions of an
+      8192x8192 RGB u8 source coded in 512-px tiles (c5_pool, c5_schedule)
 
 Entropy coding is not modelled: the forward analysis produces the arenas the
 entropy decoder would hand the reconstruction.
@@ -90,6 +93,73 @@ def _c4_one(i):
 def c4(count=128):
     with ThreadPoolExecutor(_threads()) as ex:
         return list(ex.map(_c4_one, range(count)))
+
+
+# ---- c5: serving stream ------------------------------------------------------
+C5_RATE = 2000.0      # requests per second (Poisson)
+C5_MIX = (0.7, 0.2, 0.1)  # small RGB u8, 4096x4096 gray u16, 512x512 region of 8192x8192
+
+
+def _c5_small(p):
+    px = synth.gradient_noise(p["w"], p["h"], p["seed"], depth=8, sigma=8.0)
+    qp = quality_qp(75)
+    enc = EncodedImage(px, color=abi.COLOR_YUV444, overlap=1, scaled=True, qp=(qp, qp, qp + 4), seed=p["seed"])
+    return enc, rgb_out(8, False)
+
+
+def _c5_gray(seed):
+    px = synth.pink_noise(4096, 4096, seed)
+    qp = quality_qp(75)
+    enc = EncodedImage(px, color=abi.COLOR_LUMA, overlap=2, tiles=([32] * 8, [32] * 8), hard_tiles=False,
+                       scaled=True, qp=(qp, qp, qp + 4), bands=abi.BANDS_ALL)
+    return enc, OutputSpec(storage=abi.STORE_U16, depth=abi.DEPTH_U16, layout=abi.LAYOUT_LUMA,
+                           output_color=abi.COLOR_LUMA)
+
+
+def c5_pool(small=96, gray=2, regions=64, seed=SEED + 5000, cache=True):
+    """The distinct images serving requests draw from: `small` RGB u8 images
+    of 256..1024 px a side (step 16) at quality 75, 4:4:4, overlap 1 ->
+    interleaved RGB u8; `gray` 4096x4096 u16 pink-noise images in an 8x8
+    grid of soft tiles, overlap 2, quality 75 -> u16; and `regions` 512x512
+    windows at 16-aligned positions of one 8192x8192 RGB u8 source coded in
+    16x16 tiles of 512 px, each as the arena a region decode prepares.
+    Returns {"small": [...], "gray": [...], "region": [...]} of
+    (image, spec). Without `cache` the arenas have no identity, so every
+    request uploads its coefficients again."""
+    from .prepare import RegionImage
+    rng = np.random.default_rng(seed)
+    ps = [dict(w=int(rng.integers(16, 65)) * 16, h=int(rng.integers(16, 65)) * 16, seed=seed + i)
+          for i in range(small)]
+    with ThreadPoolExecutor(_threads()) as ex:
+        smalls = list(ex.map(_c5_small, ps))
+        grays = list(ex.map(_c5_gray, [seed + 1000 + i for i in range(gray)]))
+    src_px = synth.gradient_noise_large(8192, 8192, seed + 2000)
+    qp = quality_qp(75)
+    source = EncodedImage(src_px, color=abi.COLOR_YUV444, overlap=1, tiles=([32] * 16, [32] * 16),
+                          hard_tiles=False, scaled=True, qp=(qp, qp, qp + 4))
+    del src_px
+    regs = []
+    for _ in range(regions):
+        x = int(rng.integers(0, (8192 - 512) // 16 + 1)) * 16
+        y = int(rng.integers(0, (8192 - 512) // 16 + 1)) * 16
+        reg = RegionImage(source, (x, y, 512, 512), cache=cache)
+        spec = rgb_out(8, False)
+        spec.crop = (x, y, 512, 512)
+        regs.append((reg, spec))
+    if not cache:
+        for enc, _ in smalls + grays:
+            enc.primary.contents.identity = 0
+    return {"small": smalls, "gray": grays, "region": regs, "source": source}
+
+
+def c5_schedule(seconds, rate=C5_RATE, mix=C5_MIX, counts=(96, 2, 64), seed=SEED + 6000):
+    """Poisson arrivals over `seconds`: (arrival seconds, kind, pool index)."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.poisson(rate * seconds))
+    t = np.sort(rng.uniform(0.0, seconds, n))
+    kinds = rng.choice(3, size=n, p=np.asarray(mix) / sum(mix))
+    names = ("small", "gray", "region")
+    return [(float(t[i]), names[k], int(rng.integers(0, counts[k]))) for i, k in enumerate(kinds)]
 
 
 WORKLOADS = {

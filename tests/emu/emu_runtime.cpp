@@ -113,6 +113,11 @@ hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b) {
   *ms = (float)(b->t - a->t);
   return hipSuccess;
 }
+hipError_t hipMemGetInfo(size_t *free_bytes, size_t *total_bytes) {
+  *free_bytes = size_t(8) << 30;
+  *total_bytes = size_t(8) << 30;
+  return hipSuccess;
+}
 hipError_t hipMalloc(void **p, size_t bytes) {
   *p = malloc(bytes ? bytes : 1);
   return *p ? hipSuccess : hipErrorInvalidValue;

@@ -120,6 +120,7 @@ hipError_t hipEventQuery(hipEvent_t e);
 hipError_t hipEventSynchronize(hipEvent_t e);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipMalloc(void **p, size_t bytes);
+hipError_t hipMemGetInfo(size_t *free_bytes, size_t *total_bytes);
 hipError_t hipFree(void *p);
 hipError_t hipHostMalloc(void **p, size_t bytes, unsigned flags);
 hipError_t hipHostFree(void *p);

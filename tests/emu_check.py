@@ -97,6 +97,42 @@ def main():
         if st or not np.array_equal(b.to_host(), want):
             print(f"MISMATCH tuned batch {name} stage {st}", flush=True)
             failures += 1
+    # serving loop: every case in one mix, half into destinations, half into
+    # server-allocated outputs, under a budget that admits a few at a time
+    from jxr_amd import HipServer
+    budget = max(p.output_bytes for p in plans) * 6 + (64 << 20)
+    server = HipServer(session, hbm_budget_bytes=budget, max_wave_images=5, max_waves=2)
+    dsts = {}
+    for i, plan in enumerate(plans):
+        if i % 2 == 0:
+            b = session.allocate_destination(plan.output_bytes)
+            b.fill(0x3C)
+            dsts[i] = b
+            server.submit(plan, b.ptr, plan.output_bytes, tag=i)
+        else:
+            server.submit(plan, tag=i)
+    server.drain()
+    results = server.poll(capacity=len(plans) + 8)
+    if sorted(r.tag for r in results) != list(range(len(plans))):
+        print(f"SERVER returned tags {sorted(r.tag for r in results)}", flush=True)
+        failures += 1
+    for r in results:
+        got = dsts[r.tag].to_host() if r.tag in dsts else (r.image.readback() if r.image else None)
+        if r.status or got is None or not np.array_equal(got[: wants[r.tag].nbytes], wants[r.tag]):
+            print(f"MISMATCH server {names[r.tag]} status {r.status} stage {r.stage}", flush=True)
+            failures += 1
+        if r.image:
+            r.image.close()
+    st = server.stats()
+    if st["completed"] != len(plans) or st["inflight_high_water_bytes"] > budget or st["waves"] < 2:
+        print(f"SERVER stats {st}", flush=True)
+        failures += 1
+    server.close()
+    # pool accounting: reused buffers count as live while held
+    hw = session.diagnostics().scratch_high_water_bytes
+    if not 0 < hw < (1 << 36):
+        print(f"POOL high water {hw}", flush=True)
+        failures += 1
     print(f"emulated parity: {failures} failures over {len(names)} cases", flush=True)
     sys.exit(1 if failures else 0)
 
