@@ -556,7 +556,7 @@ struct CornerRange {
   __device__ int count() const { return (mx1 - mx0) * (my1 - my0) * 2; }
 };
 __device__ __forceinline__ CornerRange corner_range(const DevImage &im, const DevPlane &p, int tmx, int tmy, int nx, int ny) {
-  const int h = (int)low_halo(p.bc);
+  const int h = (int)low_halo(p.bc, p.overlap);
   return CornerRange{max(tmx - h, 0), min(tmx + nx + h, (int)im.mbx), max(tmy - h, 0),
                      min(tmy + ny + h, (int)im.mby)};
 }
@@ -1626,7 +1626,7 @@ __device__ __forceinline__ void plane_setup(TileShared &ts, int pl, uint32_t til
   g.soy = tmy * (int)g.mh - kSampleHaloRows;
   g.srw = (uint32_t)TX * g.mw + 8;
   g.srh = (uint32_t)TY * g.mh + 2 * kSampleHaloRows;
-  const int h = (int)low_halo(p.bc);
+  const int h = (int)low_halo(p.bc, p.overlap);
   g.lox = (tmx - h) * (int)p.bc;
   g.loy = (tmy - h) * (int)p.br;
   g.lrw = (uint32_t)(TX + 2 * h) * p.bc;
@@ -1720,7 +1720,7 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
   for (int t = threadIdx.x; t < ts.sA[np]; t += blockDim.x) {
     const int pl = plane_of(ts.sA, np, t);
     const DevPlane &p = im.planes[pl];
-    const int h = (int)low_halo(p.bc);
+    const int h = (int)low_halo(p.bc, p.overlap);
     const DivMod u = divmod(t - ts.sA[pl], ts.w[pl].wA, ts.w[pl].mA);
     const int mx = tmx - h + u.r, my = tmy - h + u.q;
     if (mx < 0 || my < 0 || mx >= (int)im.mbx || my >= (int)im.mby) continue;

@@ -132,9 +132,12 @@ struct KernelArgs {
   uint32_t storage;      // JXRHIP_STORE_* of every image of the launch
 };
 
-// Halo of the low-pass region in macroblocks: the subsampled first-level
-// overlap reads a corner's neighbour one pair beyond the block halo.
-__host__ __device__ inline uint32_t low_halo(uint32_t bc) { return bc == 2 ? 2u : 1u; }
+// Halo of the low-pass region in macroblocks: one macroblock holds the DC of
+// every block in the one-block halo of the second level; the subsampled
+// first-level overlap (mode 2) reads a corner's neighbour one pair further.
+__host__ __device__ inline uint32_t low_halo(uint32_t bc, uint32_t overlap) {
+  return bc == 2 && overlap == 2 ? 2u : 1u;
+}
 
 // Rows of second-level output kept above and below a tile: the sample-level
 // overlap reaches two rows past the tile edge and chroma upsampling one.
@@ -147,9 +150,9 @@ __host__ __device__ inline uint32_t sample_region_words(uint32_t bc, uint32_t br
                                                         uint32_t ty) {
   return (tx * bc * 4 + 8) * (ty * br * 4 + 2 * kSampleHaloRows);
 }
-__host__ __device__ inline uint32_t low_region_words(uint32_t bc, uint32_t br, uint32_t tx,
+__host__ __device__ inline uint32_t low_region_words(uint32_t bc, uint32_t br, uint32_t overlap, uint32_t tx,
                                                      uint32_t ty) {
-  uint32_t h = low_halo(bc);
+  uint32_t h = low_halo(bc, overlap);
   return (tx + 2 * h) * bc * (ty + 2 * h) * br;
 }
 

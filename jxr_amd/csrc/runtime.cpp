@@ -972,7 +972,7 @@ extern "C" jxrhip_status jxrhip_session_clear_cache(jxrhip_session *s) {
 static uint32_t tile_words(const jxrhip_plan *p, uint32_t tx, uint32_t ty) {
   uint32_t w = 0;
   for (const PlanPlane &pp : p->planes)
-    w += sample_region_words(pp.bc, pp.br, tx, ty) + low_region_words(pp.bc, pp.br, tx, ty);
+    w += sample_region_words(pp.bc, pp.br, tx, ty) + low_region_words(pp.bc, pp.br, pp.overlap, tx, ty);
   return w;
 }
 
@@ -1317,6 +1317,9 @@ static jxrhip_status prepare_persistent(jxrhip_session *s, const jxrhip_plan *co
   for (auto &kv : buckets) {
     uint32_t tx, ty, words;
     choose_tile(s, plans[kv.second[0]], tx, ty, words, force);
+    // the bucket shares a tile shape; its images' halos (and so their LDS)
+    // can differ with the overlap mode
+    for (uint32_t i : kv.second) words = std::max(words, tile_words(plans[i], tx, ty));
     if ((size_t)words * 4 > s->max_lds) return fail(JXRHIP_ERR_INVALID_PLAN, "tile does not fit in LDS");
     BucketLaunch L{slots.size(), (uint32_t)kv.second.size(), 0, tx, ty, words, plans[kv.second[0]]->dev.storage, 0};
     for (uint32_t i : kv.second) {
@@ -1716,7 +1719,9 @@ extern "C" jxrhip_status jxrhip_session_autotune(jxrhip_session *s, const jxrhip
     std::vector<std::pair<uint32_t, uint32_t>> shapes{{dx, dy}};
     for (auto &c : cand) {
       if (c[0] == dx && c[1] == dy) continue;
-      if ((size_t)tile_words(p0, c[0], c[1]) * 4 > std::min<size_t>(s->max_lds, 64 * 1024)) continue;
+      uint32_t w = 0;
+      for (const jxrhip_plan *p : bp) w = std::max(w, tile_words(p, c[0], c[1]));
+      if ((size_t)w * 4 > std::min<size_t>(s->max_lds, 64 * 1024)) continue;
       shapes.push_back({c[0], c[1]});
     }
     double best_ms = 1e30, default_ms = 0;
