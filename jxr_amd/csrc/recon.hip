@@ -115,10 +115,25 @@ struct Checked {
 // lifting step to prove them).
 constexpr uint32_t kPctBound = 1u << 25;
 constexpr uint32_t kPotBound = 1u << 23;
+// Inputs of magnitude below 2^20 give second-level transform outputs below
+// 64x that (the transform's gain is under 36, tests/test_bounds.py), so 64x
+// the input bound stands in for the outputs' magnitudes where only "below
+// 2^27" matters.
+constexpr uint32_t kGainBoundInput = 1u << 20;
+constexpr uint32_t kPctGainShift = 6;
+// Likewise the sample-level overlap operators (gain under 100): inputs whose
+// magnitudes OR to below 2^18 give outputs below 128 x (that OR + 1).
+constexpr uint32_t kPotGainInput = 1u << 18;
+constexpr uint32_t kPotGainShift = 7;
+
 
 // An upper bound of |v| (exclusive) for a set of values: OR of v ^ (v >> 31)
 // plus one.
 __device__ __forceinline__ uint32_t magbits(int v) { return (uint32_t)(v ^ (v >> 31)); }
+__device__ __forceinline__ uint32_t pot_out_mag(uint32_t in_mag, int a0, int a1, int a2, int a3) {
+  return in_mag < kPotGainInput ? (in_mag + 1u) << kPotGainShift
+                                : magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
+}
 
 // ---- core transform (crates/jxr-math/src/transform.rs:8-361) -------------
 template <class A>
@@ -448,7 +463,9 @@ __device__ __forceinline__ void first_level_mb(const DevImage &im, const DevPlan
 // 4-tap edge strips across the split; slots split along both are four 2x2
 // corners (reconstruct/overlap.rs:124-323; device_plan/overlap.rs:125-178).
 // Only positions inside [0, width) x [0, height) are touched.
-// Returns the OR of magbits of the values written.
+// Returns the OR of magbits of the values written or, for inputs whose
+// magnitudes OR to below 2^18, 128 times (that OR + 1): below 2^26 and above
+// every written value's magnitude.
 __device__ __forceinline__ uint32_t overlap_slot(int *s, int rw, int ox, int oy, int kx, int ky, int width,
                                                  int height, bool sx, bool sy, uint32_t *status) {
   const int x0 = 4 * kx + 2, y0 = 4 * ky + 2;
@@ -470,8 +487,12 @@ __device__ __forceinline__ uint32_t overlap_slot(int *s, int rw, int ox, int oy,
 #pragma unroll
     for (int k = 0; k < 16; k++) mag |= magbits(v[k]);
     ovf = RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap4x4(v, a); });
+    if (mag < kPotGainInput) {
+      omag = (mag + 1u) << kPotGainShift;
+    } else {
 #pragma unroll
-    for (int k = 0; k < 16; k++) omag |= magbits(v[k]);
+      for (int k = 0; k < 16; k++) omag |= magbits(v[k]);
+    }
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       *reinterpret_cast<int2 *>(base + r * rw) = make_int2(v[r * 4 + 0], v[r * 4 + 1]);
@@ -487,7 +508,7 @@ __device__ __forceinline__ uint32_t overlap_slot(int *s, int rw, int ox, int oy,
       int a0 = q[0], a1 = q[rw], a2 = q[2 * rw], a3 = q[3 * rw];
       const uint32_t mag = magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
       ovf |= RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap4(a0, a1, a2, a3, a); });
-      omag |= magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
+      omag |= pot_out_mag(mag, a0, a1, a2, a3);
       q[0] = a0;
       q[rw] = a1;
       q[2 * rw] = a2;
@@ -502,7 +523,7 @@ __device__ __forceinline__ uint32_t overlap_slot(int *s, int rw, int ox, int oy,
       int a0 = q[0], a1 = q[1], a2 = q[2], a3 = q[3];
       const uint32_t mag = magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
       ovf |= RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap4(a0, a1, a2, a3, a); });
-      omag |= magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
+      omag |= pot_out_mag(mag, a0, a1, a2, a3);
       q[0] = a0;
       q[1] = a1;
       q[2] = a2;
@@ -519,7 +540,7 @@ __device__ __forceinline__ uint32_t overlap_slot(int *s, int rw, int ox, int oy,
         int a0 = q[0], a1 = q[1], a2 = q[rw], a3 = q[rw + 1];
         const uint32_t mag = magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
         ovf |= RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap4(a0, a1, a2, a3, a); });
-        omag |= magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
+        omag |= pot_out_mag(mag, a0, a1, a2, a3);
         q[0] = a0;
         q[1] = a1;
         q[rw] = a2;
@@ -652,7 +673,10 @@ __device__ __forceinline__ BlockRange block_range(const DevPlane &p, const Plane
                     max(tmy * (int)p.br - 1, 0), min((tmy + ny) * (int)p.br + 1, (int)g.lh)};
 }
 
-// Returns the OR of magbits of the block's sixteen samples.
+// Returns the OR of magbits of the block's sixteen samples or, when the
+// inputs are below 2^20, 64 times their bound: a value below 2^26 and above
+// every sample's magnitude, which is all the output stage's test against
+// 2^27 needs.
 __device__ __forceinline__ uint32_t second_level_block(const DevImage &im, const DevPlane &p, const PlaneGeo &g,
                                                        const int *low, int *smp, const BlockRange &br, DivMod t,
                                                        uint32_t *status) {
@@ -664,6 +688,7 @@ __device__ __forceinline__ uint32_t second_level_block(const DevImage &im, const
 #pragma unroll
   for (int k = 0; k < 16; k++) v[k] = 0;
   bool povf = false, ovf = false;
+  uint32_t hp_bound = 0;  // bound on |v[1..15]| after dequantization
   if (r.bands >= JXRHIP_BANDS_NO_FLEXBITS) {
     const JXR_GLOBAL int *c = gptr(p.coef) + r.coef_off;
     const int k = (rr << g.bcs) + cc;
@@ -715,19 +740,31 @@ __device__ __forceinline__ uint32_t second_level_block(const DevImage &im, const
     uint32_t mag = 0;
 #pragma unroll
     for (int j = 1; j < 16; j++) mag |= magbits(v[j]);
-    if ((unsigned long long)(mag + 1ull) * r.qhp <= 0x7fffffffull) {
+    // |residual| <= mag + 1, so every dequantized value is within the product
+    const unsigned long long prod = (mag + 1ull) * r.qhp;
+    if (prod <= 0x7fffffffull) {
 #pragma unroll
       for (int j = 1; j < 16; j++) v[j] *= (int)r.qhp;
+      hp_bound = (uint32_t)prod;
     } else {
 #pragma unroll
       for (int j = 1; j < 16; j++) v[j] = dequant_checked(v[j], r.qhp, ovf);
+      hp_bound = 0xffffffffu;
     }
   }
   v[0] = low[(by - g.loy) * (int)g.lrw + (bx - g.lox)];
-  uint32_t mag = 0;
+  // a bound on the transform's inputs: below kPctBound it selects the plain
+  // transform without scanning the sixteen values (the scan would agree)
+  const uint32_t in_bound = max(hp_bound, magbits(v[0]) + 1u);
+  if (in_bound < kPctBound) {
+    Fast a;
+    inverse_core(v, a);
+  } else {
+    uint32_t mag = 0;
 #pragma unroll
-  for (int j = 0; j < 16; j++) mag |= magbits(v[j]);
-  ovf |= RUN_BOUNDED(mag, kPctBound, [&](auto &a) { inverse_core(v, a); });
+    for (int j = 0; j < 16; j++) mag |= magbits(v[j]);
+    ovf |= RUN_BOUNDED(mag, kPctBound, [&](auto &a) { inverse_core(v, a); });
+  }
   if (povf) fail(status, JXRHIP_STAGE_PREDICTION);
   if (ovf) fail(status, JXRHIP_STAGE_SECOND_TRANSFORM);
   // rows outside the region (the outer half of a halo block) are not kept
@@ -737,6 +774,7 @@ __device__ __forceinline__ uint32_t second_level_block(const DevImage &im, const
   for (int y = 0; y < 4; y++)
     if ((uint32_t)(r0 + y) < g.srh)
       *reinterpret_cast<v4i *>(o + y * (int)g.srw) = v4i{v[y * 4], v[y * 4 + 1], v[y * 4 + 2], v[y * 4 + 3]};
+  if (in_bound < kGainBoundInput) return in_bound << kPctGainShift;
   uint32_t omag = 0;
 #pragma unroll
   for (int j = 0; j < 16; j++) omag |= magbits(v[j]);

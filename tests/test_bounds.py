@@ -263,3 +263,119 @@ def test_fast_path_bound_keeps_every_intermediate_in_i32(fn, n, oracle, bits):
     out = fn([Iv(-B, B - 1) for _ in range(n)])
     assert Iv.peak <= I32_MAX, f"{oracle}: intermediate reaches {Iv.peak} for inputs below 2^{bits}"
     assert all(isinstance(o, Iv) for o in out)
+
+
+# ---- gains: output magnitude per input bound --------------------------------
+def _peak_out(fn, n, B):
+    out = fn([Iv(-B, B) for _ in range(n)])
+    return max(max(abs(o.lo), abs(o.hi)) for o in out)
+
+
+@pytest.mark.parametrize("B", [1, 2, 3, 7, 100, 1023, 2**15, 2**19 + 5, 2**20])
+def test_second_level_gain_bound(B):
+    """recon.hip stands 64 x the input bound in for the magnitudes of the
+    second-level transform's outputs when inputs are below 2^20
+    (kGainBoundInput, kPctGainShift): outputs stay below 64 x the bound at
+    every scale (the gain approaches 35.3 for large inputs; rounding adds
+    a little for tiny ones), so below 2^26."""
+    peak = _peak_out(inverse_core, 16, B)
+    assert peak < 64 * B
+    if B >= 100:
+        assert peak <= 36 * B
+
+
+@pytest.mark.parametrize("fn,n", [(inverse_overlap4x4, 16), (inverse_overlap4, 4), (inverse_overlap2x2, 4),
+                                  (inverse_overlap2, 2)], ids=["4x4", "4", "2x2", "2"])
+@pytest.mark.parametrize("B", [1, 2, 5, 100, 2**12, 2**18])
+def test_overlap_gain_bound(fn, n, B):
+    """recon.hip stands 128 x (input OR + 1) in for the magnitudes of the
+    sample-level overlap's outputs when the OR of the inputs' magnitudes is
+    below 2^18 (kPotGainInput, kPotGainShift): |input| <= OR + 1 = B, and the
+    outputs stay below 128 B."""
+    assert _peak_out(fn, n, B) < 128 * B
+    assert 128 * 2**18 < 2**26
+
+
+def test_gain_bound_is_not_loose_by_construction():
+    """The bound above is met with little room: a 16-input interval's output
+    reaches more than 30 x its bound, so a transform with a larger gain (a
+    dropped halving, say) would fail it."""
+    assert _peak_out(inverse_core, 16, 2**20) > 30 * 2**20
+
+
+# ---- output stage under plain arithmetic ------------------------------------
+def _ceil_half(v):
+    # (x >> 1) + (x & 1), monotone in x
+    return Iv(-((-v.lo) >> 1), -((-v.hi) >> 1))
+
+
+def _weighted_fast(a, b, wa):
+    """b + (((a - b) * wa + 4) >> 3), wa in 0..8 (recon.hip `weighted`,
+    Fast). The intermediates are propagated as intervals; the result is
+    floor((a*wa + b*(8-wa) + 4) / 8), a rounded convex combination, so it
+    lies between min(a, b) and max(a, b) (which interval arithmetic, losing
+    the correlation of the two uses of b, cannot see; checked by brute force
+    in test_weighted_stays_between_its_inputs)."""
+    _ = ((a - b) * wa + 4) >> 3
+    return Iv(min(a.lo, b.lo), max(a.hi, b.hi))
+
+
+def test_weighted_stays_between_its_inputs():
+    for a in range(-20, 21):
+        for b in range(-20, 21):
+            for wa in range(9):
+                r = b + (((a - b) * wa + 4) >> 3)
+                assert min(a, b) <= r <= max(a, b)
+
+
+def _upsampled(S):
+    """Every chroma value the plain output path computes from samples in S:
+    two passes of weighted picks (centered) or the centering-0 averages."""
+    outs = []
+    for wa in range(9):
+        once = _weighted_fast(S, S, wa)
+        outs.append(_weighted_fast(once, once, wa))
+    avg = (S + S + 1) >> 1
+    outs.append((avg + avg + 1) >> 1)
+    lo = min(o.lo for o in outs)
+    hi = max(o.hi for o in outs)
+    return Iv(lo, hi)
+
+
+def test_plain_output_arithmetic_stays_in_i32():
+    """The output stage runs plain i32 on a tile when every sample is below
+    2^27 in magnitude and every channel's |bias| + rounding is below 2^29
+    (kOutputSampleBound, out_fast): chroma interpolation, the YUV/YUVK
+    inverses and the scale's additions then stay inside i32."""
+    Iv.peak = 0
+    S = Iv(-(2**27), 2**27 - 1)
+    C = _upsampled(S)
+    bias = Iv(-(2**29), 2**29)
+    outs = []
+    # YUV -> RGB (output_format/color.rs:100-110)
+    y, u, v = S, C, C
+    t0 = -u
+    g = y - (t0 >> 1)
+    rr = (t0 + g) - _ceil_half(v)
+    outs += [rr, g, v + rr]
+    # YUVK -> CMYK (color.rs:112-123)
+    k = S
+    black = k + (y >> 1)
+    magenta = (black - y) - (u >> 1)
+    cyan = (u + magenta) + (v >> 1)
+    outs += [cyan, magenta, cyan - v, black]
+    for o in outs + [S, C]:
+        _ = o + bias  # bias and rounding, added before the shift
+    assert Iv.peak <= I32_MAX
+
+
+def test_plain_output_bound_is_tight_enough_to_matter():
+    """Samples of 2^29 would not be safe under the same arithmetic, so the
+    2^27 limit is doing work."""
+    Iv.peak = 0
+    S = Iv(-(2**29), 2**29 - 1)
+    t0 = -S
+    g = S - (t0 >> 1)
+    rr = (t0 + g) - _ceil_half(S)
+    _ = (S + rr) + Iv(-(2**29), 2**29)
+    assert Iv.peak > I32_MAX
