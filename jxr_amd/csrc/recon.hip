@@ -403,8 +403,12 @@ __device__ __forceinline__ bool row_boundary(const DevPlane &p, uint32_t mby, ui
 // Dequantize DC/LP and run the first-level inverse transform of macroblock
 // (mx, my), written to the plane's low region.
 __device__ __forceinline__ void first_level_mb(const DevImage &im, const DevPlane &p, const PlaneGeo &g, int *low,
-                               int mx, int my, uint32_t *status) {
+                               uint32_t *tab, int tx, int ty, int tabw, int tabh, int mx, int my,
+                               uint32_t *status) {
   const MbRec r = load_mb(p.mbs, (uint32_t)my * im.mbx + (uint32_t)mx);
+  // what the second level needs of the record, for macroblocks it reaches
+  if ((unsigned)tx < (unsigned)tabw && (unsigned)ty < (unsigned)tabh)
+    *reinterpret_cast<v4u *>(tab + 4 * (ty * tabw + tx)) = v4u{r.coef_off, r.qhp, r.bands | (r.pred << 8), 0u};
   const JXR_GLOBAL int *c = gptr(p.coef) + r.coef_off;
   const bool full = p.bc == 4;
   const uint32_t blocks = p.bc * p.br;
@@ -678,12 +682,19 @@ __device__ __forceinline__ BlockRange block_range(const DevPlane &p, const Plane
 // every sample's magnitude, which is all the output stage's test against
 // 2^27 needs.
 __device__ __forceinline__ uint32_t second_level_block(const DevImage &im, const DevPlane &p, const PlaneGeo &g,
-                                                       const int *low, int *smp, const BlockRange &br, DivMod t,
+                                                       const int *low, int *smp, const uint32_t *tab, int tmx,
+                                                       int tmy, int tabw, const BlockRange &br, DivMod t,
                                                        uint32_t *status) {
   const int bx = br.bx0 + t.r, by = br.by0 + t.q;
   const int mx = bx >> g.bcs, my = by >> g.brs;
   const int cc = bx & ((int)g.bc - 1), rr = by & ((int)g.br - 1);
-  const MbRec r = load_mb(p.mbs, (uint32_t)my * im.mbx + (uint32_t)mx);
+  // the macroblock's record as the first level left it in the table
+  const v4u e = *reinterpret_cast<const v4u *>(tab + 4 * ((my - tmy + 1) * tabw + (mx - tmx + 1)));
+  MbRec r;
+  r.coef_off = e[0];
+  r.qhp = e[1];
+  r.bands = e[2] & 0xffu;
+  r.pred = e[2] >> 8;
   int v[16];
 #pragma unroll
   for (int k = 0; k < 16; k++) v[k] = 0;
@@ -1618,6 +1629,8 @@ struct TileShared {
   PlaneWork w[kMaxPlanes];
   int soff[kMaxPlanes];  // sample region of each plane, in dynamic LDS words
   int loff[kMaxPlanes];  // low region of each plane
+  int toff[kMaxPlanes];  // macroblock table of each plane
+  int tabw, tabh;        // macroblock table entries across and down
   int sA[kMaxPlanes + 1], sB1[kMaxPlanes + 1], sB2[kMaxPlanes + 1], sB3[kMaxPlanes + 1];
   int sC[kMaxPlanes + 1], sD[kMaxPlanes + 1];
   int tmx, tmy, nx, ny;
@@ -1647,6 +1660,8 @@ __device__ __forceinline__ void plane_setup(TileShared &ts, int pl, uint32_t til
     ts.tmy = tmy;
     ts.nx = nx;
     ts.ny = ny;
+    ts.tabw = TX + 2;
+    ts.tabh = TY + 2;
   }
   const DevPlane &p = im.planes[pl];
   PlaneGeo &g = ts.g[pl];
@@ -1711,6 +1726,10 @@ __device__ __forceinline__ void tile_finish(TileShared &ts) {
     ts.loff[pl] = cursor;
     cursor += (int)(ts.g[pl].lrw * ts.g[pl].lrh);
   }
+  for (int pl = 0; pl < np; pl++) {
+    ts.toff[pl] = cursor;
+    cursor += ts.tabw * ts.tabh * 4;
+  }
   ts.any_lp = any_lp;
   ts.any_sub = any_sub;
   ts.any_pot = any_pot;
@@ -1762,7 +1781,8 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
     const DivMod u = divmod(t - ts.sA[pl], ts.w[pl].wA, ts.w[pl].mA);
     const int mx = tmx - h + u.r, my = tmy - h + u.q;
     if (mx < 0 || my < 0 || mx >= (int)im.mbx || my >= (int)im.mby) continue;
-    first_level_mb(im, p, ts.g[pl], lds + ts.loff[pl], mx, my, status);
+    first_level_mb(im, p, ts.g[pl], lds + ts.loff[pl], reinterpret_cast<uint32_t *>(lds + ts.toff[pl]),
+                   mx - tmx + 1, my - tmy + 1, ts.tabw, ts.tabh, mx, my, status);
   }
   __syncthreads();
 
@@ -1804,8 +1824,9 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
   for (int t = threadIdx.x; t < ts.sC[np]; t += blockDim.x) {
     const int pl = plane_of(ts.sC, np, t);
     const DivMod u = divmod(t - ts.sC[pl], ts.w[pl].wC, ts.w[pl].mC);
-    omag |= second_level_block(im, im.planes[pl], ts.g[pl], lds + ts.loff[pl], lds + ts.soff[pl], ts.br[pl], u,
-                               status);
+    omag |= second_level_block(im, im.planes[pl], ts.g[pl], lds + ts.loff[pl], lds + ts.soff[pl],
+                               reinterpret_cast<const uint32_t *>(lds + ts.toff[pl]), tmx, tmy, ts.tabw, ts.br[pl],
+                               u, status);
   }
   if (!ts.any_pot) atomicOr(&ts.out_mag, omag);
   __syncthreads();

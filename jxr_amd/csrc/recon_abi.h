@@ -155,5 +155,9 @@ __host__ __device__ inline uint32_t low_region_words(uint32_t bc, uint32_t br, u
   uint32_t h = low_halo(bc, overlap);
   return (tx + 2 * h) * bc * (ty + 2 * h) * br;
 }
+// Per plane, the second level's view of the macroblock records of the tile
+// and its one-macroblock halo (coefficient offset, HP step, bands and HP
+// direction: four words each), written by the first level.
+__host__ __device__ inline uint32_t mb_table_words(uint32_t tx, uint32_t ty) { return (tx + 2) * (ty + 2) * 4; }
 
 }  // namespace jxr

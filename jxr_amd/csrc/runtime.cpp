@@ -972,7 +972,8 @@ extern "C" jxrhip_status jxrhip_session_clear_cache(jxrhip_session *s) {
 static uint32_t tile_words(const jxrhip_plan *p, uint32_t tx, uint32_t ty) {
   uint32_t w = 0;
   for (const PlanPlane &pp : p->planes)
-    w += sample_region_words(pp.bc, pp.br, tx, ty) + low_region_words(pp.bc, pp.br, pp.overlap, tx, ty);
+    w += sample_region_words(pp.bc, pp.br, tx, ty) + low_region_words(pp.bc, pp.br, pp.overlap, tx, ty) +
+         mb_table_words(tx, ty);
   return w;
 }
 
