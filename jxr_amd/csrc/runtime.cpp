@@ -1697,8 +1697,11 @@ extern "C" jxrhip_status jxrhip_session_autotune(jxrhip_session *s, const jxrhip
   std::map<uint64_t, std::vector<uint32_t>> buckets;
   for (uint32_t i = 0; i < count; i++) buckets[plans[i]->geometry_key].push_back(i);
   if (bucket_count) *bucket_count = (uint32_t)buckets.size();
-  static const uint32_t cand[][2] = {{4, 4}, {8, 4}, {4, 8}, {8, 2}, {2, 8}, {4, 2}, {2, 4},
-                                     {8, 8}, {16, 2}, {16, 4}, {2, 2}, {4, 1}};
+  // widths and heights that are not powers of two pack a stage's items
+  // into fewer rounds of 256 threads for some plane layouts
+  static const uint32_t cand[][2] = {{4, 4}, {8, 4}, {4, 8}, {8, 2}, {2, 8}, {4, 2}, {2, 4}, {8, 8},
+                                     {16, 2}, {16, 4}, {2, 2}, {4, 1}, {6, 4}, {4, 6}, {6, 2}, {6, 3},
+                                     {5, 4}, {3, 4}, {4, 3}, {5, 3}, {6, 6}, {12, 2}, {10, 2}, {3, 3}};
   std::lock_guard<std::mutex> g(s->lock);
   make_current(s);
   hipStream_t s0 = s->streams[0];
