@@ -1646,15 +1646,85 @@ __device__ __forceinline__ int plane_of(const int *start, int np, int t) {
 
 __device__ __forceinline__ uint32_t log2_small(uint32_t v) { return v == 4 ? 2u : v == 2 ? 1u : 0u; }
 
-// Tile layout, in two steps: thread pl lays out plane pl (regions, ranges,
-// work widths and counts), then thread 0 places the regions in LDS and sums
-// the per-plane work counts of every stage.
+// What one plane adds to a tile: its LDS region sizes (words) and its work
+// items in each stage. Cheap enough for every plane's thread to compute for
+// every plane, so that offsets and prefix sums need no second pass.
+struct PlaneShare {
+  int sw, lw, tw;                   // sample region, low region, macroblock table
+  int cA, cB1, cB2, cB3, cC, cD;    // work items per stage
+  int lp, sub, pot;                 // LP overlap, subsampled LP overlap, sample overlap
+};
+
+__device__ __forceinline__ PlaneShare plane_share(const DevImage &im, int q, int tmx, int tmy, int nx, int ny,
+                                                  int TX, int TY) {
+  const DevPlane &p = im.planes[q];
+  const int bc = (int)p.bc, br = (int)p.br;
+  const int h = (int)low_halo(p.bc, p.overlap);
+  const int lw = (int)im.mbx * bc, lh = (int)im.mby * br;
+  const bool lp = p.overlap == 2, sub = lp && bc == 2;
+  PlaneShare c;
+  c.sw = (int)sample_region_words(p.bc, p.br, (uint32_t)TX, (uint32_t)TY);
+  c.lw = (int)low_region_words(p.bc, p.br, p.overlap, (uint32_t)TX, (uint32_t)TY);
+  c.tw = (int)mb_table_words((uint32_t)TX, (uint32_t)TY);
+  const int cmx0 = max(tmx - h, 0), cmx1 = min(tmx + nx + h, (int)im.mbx);
+  const int cmy0 = max(tmy - h, 0), cmy1 = min(tmy + ny + h, (int)im.mby);
+  const int corners = (cmx1 - cmx0) * (cmy1 - cmy0) * 2;
+  c.cA = (nx + 2 * h) * (ny + 2 * h);
+  c.cB1 = !lp ? 0 : bc == 4 ? (nx + 1) * (ny + 1) : corners;
+  c.cB2 = sub ? cell_columns(nx) * cell_rows(p, ny) : 0;
+  c.cB3 = sub ? corners : 0;
+  c.cC = (min((tmx + nx) * bc + 1, lw) - max(tmx * bc - 1, 0)) * (min((tmy + ny) * br + 1, lh) - max(tmy * br - 1, 0));
+  c.cD = p.overlap >= 1 ? (nx * bc + 1) * (ny * br + 1) : 0;
+  c.lp = lp;
+  c.sub = sub;
+  c.pot = p.overlap >= 1;
+  return c;
+}
+
+// Tile layout, one thread per plane: plane pl's regions, ranges, work
+// widths and counts, its regions' place in LDS and its entries of the
+// stages' prefix sums over planes (recomputing the earlier planes' shares
+// in registers rather than waiting for their threads).
 __device__ __forceinline__ void plane_setup(TileShared &ts, int pl, uint32_t tile, uint32_t tiles_x, int TX,
                                             int TY) {
   const DevImage &im = ts.im;
+  const int np = (int)im.plane_count;
   const uint32_t ty_idx = tile / tiles_x;
   const int tmx = (int)(tile - ty_idx * tiles_x) * TX, tmy = (int)ty_idx * TY;
   const int nx = min(TX, (int)im.mbx - tmx), ny = min(TY, (int)im.mby - tmy);
+  // offsets and prefix sums over planes
+  int ps = 0, pl_ = 0, pt = 0, pA = 0, pB1 = 0, pB2 = 0, pB3 = 0, pC = 0, pD = 0;
+  int ts_ = 0, tl = 0, any_lp = 0, any_sub = 0, any_pot = 0;
+  for (int q = 0; q < np; q++) {
+    const PlaneShare c = plane_share(im, q, tmx, tmy, nx, ny, TX, TY);
+    if (q < pl) {
+      ps += c.sw;
+      pl_ += c.lw;
+      pt += c.tw;
+    }
+    if (q <= pl) {
+      pA += c.cA;
+      pB1 += c.cB1;
+      pB2 += c.cB2;
+      pB3 += c.cB3;
+      pC += c.cC;
+      pD += c.cD;
+    }
+    ts_ += c.sw;
+    tl += c.lw;
+    any_lp |= c.lp;
+    any_sub |= c.sub;
+    any_pot |= c.pot;
+  }
+  ts.soff[pl] = ps;
+  ts.loff[pl] = ts_ + pl_;
+  ts.toff[pl] = ts_ + tl + pt;
+  ts.sA[pl + 1] = pA;
+  ts.sB1[pl + 1] = pB1;
+  ts.sB2[pl + 1] = pB2;
+  ts.sB3[pl + 1] = pB3;
+  ts.sC[pl + 1] = pC;
+  ts.sD[pl + 1] = pD;
   if (pl == 0) {
     ts.tmx = tmx;
     ts.tmy = tmy;
@@ -1662,6 +1732,11 @@ __device__ __forceinline__ void plane_setup(TileShared &ts, int pl, uint32_t til
     ts.ny = ny;
     ts.tabw = TX + 2;
     ts.tabh = TY + 2;
+    ts.sA[0] = ts.sB1[0] = ts.sB2[0] = ts.sB3[0] = ts.sC[0] = ts.sD[0] = 0;
+    ts.any_lp = any_lp;
+    ts.any_sub = any_sub;
+    ts.any_pot = any_pot;
+    ts.out_mag = 0;
   }
   const DevPlane &p = im.planes[pl];
   PlaneGeo &g = ts.g[pl];
@@ -1688,7 +1763,6 @@ __device__ __forceinline__ void plane_setup(TileShared &ts, int pl, uint32_t til
   const CornerRange cr = corner_range(im, p, tmx, tmy, nx, ny);
   ts.br[pl] = brg;
   ts.cr[pl] = cr;
-  const bool lp = p.overlap == 2, sub = lp && p.bc == 2;
   PlaneWork &w = ts.w[pl];
   w.wA = (uint32_t)(nx + 2 * h);
   w.wB = p.bc == 4 ? (uint32_t)(nx + 1) : (uint32_t)(cr.mx1 - cr.mx0);
@@ -1700,49 +1774,6 @@ __device__ __forceinline__ void plane_setup(TileShared &ts, int pl, uint32_t til
   w.mB2 = div_magic(w.wB2);
   w.mC = div_magic(w.wC);
   w.mD = div_magic(w.wD);
-  // per-plane counts, summed by thread 0
-  ts.sA[pl + 1] = (nx + 2 * h) * (ny + 2 * h);
-  ts.sB1[pl + 1] = !lp ? 0 : p.bc == 4 ? (nx + 1) * (ny + 1) : cr.count();
-  ts.sB2[pl + 1] = sub ? cell_columns(nx) * cell_rows(p, ny) : 0;
-  ts.sB3[pl + 1] = sub ? cr.count() : 0;
-  ts.sC[pl + 1] = brg.count();
-  ts.sD[pl + 1] = p.overlap >= 1 ? (nx * (int)p.bc + 1) * (ny * (int)p.br + 1) : 0;
-}
-
-__device__ __forceinline__ void tile_finish(TileShared &ts) {
-  const DevImage &im = ts.im;
-  const int np = (int)im.plane_count;
-  int cursor = 0;
-  int any_lp = 0, any_sub = 0, any_pot = 0;
-  for (int pl = 0; pl < np; pl++) {
-    ts.soff[pl] = cursor;
-    cursor += (int)(ts.g[pl].srw * ts.g[pl].srh);
-    const DevPlane &p = im.planes[pl];
-    any_lp |= p.overlap == 2;
-    any_sub |= p.overlap == 2 && p.bc == 2;
-    any_pot |= p.overlap >= 1;
-  }
-  for (int pl = 0; pl < np; pl++) {
-    ts.loff[pl] = cursor;
-    cursor += (int)(ts.g[pl].lrw * ts.g[pl].lrh);
-  }
-  for (int pl = 0; pl < np; pl++) {
-    ts.toff[pl] = cursor;
-    cursor += ts.tabw * ts.tabh * 4;
-  }
-  ts.any_lp = any_lp;
-  ts.any_sub = any_sub;
-  ts.any_pot = any_pot;
-  ts.out_mag = 0;
-  ts.sA[0] = ts.sB1[0] = ts.sB2[0] = ts.sB3[0] = ts.sC[0] = ts.sD[0] = 0;
-  for (int pl = 0; pl < np; pl++) {
-    ts.sA[pl + 1] += ts.sA[pl];
-    ts.sB1[pl + 1] += ts.sB1[pl];
-    ts.sB2[pl + 1] += ts.sB2[pl];
-    ts.sB3[pl + 1] += ts.sB3[pl];
-    ts.sC[pl + 1] += ts.sC[pl];
-    ts.sD[pl + 1] += ts.sD[pl];
-  }
 }
 
 // One tile of one image. `reload` is false when the previous tile of this
@@ -1763,8 +1794,6 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
   }
   for (int pl = threadIdx.x; pl < (int)ts.im.plane_count; pl += blockDim.x)
     plane_setup(ts, pl, tile, slot.tiles_x, (int)args.tile_x, (int)args.tile_y);
-  __syncthreads();
-  if (threadIdx.x == 0) tile_finish(ts);
   __syncthreads();
   const DevImage &im = ts.im;
   const int np = (int)im.plane_count;
