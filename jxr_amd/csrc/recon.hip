@@ -367,7 +367,10 @@ __device__ __forceinline__ int dequant_checked(int v, uint32_t step, bool &ovf) 
 // (m = 0 encodes d = 1). Exact for n * d < 2^32, far above any work-item
 // count of a tile; gfx950 has no integer divide instruction.
 __device__ __forceinline__ uint32_t div_magic(uint32_t d) { return 0xFFFFFFFFu / d + 1u; }
-__device__ __forceinline__ uint32_t div_by(uint32_t n, uint32_t m) { return m ? __umulhi(n, m) : n; }
+__device__ __forceinline__ uint32_t div_by(uint32_t n, uint32_t m) {
+  const uint32_t q = __umulhi(n, m);  // a select, not a branch around the multiply
+  return m ? q : n;
+}
 struct DivMod {
   int q, r;
 };
@@ -1638,9 +1641,22 @@ struct TileShared {
   uint32_t out_mag;  // OR of magbits of every sample the tile's output stage can read
 };
 
-__device__ __forceinline__ int plane_of(const int *start, int np, int t) {
+// A stage's per-plane item starts, read once into registers before its loop.
+struct Starts {
+  int v[kMaxPlanes];
+};
+__device__ __forceinline__ Starts starts_of(const int *start, int np) {
+  Starts s;
+#pragma unroll
+  for (int k = 0; k < (int)kMaxPlanes; k++) s.v[k] = k < np ? start[k] : 0x7fffffff;
+  return s;
+}
+// Plane of flattened item t: how many later planes start at or before t
+// (compares and adds, no loop to leave lane by lane).
+__device__ __forceinline__ int plane_of(const Starts &s, int t) {
   int pl = 0;
-  while (pl + 1 < np && t >= start[pl + 1]) pl++;
+#pragma unroll
+  for (int k = 1; k < (int)kMaxPlanes; k++) pl += t >= s.v[k] ? 1 : 0;
   return pl;
 }
 
@@ -1803,8 +1819,9 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
   if (threadIdx.x == 0 && claim_from) claim = atomicAdd(claim_from, 1u);
 
   // stage A: first level of the tile's macroblocks and their halo
+  const Starts stA = starts_of(ts.sA, np);
   for (int t = threadIdx.x; t < ts.sA[np]; t += blockDim.x) {
-    const int pl = plane_of(ts.sA, np, t);
+    const int pl = plane_of(stA, t);
     const DevPlane &p = im.planes[pl];
     const int h = (int)low_halo(p.bc, p.overlap);
     const DivMod u = divmod(t - ts.sA[pl], ts.w[pl].wA, ts.w[pl].mA);
@@ -1817,8 +1834,9 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
 
   // stage B: first-level overlap (mode 2)
   if (ts.any_lp) {
+    const Starts stB1 = starts_of(ts.sB1, np);
     for (int t = threadIdx.x; t < ts.sB1[np]; t += blockDim.x) {
-      const int pl = plane_of(ts.sB1, np, t);
+      const int pl = plane_of(stB1, t);
       const DevPlane &p = im.planes[pl];
       const PlaneGeo &g = ts.g[pl];
       const DivMod u = divmod(t - ts.sB1[pl], ts.w[pl].wB, ts.w[pl].mB);
@@ -1833,14 +1851,16 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
     }
     if (ts.any_sub) {
       __syncthreads();
+      const Starts stB2 = starts_of(ts.sB2, np);
       for (int t = threadIdx.x; t < ts.sB2[np]; t += blockDim.x) {
-        const int pl = plane_of(ts.sB2, np, t);
+        const int pl = plane_of(stB2, t);
         const DivMod u = divmod(t - ts.sB2[pl], ts.w[pl].wB2, ts.w[pl].mB2);
         subsampled_cell_item(im, im.planes[pl], ts.g[pl], lds + ts.loff[pl], tmx, tmy, u, status);
       }
       __syncthreads();
+      const Starts stB3 = starts_of(ts.sB3, np);
       for (int t = threadIdx.x; t < ts.sB3[np]; t += blockDim.x) {
-        const int pl = plane_of(ts.sB3, np, t);
+        const int pl = plane_of(stB3, t);
         const DivMod u = divmod(t - ts.sB3[pl], ts.w[pl].wB, ts.w[pl].mB);
         subsampled_corner_item(im, im.planes[pl], ts.g[pl], lds + ts.loff[pl], ts.cr[pl], u, true, status);
       }
@@ -1850,8 +1870,9 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
 
   // stage C: high-pass prediction and second level, tile blocks + halo
   uint32_t omag = 0;  // magnitude of every sample this thread writes in stages C and D
+  const Starts stC = starts_of(ts.sC, np);
   for (int t = threadIdx.x; t < ts.sC[np]; t += blockDim.x) {
-    const int pl = plane_of(ts.sC, np, t);
+    const int pl = plane_of(stC, t);
     const DivMod u = divmod(t - ts.sC[pl], ts.w[pl].wC, ts.w[pl].mC);
     omag |= second_level_block(im, im.planes[pl], ts.g[pl], lds + ts.loff[pl], lds + ts.soff[pl],
                                reinterpret_cast<const uint32_t *>(lds + ts.toff[pl]), tmx, tmy, ts.tabw, ts.br[pl],
@@ -1862,8 +1883,9 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
 
   // stage D: sample-level overlap (modes 1, 2)
   if (ts.any_pot) {
+    const Starts stD = starts_of(ts.sD, np);
     for (int t = threadIdx.x; t < ts.sD[np]; t += blockDim.x) {
-      const int pl = plane_of(ts.sD, np, t);
+      const int pl = plane_of(stD, t);
       const DevPlane &p = im.planes[pl];
       const PlaneGeo &g = ts.g[pl];
       const DivMod u = divmod(t - ts.sD[pl], ts.w[pl].wD, ts.w[pl].mD);
