@@ -401,6 +401,21 @@ __device__ __forceinline__ bool col_boundary(const DevPlane &p, uint32_t mbx, ui
 __device__ __forceinline__ bool row_boundary(const DevPlane &p, uint32_t mby, uint32_t m) {
   return m == 0 || m == mby || mask_bit(p.rowmask, m);
 }
+// Region boundaries at macroblock positions t0 .. t0 + n (n <= 16; bit i
+// stands for t0 + i): the window's edges and the hard tile edges of `mask`.
+// The mask holds extent / 32 + 2 words, so the word after t0's is in bounds.
+__device__ __forceinline__ uint32_t boundary_bits(const uint32_t *mask, uint32_t extent, int t0, int n) {
+  uint32_t b = 0;
+  if (mask) {
+    const JXR_GLOBAL uint32_t *m = gptr(mask);
+    const uint32_t w = (uint32_t)t0 >> 5;
+    const uint64_t pair = ((uint64_t)m[w + 1] << 32) | m[w];
+    b = (uint32_t)(pair >> (t0 & 31));
+  }
+  if (t0 == 0) b |= 1u;
+  if ((int)extent - t0 <= n) b |= 1u << ((int)extent - t0);
+  return b & ((2u << n) - 1u);
+}
 
 // ---- stage bodies, one work item each ---------------------------------------
 // Dequantize DC/LP and run the first-level inverse transform of macroblock
@@ -1634,6 +1649,7 @@ struct TileShared {
   int loff[kMaxPlanes];  // low region of each plane
   int toff[kMaxPlanes];  // macroblock table of each plane
   int tabw, tabh;        // macroblock table entries across and down
+  uint32_t colb[kMaxPlanes], rowb[kMaxPlanes];  // region boundaries at MB columns tmx.. and rows tmy..
   int sA[kMaxPlanes + 1], sB1[kMaxPlanes + 1], sB2[kMaxPlanes + 1], sB3[kMaxPlanes + 1];
   int sC[kMaxPlanes + 1], sD[kMaxPlanes + 1];
   int tmx, tmy, nx, ny;
@@ -1779,6 +1795,8 @@ __device__ __forceinline__ void plane_setup(TileShared &ts, int pl, uint32_t til
   const CornerRange cr = corner_range(im, p, tmx, tmy, nx, ny);
   ts.br[pl] = brg;
   ts.cr[pl] = cr;
+  ts.colb[pl] = boundary_bits(p.colmask, im.mbx, tmx, nx);
+  ts.rowb[pl] = boundary_bits(p.rowmask, im.mby, tmy, ny);
   PlaneWork &w = ts.w[pl];
   w.wA = (uint32_t)(nx + 2 * h);
   w.wB = p.bc == 4 ? (uint32_t)(nx + 1) : (uint32_t)(cr.mx1 - cr.mx0);
@@ -1843,7 +1861,7 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
       if (p.bc == 4) {
         const int kx = tmx - 1 + u.r, ky = tmy - 1 + u.q;
         overlap_slot(lds + ts.loff[pl], (int)g.lrw, g.lox, g.loy, kx, ky, (int)g.lw, (int)g.lh,
-                     col_boundary(p, im.mbx, (uint32_t)(kx + 1)), row_boundary(p, im.mby, (uint32_t)(ky + 1)),
+                     (ts.colb[pl] >> (kx + 1 - tmx)) & 1u, (ts.rowb[pl] >> (ky + 1 - tmy)) & 1u,
                      status);
       } else {
         subsampled_corner_item(im, p, g, lds + ts.loff[pl], ts.cr[pl], u, false, status);
@@ -1891,8 +1909,9 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
       const DivMod u = divmod(t - ts.sD[pl], ts.w[pl].wD, ts.w[pl].mD);
       const int kx = (tmx << g.bcs) - 1 + u.r, ky = (tmy << g.brs) - 1 + u.q;
       const int px = 4 * kx + 4, py = 4 * ky + 4;
-      const bool xs = (px & ((int)g.mw - 1)) == 0 && col_boundary(p, im.mbx, (uint32_t)(px >> (g.bcs + 2)));
-      const bool ys = (py & ((int)g.mh - 1)) == 0 && row_boundary(p, im.mby, (uint32_t)(py >> (g.brs + 2)));
+      // macroblock positions tmx .. tmx + nx (rows likewise), so the shifts stay in 0..16
+      const bool xs = ((px & ((int)g.mw - 1)) == 0) & ((ts.colb[pl] >> ((px >> (g.bcs + 2)) - tmx)) & 1u);
+      const bool ys = ((py & ((int)g.mh - 1)) == 0) & ((ts.rowb[pl] >> ((py >> (g.brs + 2)) - tmy)) & 1u);
       omag |= overlap_slot(lds + ts.soff[pl], (int)g.srw, g.sox, g.soy, kx, ky, (int)g.pw, (int)g.ph, xs, ys,
                            status);
     }
