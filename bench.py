@@ -245,10 +245,20 @@ def main():
         f"device {np.median(batched['device_ms']):.3f} ms, launches/step {batched['launches']}")
     if graph is not None:
         graph.close()
-    per_image = None
+    per_image = per_image_sync = None
     if not args.no_per_image:
         k = max(3, args.steps // 2)
         per_image = time_path(session, lambda: session.submit_batch(plans, ptrs, abi.BATCH_PER_IMAGE), k, 1)
+        # the drop-in used as the reference's decoder uses its backend: one
+        # image submitted and waited for at a time
+        sizes = [p.output_bytes for p in plans]
+
+        class _Each:
+            def wait(self):
+                for p, d, n in zip(plans, ptrs, sizes):
+                    session.submit_into(p, d, n).wait()
+        per_image_sync = time_path(session, _Each, max(2, k // 2), 1)
+        log(f"[bench] per-image, one at a time: median wall {np.median(per_image_sync['wall_ms']):.3f} ms")
         log(f"[bench] per-image: median wall {np.median(per_image['wall_ms']):.3f} ms, "
             f"device {np.median(per_image['device_ms']):.3f} ms")
 
@@ -297,6 +307,11 @@ def main():
                             "latency_ms": {"p50": round(percentile(per_image["wall_ms"], 50), 4),
                                            "p99": round(percentile(per_image["wall_ms"], 99), 4)}}
         out["batched_over_per_image"] = round(value / pi_value, 3)
+        ps_ms = per_image_sync["total_ms"] / len(per_image_sync["wall_ms"])
+        ps_value = pixels / (ps_ms / 1e3) / 1e6
+        out["per_image_one_at_a_time"] = {"value": round(ps_value, 2), "unit": "Mpixel/s",
+                                          "ms_per_step": round(ps_ms, 4)}
+        out["batched_over_per_image_one_at_a_time"] = round(value / ps_value, 3)
     if args.serve_seconds > 0:
         out["serving"] = serve(session, args.serve_seconds, args.serve_uploads == "resident", args.serve_budget_gb)
     if not args.no_cpu_baseline:
