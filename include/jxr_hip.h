@@ -298,6 +298,9 @@ uint64_t jxrhip_plan_algorithmic_bytes(const jxrhip_plan *plan);
 
 /* ---- sessions (session.rs:16-268) --------------------------------------- */
 jxrhip_status jxrhip_session_create(int32_t device_ordinal, jxrhip_session **out);
+/* Waits for the session's streams and frees it. Refused (the session stays,
+ * and jxrhip_last_error says why) while captured batches or servers of the
+ * session exist: destroy those first. */
 void jxrhip_session_destroy(jxrhip_session *session);
 jxrhip_status jxrhip_session_diagnostics(jxrhip_session *session, jxrhip_diagnostics *out);
 /* Upload-cache byte budget (default 64 GiB of the device's HBM). */

@@ -672,7 +672,7 @@ struct jxrhip_session {
   uint32_t next_stream = 0;
   uint32_t tile_x = 0, tile_y = 0, groups_per_cu = 0;
   std::map<uint64_t, std::pair<uint32_t, uint32_t>> tuned;  // bucket key -> tile shape
-  uint32_t live_graphs = 0;
+  uint32_t live_graphs = 0, live_servers = 0;  // objects that must be destroyed before the session
   double last_ms = 0;
   uint32_t last_launches = 0;
   jxrhip_diagnostics diag{};
@@ -750,6 +750,15 @@ extern "C" jxrhip_status jxrhip_session_create(int32_t ordinal, jxrhip_session *
 
 extern "C" void jxrhip_session_destroy(jxrhip_session *s) {
   if (!s) return;
+  {
+    // graphs and servers point into the session: destroying it under them
+    // would leave them dangling, so it is refused (and the session leaks)
+    std::lock_guard<std::mutex> g(s->lock);
+    if (s->live_graphs || s->live_servers) {
+      (void)fail(JXRHIP_ERR_INVALID_ARGUMENT, "session still has captured batches or servers");
+      return;
+    }
+  }
   make_current(s);
   for (uint32_t i = 0; i < kStreams; i++)
     if (s->streams[i]) (void)hipStreamSynchronize(s->streams[i]);
@@ -1544,6 +1553,7 @@ extern "C" jxrhip_status jxrhip_batch_wait(jxrhip_batch *b, uint32_t *stages) {
     if (hipEventElapsedTime(&ms, s->t0, s->t1) == hipSuccess) s->last_ms = ms;
   }
   if (b->graph) {
+    std::lock_guard<std::mutex> g(s->lock);
     b->graph->in_flight = false;
   } else {
     {
@@ -2050,6 +2060,10 @@ extern "C" jxrhip_status jxrhip_server_create(jxrhip_session *s, const jxrhip_se
   } catch (...) {
     return undo(fail(JXRHIP_ERR_INVALID_ARGUMENT, "cannot start the server thread"));
   }
+  {
+    std::lock_guard<std::mutex> g(s->lock);
+    s->live_servers++;
+  }
   *out = sv.release();
   return JXRHIP_OK;
 }
@@ -2149,5 +2163,9 @@ extern "C" void jxrhip_server_destroy(jxrhip_server *sv) {
     if (sv->host_status[i]) (void)hipHostFree(sv->host_status[i]);
   }
   if (sv->upload) (void)hipStreamDestroy(sv->upload);
+  {
+    std::lock_guard<std::mutex> g(sv->s->lock);
+    sv->s->live_servers--;
+  }
   delete sv;
 }
