@@ -1505,20 +1505,9 @@ __device__ __forceinline__ uint32_t recipe_store(const RecipeScale &r, int v) {
   return (uint32_t)min(max((v + r.add) >> r.shift, r.lo), r.hi);
 }
 
-// Four pixels of three interleaved or planar channels (c[0..2] in stored
-// order) on output row oy.
+// Four interleaved three-channel pixels from px0.
 template <uint32_t B>
-__device__ __forceinline__ void store_rgb_row(const DevImage &im, JXR_GLOBAL uint8_t *out, uint32_t oy, int ox0,
-                                              uint32_t valid, const uint32_t c[3][4]) {
-  if (im.planar) {
-#pragma unroll
-    for (uint32_t k = 0; k < 3; k++) {
-      JXR_GLOBAL uint8_t *row = out + im.plane_offset[k] + (uint64_t)oy * im.row_stride[k];
-      store_run<B>(row + (int64_t)ox0 * B, c[k], valid);
-    }
-    return;
-  }
-  JXR_GLOBAL uint8_t *px0 = out + im.plane_offset[0] + (uint64_t)oy * im.row_stride[0] + (int64_t)ox0 * 3 * B;
+__device__ __forceinline__ void store_rgb_interleaved(JXR_GLOBAL uint8_t *px0, uint32_t valid, const uint32_t c[3][4]) {
   if (valid == 0xFu && (reinterpret_cast<uintptr_t>(px0) & 3u) == 0) {
     constexpr uint32_t W = 3 * B;
     constexpr uint32_t mask = B == 4 ? 0xFFFFFFFFu : (1u << (8 * B)) - 1u;
@@ -1543,26 +1532,37 @@ __device__ __forceinline__ void store_rgb_row(const DevImage &im, JXR_GLOBAL uin
   }
 }
 
+// What the YUV -> RGB recipe reads of the image for every group of a tile,
+// read once per tile: each output plane's first row (interleaved output uses
+// plane 0) and stride, and the luma sample region's origin and row length.
+struct RgbCtx {
+  JXR_GLOBAL uint8_t *row0[3];
+  uint64_t stride[3];
+  const int *luma;  // sample (0, 0) of the tile's coordinates: luma + y * lw + x
+  int lw;
+  int crop_x, crop_y;
+  bool planar, reverse, upsample;
+};
+
 // YUV -> RGB/BGR (output_format/color.rs:100-110), three stored channels.
 template <uint32_t STORE>
-__device__ __forceinline__ void rgb3_group(const OutCtx &o, JXR_GLOBAL uint8_t *out, int gx, int gy, int x0, int x1,
+__device__ __forceinline__ void rgb3_group(const OutCtx &o, const RgbCtx &c3, int gx, int gy, int x0, int x1,
                                            int y0, int y1, const RecipeScale &rs) {
   constexpr uint32_t B = StoreBytes<STORE>::value;
-  const DevImage &im = *o.im;
   const int xb = 4 * gx, yb = 2 * gy;
   int chroma[2][2][4];
-  if (im.upsample) group_chroma<Fast>(o, gx, gy, chroma);
+  if (c3.upsample) group_chroma<Fast>(o, gx, gy, chroma);
   uint32_t valid = 0;
 #pragma unroll
   for (int i = 0; i < 4; i++) valid |= (xb + i >= x0 && xb + i < x1) ? 1u << i : 0u;
-  const int ox0 = xb - (int)im.crop_x;
+  const int ox0 = xb - c3.crop_x;
 #pragma unroll 1
   for (int j = 0; j < 2; j++) {
     const int y = yb + j;
     if (y < y0 || y >= y1) continue;
-    const v4i l = sample_quad(o, 0, xb, y);
+    const v4i l = *reinterpret_cast<const v4i *>(c3.luma + y * c3.lw + xb);
     v4i u, v;
-    if (im.upsample) {
+    if (c3.upsample) {
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         u[i] = j ? chroma[0][1][i] : chroma[0][0][i];
@@ -1579,11 +1579,17 @@ __device__ __forceinline__ void rgb3_group(const OutCtx &o, JXR_GLOBAL uint8_t *
       const int g = l[i] - (t0 >> 1);
       const int r = t0 + g - ((v[i] >> 1) + (v[i] & 1));
       const int b = v[i] + r;
-      c[0][i] = recipe_store(rs, im.reverse ? b : r);
+      c[0][i] = recipe_store(rs, c3.reverse ? b : r);
       c[1][i] = recipe_store(rs, g);
-      c[2][i] = recipe_store(rs, im.reverse ? r : b);
+      c[2][i] = recipe_store(rs, c3.reverse ? r : b);
     }
-    store_rgb_row<B>(im, out, (uint32_t)(y - (int)im.crop_y), ox0, valid, c);
+    const uint64_t oy = (uint32_t)(y - c3.crop_y);
+    if (c3.planar) {
+#pragma unroll
+      for (uint32_t k = 0; k < 3; k++) store_run<B>(c3.row0[k] + oy * c3.stride[k] + (int64_t)ox0 * B, c[k], valid);
+    } else {
+      store_rgb_interleaved<B>(c3.row0[0] + oy * c3.stride[0] + (int64_t)ox0 * 3 * B, valid, c);
+    }
   }
 }
 
@@ -1617,10 +1623,25 @@ __device__ __forceinline__ void recipe_tile(const OutCtx &o, JXR_GLOBAL uint8_t 
   const DevImage &im = *o.im;
   const RecipeScale rs{im.recipe_add, im.recipe_lo, im.recipe_hi, im.recipe_shift};
   const uint32_t m = div_magic((uint32_t)ngx);
+  RgbCtx c3{};
+  if constexpr (RECIPE == kRecipeRgb3) {
+    for (int k = 0; k < 3; k++) {
+      c3.row0[k] = out + im.plane_offset[im.planar ? k : 0];
+      c3.stride[k] = im.row_stride[im.planar ? k : 0];
+    }
+    const PlaneGeo &g = o.G[0];
+    c3.lw = (int)g.srw;
+    c3.luma = o.lds + o.soff[0] - g.soy * c3.lw - g.sox;
+    c3.crop_x = (int)im.crop_x;
+    c3.crop_y = (int)im.crop_y;
+    c3.planar = im.planar != 0;
+    c3.reverse = im.reverse != 0;
+    c3.upsample = im.upsample != 0;
+  }
   for (int t = threadIdx.x; t < ngx * ngy; t += blockDim.x) {
     const DivMod u = divmod(t, (uint32_t)ngx, m);
     if constexpr (RECIPE == kRecipeRgb3)
-      rgb3_group<STORE>(o, out, gx0 + u.r, gy0 + u.q, x0, x1, y0, y1, rs);
+      rgb3_group<STORE>(o, c3, gx0 + u.r, gy0 + u.q, x0, x1, y0, y1, rs);
     else
       luma1_group<STORE>(o, out, gx0 + u.r, gy0 + u.q, x0, x1, y0, y1, rs);
   }
