@@ -88,6 +88,7 @@ inline int max(int a, int b) { return a > b ? a : b; }
 typedef int hipError_t;
 #define hipSuccess 0
 #define hipErrorNotReady 600
+#define hipErrorOutOfMemory 2
 #define hipErrorInvalidValue 1
 typedef struct emu_stream *hipStream_t;
 typedef struct emu_event *hipEvent_t;

@@ -100,6 +100,23 @@ def main():
     # serving loop: every case in one mix, half into destinations, half into
     # server-allocated outputs, under a budget that admits a few at a time
     from jxr_amd import HipServer
+    from jxr_amd.prepare import RegionImage
+    # two region windows whose arenas have no identity: uploaded per request
+    # through the server's pinned staging
+    import copy
+    extra_names, extra_plans, extra_keep = [], [], []
+    for name, region in (("rgb8_444_ov2_hard", (32, 16, 64, 48)), ("rgb8_420_ov2_hard_1mb", (16, 16, 48, 32))):
+        enc, spec, _ = make(CASES[name])
+        reg = RegionImage(enc, region, cache=False)
+        sp = copy.copy(spec)
+        sp.crop = region
+        d = image_desc(reg, sp)
+        extra_names.append(name + "_region")
+        extra_plans.append(HipDecodePlan(d, keepalive=(reg, enc)))
+        wants.append(O.decode(d))
+        extra_keep.append(reg)
+    names = names + extra_names
+    plans = plans + extra_plans
     budget = max(p.output_bytes for p in plans) * 6 + (64 << 20)
     server = HipServer(session, hbm_budget_bytes=budget, max_wave_images=5, max_waves=2)
     dsts = {}
