@@ -121,8 +121,9 @@ constexpr uint32_t kPotBound = 1u << 23;
 // 2^27" matters.
 constexpr uint32_t kGainBoundInput = 1u << 20;
 constexpr uint32_t kPctGainShift = 6;
-// Likewise the sample-level overlap operators (gain under 100): inputs whose
-// magnitudes OR to below 2^18 give outputs below 128 x (that OR + 1).
+// Likewise the sample-level overlap operators (gain under 100): inputs of
+// magnitude at most B, below 2^18, give outputs below 128 B (the largest
+// magnitude is such a B, and so is an OR of magbits plus one).
 constexpr uint32_t kPotGainInput = 1u << 18;
 constexpr uint32_t kPotGainShift = 7;
 
@@ -130,6 +131,20 @@ constexpr uint32_t kPotGainShift = 7;
 // An upper bound of |v| (exclusive) for a set of values: OR of v ^ (v >> 31)
 // plus one.
 __device__ __forceinline__ uint32_t magbits(int v) { return (uint32_t)(v ^ (v >> 31)); }
+// The largest magnitude among v[F..N) (exact, also for INT32_MIN), from a
+// running maximum and minimum: three-operand max/min take two values an
+// instruction, against three instructions a value for an OR of magbits.
+// Below 2^k it puts every value in [-(2^k - 1), 2^k - 1].
+template <int N, int F = 0>
+__device__ __forceinline__ uint32_t max_abs(const int *v) {
+  int hi = v[F], lo = v[F];
+#pragma unroll
+  for (int j = F + 1; j < N; j++) {
+    hi = max(hi, v[j]);
+    lo = min(lo, v[j]);
+  }
+  return max((uint32_t)max(hi, 0), 0u - (uint32_t)min(lo, 0));
+}
 __device__ __forceinline__ uint32_t pot_out_mag(uint32_t in_mag, int a0, int a1, int a2, int a3) {
   return in_mag < kPotGainInput ? (in_mag + 1u) << kPotGainShift
                                 : magbits(a0) | magbits(a1) | magbits(a2) | magbits(a3);
@@ -442,9 +457,7 @@ __device__ __forceinline__ void first_level_mb(const DevImage &im, const DevPlan
   v[0] = dequant_checked(raw[0], r.qdc, ovf);
 #pragma unroll
   for (uint32_t k = 1; k < 16; k++) v[k] = (lp && k < blocks) ? dequant_checked(raw[k], r.qlp, ovf) : 0;
-  uint32_t mag = 0;
-#pragma unroll
-  for (int k = 0; k < 16; k++) mag |= magbits(v[k]);
+  const uint32_t mag = max_abs<16>(v);
   if (full) {
     ovf |= RUN_BOUNDED(mag, kPctBound, [&](auto &a) { inverse_core(v, a); });
   } else if (p.br == 4) {
@@ -485,9 +498,9 @@ __device__ __forceinline__ void first_level_mb(const DevImage &im, const DevPlan
 // 4-tap edge strips across the split; slots split along both are four 2x2
 // corners (reconstruct/overlap.rs:124-323; device_plan/overlap.rs:125-178).
 // Only positions inside [0, width) x [0, height) are touched.
-// Returns the OR of magbits of the values written or, for inputs whose
-// magnitudes OR to below 2^18, 128 times (that OR + 1): below 2^26 and above
-// every written value's magnitude.
+// Returns the OR of magbits of the values written or, for inputs below 2^18
+// in magnitude, 128 times a bound on them: below 2^26 and above every
+// written value's magnitude.
 __device__ __forceinline__ uint32_t overlap_slot(int *s, int rw, int ox, int oy, int kx, int ky, int width,
                                                  int height, bool sx, bool sy, uint32_t *status) {
   const int x0 = 4 * kx + 2, y0 = 4 * ky + 2;
@@ -496,7 +509,6 @@ __device__ __forceinline__ uint32_t overlap_slot(int *s, int rw, int ox, int oy,
   uint32_t omag = 0;
   if (!sx && !sy) {
     int v[16];
-    uint32_t mag = 0;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int2 lo = *reinterpret_cast<const int2 *>(base + r * rw);
@@ -506,11 +518,10 @@ __device__ __forceinline__ uint32_t overlap_slot(int *s, int rw, int ox, int oy,
       v[r * 4 + 2] = hi.x;
       v[r * 4 + 3] = hi.y;
     }
-#pragma unroll
-    for (int k = 0; k < 16; k++) mag |= magbits(v[k]);
+    const uint32_t mag = max_abs<16>(v);  // every |input| <= mag
     ovf = RUN_BOUNDED(mag, kPotBound, [&](auto &a) { inv_overlap4x4(v, a); });
     if (mag < kPotGainInput) {
-      omag = (mag + 1u) << kPotGainShift;
+      omag = mag << kPotGainShift;
     } else {
 #pragma unroll
       for (int k = 0; k < 16; k++) omag |= magbits(v[k]);
@@ -766,11 +777,9 @@ __device__ __forceinline__ uint32_t second_level_block(const DevImage &im, const
         povf |= __builtin_add_overflow(v[3], s2, &v[3]);
       }
     }
-    uint32_t mag = 0;
-#pragma unroll
-    for (int j = 1; j < 16; j++) mag |= magbits(v[j]);
-    // |residual| <= mag + 1, so every dequantized value is within the product
-    const unsigned long long prod = (mag + 1ull) * r.qhp;
+    // every dequantized value is within the largest residual magnitude
+    // times the step
+    const unsigned long long prod = (unsigned long long)max_abs<16, 1>(v) * r.qhp;
     if (prod <= 0x7fffffffull) {
 #pragma unroll
       for (int j = 1; j < 16; j++) v[j] *= (int)r.qhp;
