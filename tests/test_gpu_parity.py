@@ -159,3 +159,17 @@ def test_large_samples_take_checked_output_path(session, index):
         assert err.value.code == abi.ERR_KERNEL_ARITHMETIC and err.value.stage == stage, name
     else:
         assert np.array_equal(session.decode_to_host(plan), O.decode(desc)), name
+
+
+@pytest.mark.parametrize("color", [abi.COLOR_YUV444, abi.COLOR_YUV420, abi.COLOR_YUV422])
+@pytest.mark.parametrize("ov", [0, 1, 2])
+def test_soft_tiles_do_not_change_samples_on_gpu(session, color, ov):
+    """The same image with and without a soft tile grid decodes to the same
+    bytes on the GPU, and to the oracle's."""
+    base = dict(w=176, h=144, ov=ov, quality=70, color=color)
+    e1, s1, _ = make(base)
+    e2, s2, _ = make(dict(base, tiles=(3, 2)))
+    a = session.decode_to_host(HipDecodePlan.from_encoded(e1, s1))
+    b = session.decode_to_host(HipDecodePlan.from_encoded(e2, s2))
+    assert np.array_equal(a, b)
+    assert np.array_equal(a, _expected(e1, s1))
