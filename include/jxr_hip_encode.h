@@ -48,6 +48,10 @@ typedef struct jxrhip_encode_params {
   uint32_t alpha_qp;       /* QP of the alpha plane (all bands)               */
   uint32_t seed;           /* varies quantizer sets between macroblocks       */
   uint32_t vary_quantizers; /* nonzero: per-tile and per-macroblock steps vary */
+  uint32_t chroma_qp_set;  /* nonzero: components 1.. take the qp_chroma_* QPs */
+  uint32_t qp_chroma_dc;
+  uint32_t qp_chroma_lp;
+  uint32_t qp_chroma_hp;
 } jxrhip_encode_params;
 
 typedef struct jxrhip_encoded jxrhip_encoded;
@@ -61,6 +65,19 @@ const jxrhip_arena *jxrhip_encoded_arena(const jxrhip_encoded *encoded, uint32_t
 /* Quantization step of a coded QP (T.832 QP mapping,
  * crates/jxr-native/src/tile_decode/quantizer.rs:309-331). */
 uint32_t jxrhip_quant_step(uint32_t qp, uint32_t scaled, uint32_t scaled_shift);
+/* Quantization parameters (coded QPs, 0 = lossless) and overlap mode that a
+ * quality 0..100 resolves to for an integer source whose samples span
+ * `range` values (255 for 8-bit, 65535 for 16-bit, 31 and 63 for RGB555 and
+ * RGB565 fields), as the reference encoder resolves a quality request
+ * (crates/jxr-native/src/encode/quality.rs:101-480): `luma` receives DC, LP
+ * and HP parameters of component 0, `chroma` those of components 1 and 2,
+ * which are colour differences at color_format's chroma sampling when
+ * `rgb_source` is nonzero and take the luma parameters otherwise; *overlap
+ * receives the mode the quality picks when a request leaves it open. Float
+ * sources are not covered. */
+jxrhip_status jxrhip_quality_parameters(uint32_t quality, uint32_t color_format, uint32_t rgb_source,
+                                        uint32_t range, uint32_t luma[3], uint32_t chroma[3],
+                                        uint32_t *overlap);
 /* The reconstructed sample whose f32 output is nearest `bits`
  * (crates/jxr-native/src/output_format/scaling.rs:177-229). */
 jxrhip_status jxrhip_float32_sample(uint32_t bits, uint32_t scaled, uint32_t mantissa_length,

@@ -17,13 +17,13 @@ from .decoder import (
     ResidentHipImage,
     image_desc,
     plan_hip_route,
-    quality_qp,
+    quality_parameters,
     surface_layout,
 )
 from .serving import HipServer, ServerResult
 
 __all__ = [
     "abi", "JxrHipError", "load", "EncodedImage", "HipBatch", "HipDecodePlan", "HipDecoderSession",
-    "HipSubmission", "OutputSpec", "ResidentHipImage", "image_desc", "plan_hip_route", "quality_qp",
+    "HipSubmission", "OutputSpec", "ResidentHipImage", "image_desc", "plan_hip_route", "quality_parameters",
     "surface_layout", "HipServer", "ServerResult",
 ]

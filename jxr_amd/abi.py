@@ -255,6 +255,10 @@ class EncodeParams(C.Structure):
         ("alpha_qp", C.c_uint32),
         ("seed", C.c_uint32),
         ("vary_quantizers", C.c_uint32),
+        ("chroma_qp_set", C.c_uint32),
+        ("qp_chroma_dc", C.c_uint32),
+        ("qp_chroma_lp", C.c_uint32),
+        ("qp_chroma_hp", C.c_uint32),
     ]
 
 
@@ -324,6 +328,8 @@ SIGNATURES = {
     "jxrhip_encoded_destroy": (None, [vp]),
     "jxrhip_encoded_arena": (C.POINTER(Arena), [vp, C.c_uint32]),
     "jxrhip_quant_step": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
+    "jxrhip_quality_parameters": (st, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p, u32p,
+                                      C.POINTER(C.c_uint32)]),
     "jxrhip_float32_sample": (st, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_int32)]),
 }
 

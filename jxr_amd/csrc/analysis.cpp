@@ -550,6 +550,137 @@ extern "C" uint32_t jxrhip_quant_step(uint32_t qp, uint32_t scaled, uint32_t sca
   return quant_map(qp > 255 ? 255 : qp, scaled != 0, scaled_shift);
 }
 
+// ---- quality scale (crates/jxr-native/src/encode/quality.rs) --------------
+// A quality names a fidelity calibrated on 8-bit pictures: a level in half
+// parameters of luma, split between luma (rounded up) and the colour
+// differences (rounded down), with band offsets, a chroma offset by
+// sampling, and for other sample ranges the parameter whose step is as
+// large against the range as the calibrated step is against 255.
+namespace {
+
+constexpr uint32_t kReferenceRange = 255;  // quality.rs:27
+constexpr uint32_t kOneSample = 4;         // step of one sample value (quality.rs:32)
+constexpr uint32_t kFinestLossy = 5;       // quality.rs:35
+constexpr uint32_t kWholeStepsBelow = 32;  // quality.rs:38
+
+// Step of the first component's high-pass band at `parameter` in scaled
+// arithmetic: only ratios of steps are compared (quality.rs:387-392).
+unsigned __int128 scale_step(uint32_t parameter) { return quant_map(parameter, true, 1); }
+
+// Whether the next step up is farther from `target` than `nearest`, i.e.
+// target lies below their geometric mean (quality.rs:382-384).
+bool past_target(unsigned __int128 candidate, unsigned __int128 nearest, unsigned __int128 target) {
+  return candidate * nearest > target * target;
+}
+
+// Level of a lossy quality in half luma parameters (quality.rs:416-425).
+uint32_t quality_level(uint32_t q) {
+  if (q >= 99) return 10;
+  if (q >= 93) return 16 + 8 * (98 - q);
+  if (q >= 91) return 64 + 3 * (92 - q);
+  if (q >= 76) return 70 + 2 * (90 - q);
+  if (q >= 11) return 100 + (75 - q);
+  return 165 + 6 * (10 - q);
+}
+
+// Parameter of a colour difference whose channels take `p`: sixteen more
+// at full resolution, eight at 4:2:2, four at 4:2:0, in proportion below
+// sixteen (quality.rs:448-461).
+uint32_t difference_parameter(uint32_t p, uint32_t color_format) {
+  uint32_t offset = color_format == JXRHIP_COLOR_YUV420 ? 4 : color_format == JXRHIP_COLOR_YUV422 ? 8 : 16;
+  if (p < 16) offset = p * offset / 16;
+  return std::min<uint32_t>(p + offset, 255);
+}
+
+// DC keeps half the excess of the level over 64, at most sixteen
+// parameters, below the AC parameter (quality.rs:470-479).
+void band_parameters(uint32_t level, uint32_t parameter, uint32_t out[3]) {
+  const uint32_t excess = (level > 64 ? level - 64 : 0) / 2;
+  out[0] = parameter - std::min<uint32_t>(excess, 16);
+  out[1] = out[2] = parameter;
+}
+
+// The parameter whose step is nearest range / 255 times the step of
+// `reference`, for a range of at least 255 (quality.rs:369-380).
+uint32_t scaled_parameter(uint32_t reference, uint32_t range) {
+  const unsigned __int128 target = scale_step(reference) * range;
+  uint32_t nearest = reference;
+  unsigned __int128 nearest_step = scale_step(reference);
+  for (uint32_t p = reference; p <= 255; p++) {
+    const unsigned __int128 c = scale_step(p);
+    if (past_target(c * kReferenceRange, nearest_step * kReferenceRange, target)) break;
+    nearest = p;
+    nearest_step = c;
+  }
+  return nearest;
+}
+
+// A range below 255: 5 and the multiples of four below 32, any parameter
+// from 32; a step down to one sample value is lossless, and every band and
+// component follows the primary parameter (quality.rs:340-363).
+uint32_t reduced_parameter(uint32_t primary, uint32_t range) {
+  const unsigned __int128 target = scale_step(primary) * range;
+  uint32_t nearest = kOneSample;
+  unsigned __int128 nearest_step = scale_step(kOneSample);
+  for (uint32_t p = kFinestLossy; p <= 255; p++) {
+    if (!(p == kFinestLossy || p >= kWholeStepsBelow || p % 4 == 0)) continue;
+    const unsigned __int128 c = scale_step(p);
+    if (past_target(c * kReferenceRange, nearest_step * kReferenceRange, target)) break;
+    nearest = p;
+    nearest_step = c;
+  }
+  return nearest == kOneSample ? 0u : nearest;
+}
+
+// Overlap mode of a lossy quality: none at 99, second level at 6 and below
+// (quality.rs:493-499).
+uint32_t quality_overlap(uint32_t q) { return q >= 99 ? 0u : q >= 7 ? 1u : 2u; }
+
+}  // namespace
+
+extern "C" jxrhip_status jxrhip_quality_parameters(uint32_t quality, uint32_t color_format, uint32_t rgb_source,
+                                                   uint32_t range, uint32_t luma[3], uint32_t chroma[3],
+                                                   uint32_t *overlap) {
+  if (!luma || !chroma || !overlap) {
+    jxrhip_set_error("null argument");
+    return JXRHIP_ERR_INVALID_ARGUMENT;
+  }
+  if (quality > 100 || range == 0) {
+    jxrhip_set_error("quality above 100 or empty sample range");
+    return JXRHIP_ERR_INVALID_ARGUMENT;
+  }
+  const bool difference = rgb_source && color_format != JXRHIP_COLOR_LUMA;
+  if (quality >= 100) {  // lossless, and no overlap unless asked for (quality.rs:127-128,300-302)
+    for (int b = 0; b < 3; b++) luma[b] = chroma[b] = 0;
+    *overlap = 0;
+    return JXRHIP_OK;
+  }
+  const uint32_t level = quality_level(quality);
+  const uint32_t primary = (level + 1) / 2;
+  if (range < kReferenceRange) {
+    const uint32_t p = reduced_parameter(primary, range);
+    for (int b = 0; b < 3; b++) {
+      luma[b] = p;
+      chroma[b] = p && difference ? difference_parameter(p, color_format) : p;
+    }
+    // lossless coding takes no overlap (quality.rs:126-128)
+    *overlap = p == 0 ? 0u : quality_overlap(quality);
+    return JXRHIP_OK;
+  }
+  const uint32_t diff = level / 2;
+  band_parameters(primary, primary, luma);
+  if (difference)
+    band_parameters(diff, difference_parameter(diff, color_format), chroma);
+  else
+    band_parameters(primary, primary, chroma);
+  for (int b = 0; b < 3; b++) {
+    luma[b] = scaled_parameter(luma[b], range);
+    chroma[b] = scaled_parameter(chroma[b], range);
+  }
+  *overlap = quality_overlap(quality);
+  return JXRHIP_OK;
+}
+
 extern "C" jxrhip_status jxrhip_float32_sample(uint32_t bits, uint32_t scaled, uint32_t length,
                                                int32_t exponent_bias, int32_t *sample) {
   if (length > 23 || !sample) {
@@ -725,9 +856,11 @@ extern "C" jxrhip_status jxrhip_encode(const void *pixels, uint64_t pixel_bytes,
   ArenaOut &P = enc->primary;
   P.tile_w = tcol;
   P.tile_h = trow;
-  for (uint32_t c = 0; c < comps.size(); c++)
-    analyze_plane(comps[c], c, *prm, prm->scaled && c != 0, prm->qp_dc, prm->qp_lp, prm->qp_hp,
-                  tcol, trow, P);
+  for (uint32_t c = 0; c < comps.size(); c++) {
+    const bool chroma = c != 0 && prm->chroma_qp_set;
+    analyze_plane(comps[c], c, *prm, prm->scaled && c != 0, chroma ? prm->qp_chroma_dc : prm->qp_dc,
+                  chroma ? prm->qp_chroma_lp : prm->qp_lp, chroma ? prm->qp_chroma_hp : prm->qp_hp, tcol, trow, P);
+  }
   if (has_alpha && prm->alpha_mode == JXRHIP_ALPHA_INTEGRATED)
     analyze_plane(alpha_plane, 0, *prm, false, prm->alpha_qp, prm->alpha_qp, prm->alpha_qp, tcol,
                   trow, P);

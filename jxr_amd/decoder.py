@@ -23,11 +23,17 @@ from .abi import JxrHipError, check, load
 # ---------------------------------------------------------------------------
 # encoded images (forward analysis into arenas)
 # ---------------------------------------------------------------------------
-def quality_qp(quality: int) -> int:
-    """Coded QP of a 0..100 quality: 100 is lossless (step 1)."""
-    if quality >= 100:
-        return 0
-    return int(round(4 + (100 - quality) * 0.8))
+def quality_parameters(quality: int, color: int, rgb_source: bool, sample_range: int):
+    """The reference encoder's quality scale (crates/jxr-native/src/encode/
+    quality.rs:101-499): ((DC, LP, HP) QPs of luma, (DC, LP, HP) QPs of
+    components 1 and 2, overlap mode the quality picks) for an integer
+    source whose samples span `sample_range` values (255 for 8-bit, 65535
+    for 16-bit). Components 1 and 2 of an RGB source are colour differences
+    at the chroma sampling of `color`. QP 0 is lossless."""
+    luma, chroma, ov = (C.c_uint32 * 3)(), (C.c_uint32 * 3)(), C.c_uint32()
+    check(load().jxrhip_quality_parameters(int(quality), int(color), 1 if rgb_source else 0, int(sample_range),
+                                           luma, chroma, C.byref(ov)), "quality parameters")
+    return tuple(luma), tuple(chroma), int(ov.value)
 
 
 class EncodedImage:
@@ -37,7 +43,7 @@ class EncodedImage:
                  hard_tiles: bool = False, scaled: bool = False, qp=(0, 0, 0),
                  bands: int = abi.BANDS_ALL, alpha_mode: int = abi.ALPHA_NONE,
                  shift_bits: int = 0, mantissa_length: int = 13, exponent_bias: int = 4,
-                 alpha_qp: int = 0, seed: int = 0, vary_quantizers: bool = False):
+                 alpha_qp: int = 0, seed: int = 0, vary_quantizers: bool = False, chroma_qp=None):
         lib = load()
         pixels = np.ascontiguousarray(pixels)
         if pixels.ndim == 2:
@@ -62,6 +68,9 @@ class EncodedImage:
         p.bands = bands
         p.shift_bits, p.mantissa_length, p.exponent_bias = shift_bits, mantissa_length, exponent_bias
         p.alpha_qp, p.seed, p.vary_quantizers = alpha_qp, seed, 1 if vary_quantizers else 0
+        if chroma_qp is not None:
+            p.chroma_qp_set = 1
+            p.qp_chroma_dc, p.qp_chroma_lp, p.qp_chroma_hp = chroma_qp
         handle = C.c_void_p()
         check(lib.jxrhip_encode(pixels.ctypes.data, pixels.nbytes, C.byref(p), C.byref(handle)), "encode")
         self._handle = handle
@@ -73,6 +82,18 @@ class EncodedImage:
         self.input_dtype = pixels.dtype
         self.shift_bits, self.mantissa_length, self.exponent_bias = shift_bits, mantissa_length, exponent_bias
         self.overlap = overlap
+
+    @classmethod
+    def at_quality(cls, pixels: np.ndarray, quality: int, *, color: int, overlap=None, **kw):
+        """Analysis at a quality of the reference encoder's scale: its
+        parameters for the source's sample range, scaled arithmetic unless
+        every component is lossless, and the quality's overlap mode unless
+        `overlap` is given (crates/jxr-native/src/encode/quality.rs:101-150)."""
+        sample_range = {np.dtype(np.uint8): 255, np.dtype(np.uint16): 65535}[np.asarray(pixels).dtype]
+        rgb = pixels.ndim == 3 and pixels.shape[2] >= 3
+        luma, chroma, ov = quality_parameters(quality, color, rgb, sample_range)
+        return cls(pixels, color=color, overlap=ov if overlap is None else overlap,
+                   scaled=any(luma) or any(chroma), qp=luma, chroma_qp=chroma, **kw)
 
     @property
     def primary(self):

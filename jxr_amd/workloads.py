@@ -2,7 +2,8 @@
 
   c1  512x512 RGB u8, lossless 4:4:4, one tile, overlap 1 -> interleaved RGB u8
   c2  4096x4096 gray u16 pink noise, 8x8 grid of 512-px soft tiles, overlap 2,
-      quality 75 with all bands (flexbits kept) -> gray u16
+      quality 75 (uniform DC/LP/HP parameters) with all bands (flexbits kept)
+      -> gray u16
   c3  256 RGB images, sides uniform in 128..2048 step 16, 8/16-bit at 50/50,
       4:2:0, overlap 1 or 2, quality 50..100 -> device-resident NCHW
   c4  128 x 1024x1024 RGBA f32 lognormal HDR + uniform alpha, lossless,
@@ -11,8 +12,9 @@
       q75, 20% 4096x4096 gray u16 8x8 tiles, 10% 512x512 regions of an
       8192x8192 RGB u8 source coded in 512-px tiles (c5_pool, c5_schedule)
 
-Entropy coding is not modelled: the forward analysis produces the arenas the
-entropy decoder would hand the reconstruction.
+Qualities resolve to quantizers by the reference encoder's scale
+(`quality_parameters`). Entropy coding is not modelled: the forward analysis
+produces the arenas the entropy decoder would hand the reconstruction.
 """
 import os
 from concurrent.futures import ThreadPoolExecutor
@@ -20,7 +22,7 @@ from concurrent.futures import ThreadPoolExecutor
 import numpy as np
 
 from . import abi, synth
-from .decoder import EncodedImage, OutputSpec, quality_qp
+from .decoder import EncodedImage, OutputSpec
 
 SEED = 0x4A585231
 
@@ -45,9 +47,8 @@ def c1():
 
 def c2():
     px = synth.pink_noise(4096, 4096, SEED)
-    qp = quality_qp(75)
-    enc = EncodedImage(px, color=abi.COLOR_LUMA, overlap=2, tiles=([32] * 8, [32] * 8), hard_tiles=False,
-                       scaled=True, qp=(qp, qp, qp + 4), bands=abi.BANDS_ALL)
+    enc = EncodedImage.at_quality(px, 75, color=abi.COLOR_LUMA, overlap=2, tiles=([32] * 8, [32] * 8),
+                                  hard_tiles=False, bands=abi.BANDS_ALL)
     spec = OutputSpec(storage=abi.STORE_U16, depth=abi.DEPTH_U16, layout=abi.LAYOUT_LUMA,
                       output_color=abi.COLOR_LUMA)
     return [(enc, spec)]
@@ -68,10 +69,7 @@ def c3_params(count=256, seed=SEED):
 
 def _c3_one(p):
     px = synth.gradient_noise(p["w"], p["h"], p["seed"], depth=p["depth"], sigma=8.0)
-    q = p["quality"]
-    qp = quality_qp(q)
-    enc = EncodedImage(px, color=abi.COLOR_YUV420, overlap=p["ov"], scaled=q < 100,
-                       qp=(qp, qp, qp + (4 if q < 100 else 0)), seed=p["seed"])
+    enc = EncodedImage.at_quality(px, p["quality"], color=abi.COLOR_YUV420, overlap=p["ov"], seed=p["seed"])
     return enc, rgb_out(p["depth"], True)
 
 
@@ -102,16 +100,14 @@ C5_MIX = (0.7, 0.2, 0.1)  # small RGB u8, 4096x4096 gray u16, 512x512 region of 
 
 def _c5_small(p):
     px = synth.gradient_noise(p["w"], p["h"], p["seed"], depth=8, sigma=8.0)
-    qp = quality_qp(75)
-    enc = EncodedImage(px, color=abi.COLOR_YUV444, overlap=1, scaled=True, qp=(qp, qp, qp + 4), seed=p["seed"])
+    enc = EncodedImage.at_quality(px, 75, color=abi.COLOR_YUV444, seed=p["seed"])
     return enc, rgb_out(8, False)
 
 
 def _c5_gray(seed):
     px = synth.pink_noise(4096, 4096, seed)
-    qp = quality_qp(75)
-    enc = EncodedImage(px, color=abi.COLOR_LUMA, overlap=2, tiles=([32] * 8, [32] * 8), hard_tiles=False,
-                       scaled=True, qp=(qp, qp, qp + 4), bands=abi.BANDS_ALL)
+    enc = EncodedImage.at_quality(px, 75, color=abi.COLOR_LUMA, overlap=2, tiles=([32] * 8, [32] * 8),
+                                  hard_tiles=False, bands=abi.BANDS_ALL)
     return enc, OutputSpec(storage=abi.STORE_U16, depth=abi.DEPTH_U16, layout=abi.LAYOUT_LUMA,
                            output_color=abi.COLOR_LUMA)
 
@@ -134,9 +130,8 @@ def c5_pool(small=96, gray=2, regions=64, seed=SEED + 5000, cache=True):
         smalls = list(ex.map(_c5_small, ps))
         grays = list(ex.map(_c5_gray, [seed + 1000 + i for i in range(gray)]))
     src_px = synth.gradient_noise_large(8192, 8192, seed + 2000)
-    qp = quality_qp(75)
-    source = EncodedImage(src_px, color=abi.COLOR_YUV444, overlap=1, tiles=([32] * 16, [32] * 16),
-                          hard_tiles=False, scaled=True, qp=(qp, qp, qp + 4))
+    source = EncodedImage.at_quality(src_px, 75, color=abi.COLOR_YUV444, tiles=([32] * 16, [32] * 16),
+                                     hard_tiles=False)
     del src_px
     regs = []
     for _ in range(regions):

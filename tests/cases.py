@@ -3,7 +3,16 @@ forward analysis and an output policy, at sizes the oracle decodes in
 milliseconds."""
 import numpy as np
 
-from jxr_amd import EncodedImage, OutputSpec, abi, quality_qp, synth
+from jxr_amd import EncodedImage, OutputSpec, abi, synth
+
+
+def quality_qp(quality: int) -> int:
+    """A QP ramp for the parity cases, which exercise a spread of steps: 100
+    is lossless (step 1), lower qualities coarser. (The benchmark workloads
+    use the reference encoder's scale, jxr_amd.quality_parameters.)"""
+    if quality >= 100:
+        return 0
+    return int(round(4 + (100 - quality) * 0.8))
 
 RGB8 = dict(storage=abi.STORE_U8, depth=abi.DEPTH_U8, layout=abi.LAYOUT_RGB, output_color=abi.COLOR_RGB)
 RGB16 = dict(storage=abi.STORE_U16, depth=abi.DEPTH_U16, layout=abi.LAYOUT_RGB, output_color=abi.COLOR_RGB)

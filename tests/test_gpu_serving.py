@@ -9,9 +9,9 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from cases import CASES, make
+from cases import CASES, make, quality_qp
 from jxr_amd import HipDecodePlan, HipDecoderSession, HipServer, abi, image_desc, synth, workloads
-from jxr_amd.decoder import EncodedImage, quality_qp
+from jxr_amd.decoder import EncodedImage
 from jxr_amd.prepare import RegionImage
 
 pytestmark = pytest.mark.gpu

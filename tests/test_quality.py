@@ -1,0 +1,138 @@
+"""The encoder's quality scale against the reference encoder's own known
+answers and properties (crates/jxr-native/src/encode/quality.rs:620-912):
+the calibrated anchors of 8-bit RGB, the chroma offsets by sampling, the
+shift of sixteen parameters per doubling of the sample range, the whole
+steps of ranges below eight bits, and a scale that never quantizes more
+finely as the quality falls."""
+import pytest
+
+from jxr_amd import abi, quality_parameters
+from jxr_amd.abi import load
+
+Q = quality_parameters
+RGB444, RGB422, RGB420, LUMA = abi.COLOR_YUV444, abi.COLOR_YUV422, abi.COLOR_YUV420, abi.COLOR_LUMA
+
+
+def step(p):
+    """Step of component 0's high-pass band in scaled arithmetic."""
+    return load().jxrhip_quant_step(p, 1, 1)
+
+
+def rgb8(q, color=RGB444):
+    luma, chroma, ov = Q(q, color, True, 255)
+    return list(luma) + list(chroma), ov
+
+
+@pytest.mark.parametrize("quality,params,overlap", [
+    (99, [5, 5, 5, 10, 10, 10], 0),
+    (90, [35, 35, 35, 51, 51, 51], 1),
+    (75, [50, 50, 50, 66, 66, 66], 1),
+    (50, [63, 63, 63, 78, 78, 78], 1),
+    (25, [70, 75, 75, 86, 91, 91], 1),
+    (0, [97, 113, 113, 112, 128, 128], 2),
+])
+def test_calibrated_anchors(quality, params, overlap):
+    """quality.rs:645-666"""
+    assert rgb8(quality) == (params, overlap)
+
+
+def test_quality_100_is_lossless_without_overlap():
+    assert rgb8(100) == ([0] * 6, 0)
+    assert Q(100, RGB420, True, 65535) == ((0, 0, 0), (0, 0, 0), 0)
+
+
+@pytest.mark.parametrize("color,expected", [(RGB444, [10, 32, 66]), (RGB422, [7, 24, 58]), (RGB420, [6, 20, 54])])
+def test_chroma_offsets_by_sampling(color, expected):
+    """difference_parameter of 5, 16 and 50 (quality.rs:771-783), reached
+    as the chroma of qualities 99, 96 and 75, whose difference levels are
+    those parameters with no DC relief."""
+    assert [Q(q, color, True, 255)[1][2] for q in (99, 96, 75)] == expected
+
+
+def test_gray_and_yuv_components_take_the_primary_parameters():
+    """quality.rs:680-688: luma, gray and components coded as stored share
+    the primary's parameters."""
+    for q in (99, 75, 40, 3):
+        luma, _, _ = Q(q, RGB444, True, 255)
+        g_luma, g_chroma, _ = Q(q, LUMA, False, 255)
+        assert g_luma == luma and g_chroma == luma
+        y_luma, y_chroma, _ = Q(q, RGB420, False, 255)
+        assert y_luma == luma and y_chroma == luma
+
+
+def test_sixteen_bit_is_the_eight_bit_scale_eight_octaves_up():
+    """quality.rs:691-731: sixteen parameters double a step from 16 up; the
+    table's linear part below 16 puts 5 at 116; it ends at 255."""
+    assert Q(99, RGB444, True, 65535)[0] == (116, 116, 116)
+    for q in (96, 90, 75, 40, 10):
+        eight, sixteen = rgb8(q)[0], Q(q, RGB444, True, 65535)
+        assert list(sixteen[0]) + list(sixteen[1]) == [p + 128 for p in eight]
+        luma10 = Q(q, LUMA, False, 1023)[0]
+        assert list(luma10) == [p + 32 for p in eight[:3]]
+    # chroma parameter 128 at quality 0 reaches the end of the table
+    assert Q(0, RGB444, True, 65535)[1] == (240, 255, 255)
+
+
+@pytest.mark.parametrize("sample_range", [511, 1023, 4095, 8192, 16384, 65535])
+def test_deeper_ranges_land_within_the_table_spacing(sample_range):
+    """quality.rs:700-711: the parameter found for a range is within one
+    part in 32 of range/255 times the calibrated step (checked for every
+    parameter the scale gives 8-bit luma and chroma below 114)."""
+    for q in range(0, 100):
+        eight = rgb8(q)[0]
+        luma, chroma, _ = Q(q, RGB444, True, sample_range)
+        for ref, found in zip(eight, list(luma) + list(chroma)):
+            if ref > 113:
+                continue
+            exact = step(ref) * sample_range
+            got = step(found) * 255
+            assert abs(got - exact) * 32 <= exact, (q, sample_range, ref, found)
+
+
+def test_never_finer_as_the_quality_falls():
+    """quality.rs:549-575: no band gets finer one quality down, neighbours
+    differ, overlap only strengthens; DC is never the coarsest band."""
+    for q in range(0, 100):
+        coarse, ov_c = rgb8(q)
+        fine, ov_f = rgb8(q + 1)
+        assert all(c >= f for c, f in zip(coarse, fine)), q
+        assert (coarse, ov_c) != (fine, ov_f), q
+        assert ov_c >= ov_f
+        for color in (RGB444, RGB420):
+            for rng in (255, 65535, 63, 31):
+                lc, cc, _ = Q(q, color, True, rng)
+                lf, cf, _ = Q(q + 1, color, True, rng)
+                for a, b in ((lc, lf), (cc, cf)):
+                    assert all(x >= y for x, y in zip(a, b)), (q, color, rng)
+                    assert a[0] <= a[1] <= a[2]
+    # integer differences move on the qualities between the primary's steps
+    assert Q(41, RGB444, True, 65535) != Q(42, RGB444, True, 65535)
+
+
+@pytest.mark.parametrize("sample_range", [31, 63])
+def test_ranges_below_eight_bits_keep_to_whole_steps(sample_range):
+    """quality.rs:884-912 (RGB555 and RGB565 fields): one parameter for every
+    band from 5, the multiples of four below 32 and any from 32; chroma at
+    the full-resolution difference of it; the top of the scale lossless."""
+    lossless = 0
+    for q in range(0, 100):
+        luma, chroma, ov = Q(q, RGB444, True, sample_range)
+        if luma == (0, 0, 0):
+            lossless += 1
+            assert chroma == (0, 0, 0) and ov == 0
+            continue
+        p = luma[2]
+        assert luma == (p, p, p)
+        assert p == 5 or p >= 32 or p % 4 == 0
+        d = p + (16 if p >= 16 else p)  # the 4:4:4 difference parameter
+        assert chroma == (min(d, 255),) * 3
+    assert 3 <= lossless < 40
+    assert Q(99, RGB444, True, sample_range)[0] == (0, 0, 0)
+    assert Q(50, RGB444, True, sample_range)[0] != (0, 0, 0)
+
+
+def test_invalid_requests_are_refused():
+    with pytest.raises(abi.JxrHipError):
+        Q(101, RGB444, True, 255)
+    with pytest.raises(abi.JxrHipError):
+        Q(50, RGB444, True, 0)
