@@ -29,18 +29,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(workload):
-    """HBM bytes per launch of the reconstruction kernel for `workload`, from
-    the committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, each
-    scaled by what the counter reads for a 256 MiB device copy), or None."""
+def profile_figure(name, workload):
+    """A per-launch figure of the reconstruction kernel for `workload` from the
+    committed rocprofv3 counter summaries (a line `<name> <workload> <value>`),
+    or None."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_counters.md"))):
         with open(path) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == workload:
-                    return int(parts[2])
+                if len(parts) == 3 and parts[0] == name and parts[1] == workload:
+                    return float(parts[2])
     return None
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch (FETCH_SIZE and WRITE_SIZE, each scaled by what the
+    counter reads for a 256 MiB device copy), or None."""
+    v = profile_figure("traffic_bytes_per_launch", workload)
+    return None if v is None else int(v)
 
 
 def percentile(xs, p):
@@ -299,6 +306,20 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
                      "kernel": "jxr_reconstruct_tiles", "avg_launch_ms": round(per_launch_ms, 4)},
     }
+    valu = profile_figure("valu_instructions_per_launch", args.workload)
+    if valu is not None:
+        # the kernel's VALU instruction rate against the rate a plain chain
+        # of independent integer VALU instructions reaches on this device
+        rate = session.calibrate_valu()
+        cycles = rate["simds"] * rate["clock_khz"] * 1e3
+        peak = rate["wave_instructions"] / (rate["ms"] / 1e3) / cycles
+        got = valu / (per_launch_ms / 1e3) / cycles
+        out["valu_issue"] = {"achieved": round(got, 4), "peak": round(peak, 4),
+                             "unit": "wave VALU instructions per SIMD cycle", "frac": round(got / peak, 4),
+                             "instructions_per_launch": valu, "kernel": "jxr_reconstruct_tiles",
+                             "peak_from": "jxrhip_calibrate_valu (16 independent integer VALU ops per "
+                                          "iteration, 4 workgroups per CU)"}
+        log(f"[bench] VALU issue: {got:.4f} of {peak:.4f} wave instructions per SIMD cycle ({got / peak:.1%})")
     if per_image is not None:
         pi_ms = per_image["total_ms"] / len(per_image["wall_ms"])
         pi_value = pixels / (pi_ms / 1e3) / 1e6

@@ -343,6 +343,21 @@ jxrhip_status jxrhip_copy_device(jxrhip_session *session, void *dst, const void 
 /* Waits for all work queued on the session's streams. */
 jxrhip_status jxrhip_session_synchronize(jxrhip_session *session);
 
+/* Issue rate of the vector ALU, the reference the reconstruction kernel's
+ * VALU instruction rate is read against (no counterpart in the reference:
+ * measurement support for the benchmark). Runs `iterations` rounds of
+ * sixteen independent integer VALU instructions per wave at four 256-thread
+ * workgroups per compute unit on the session's first stream and reports the
+ * timed launch. wave_instructions / (simds * ms * clock) is the peak rate in
+ * wave instructions per SIMD cycle. */
+typedef struct jxrhip_valu_rate {
+  double ms;                 /* device time of the timed launch           */
+  double wave_instructions;  /* VALU instructions it issued, per wave      */
+  uint32_t simds;            /* SIMDs of the device (four per compute unit) */
+  uint32_t clock_khz;        /* the device's reported peak shader clock    */
+} jxrhip_valu_rate;
+jxrhip_status jxrhip_calibrate_valu(jxrhip_session *session, uint32_t iterations, jxrhip_valu_rate *out);
+
 void *jxrhip_image_device_ptr(const jxrhip_image *image);
 uint64_t jxrhip_image_bytes(const jxrhip_image *image);
 void jxrhip_image_destroy(jxrhip_image *image);

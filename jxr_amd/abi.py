@@ -230,6 +230,15 @@ class ServerStats(C.Structure):
     ]
 
 
+class ValuRate(C.Structure):
+    _fields_ = [
+        ("ms", C.c_double),
+        ("wave_instructions", C.c_double),
+        ("simds", C.c_uint32),
+        ("clock_khz", C.c_uint32),
+    ]
+
+
 class EncodeParams(C.Structure):
     _fields_ = [
         ("width", C.c_uint32),
@@ -328,6 +337,7 @@ SIGNATURES = {
     "jxrhip_encoded_destroy": (None, [vp]),
     "jxrhip_encoded_arena": (C.POINTER(Arena), [vp, C.c_uint32]),
     "jxrhip_quant_step": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
+    "jxrhip_calibrate_valu": (st, [vp, C.c_uint32, C.POINTER(ValuRate)]),
     "jxrhip_quality_parameters": (st, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p, u32p,
                                       C.POINTER(C.c_uint32)]),
     "jxrhip_float32_sample": (st, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_int32)]),

@@ -2187,4 +2187,39 @@ hipError_t launch_clear_rows(uint8_t *base, uint64_t pitch, uint32_t row_bytes, 
   return hipGetLastError();
 }
 
+// Issue-rate reference for the vector ALU: eight independent add/xor chains
+// per lane, written in assembly so that every iteration is exactly sixteen
+// VALU instructions whatever the compiler does with the loop around them.
+// The reconstruction kernel's VALU instruction rate is read against it.
+constexpr uint32_t kValuPerIteration = 16;
+__global__ void __launch_bounds__(kThreads) jxr_valu_chains(uint32_t *out, uint32_t seed, uint32_t iterations) {
+  uint32_t x[8], y[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    x[k] = seed + threadIdx.x * 7u + (uint32_t)k;
+    y[k] = seed ^ ((uint32_t)k * 2654435761u);
+  }
+  for (uint32_t i = 0; i < iterations; i++) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+#if defined(__HIP_DEVICE_COMPILE__)
+      asm volatile("v_add_u32 %0, %0, %1\n\tv_xor_b32 %1, %1, %0" : "+v"(x[k]), "+v"(y[k]));
+#else  // host pass and the CPU test build: the same arithmetic
+      x[k] += y[k];
+      y[k] ^= x[k];
+#endif
+    }
+  }
+  uint32_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) acc += x[k] ^ y[k];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
+hipError_t launch_valu_chains(uint32_t *out, uint32_t groups, uint32_t iterations, hipStream_t stream) {
+  hipLaunchKernelGGL(jxr_valu_chains, dim3(groups), dim3(kThreads), 0, stream, out, 1u, iterations);
+  return hipGetLastError();
+}
+uint32_t valu_per_iteration() { return kValuPerIteration; }
+
 }  // namespace jxr

@@ -42,6 +42,8 @@ hipError_t launch_expand(const Slot *slots, uint32_t slot_count, uint32_t item_c
                          uint32_t *counters, hipStream_t stream);
 hipError_t kernel_occupancy(uint32_t storage, size_t dynamic_lds, int *groups_per_cu);
 hipError_t set_kernel_lds_limit(size_t bytes);
+hipError_t launch_valu_chains(uint32_t *out, uint32_t groups, uint32_t iterations, hipStream_t stream);
+uint32_t valu_per_iteration();
 }  // namespace jxr
 
 using namespace jxr;
@@ -1202,6 +1204,40 @@ extern "C" jxrhip_status jxrhip_copy_device(jxrhip_session *s, void *dst, const 
   HIP_TRY(hipStreamSynchronize(s->streams[0]), "hipStreamSynchronize");
   return JXRHIP_OK;
 }
+extern "C" jxrhip_status jxrhip_calibrate_valu(jxrhip_session *s, uint32_t iterations, jxrhip_valu_rate *out) {
+  if (!s || !out || iterations == 0) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument or no iterations");
+  make_current(s);
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, s->device), "hipGetDeviceProperties");
+  const uint32_t groups = s->cus * 4;  // sixteen waves per compute unit, as the reconstruction kernel runs
+  hipEvent_t a = nullptr, b = nullptr;
+  DeviceBuffer buf;
+  float ms = 0.f;
+  hipError_t e;
+  {
+    std::lock_guard<std::mutex> g(s->lock);
+    e = s->pool.get((uint64_t)groups * kThreads * 4, buf, s->diag.scratch_hits, s->diag.scratch_misses);
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    hipStream_t st = s->streams[0];
+    if (e == hipSuccess) e = launch_valu_chains((uint32_t *)buf.ptr, groups, 64, st);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(a, st);
+    if (e == hipSuccess) e = launch_valu_chains((uint32_t *)buf.ptr, groups, iterations, st);
+    if (e == hipSuccess) e = hipEventRecord(b, st);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    if (buf.ptr) s->pool.put(buf);
+  }
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  if (e != hipSuccess) return hip_fail(e, "VALU calibration");
+  out->ms = ms;
+  out->wave_instructions = (double)groups * (kThreads / 64) * iterations * valu_per_iteration();
+  out->simds = s->cus * 4;
+  out->clock_khz = (uint32_t)prop.clockRate;
+  return JXRHIP_OK;
+}
+
 extern "C" jxrhip_status jxrhip_session_synchronize(jxrhip_session *s) {
   if (!s) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null session");
   make_current(s);

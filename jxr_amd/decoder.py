@@ -564,6 +564,13 @@ class HipDecoderSession:
                  "candidates": c.candidates, "best_ms": c.best_ms, "default_ms": c.default_ms}
                 for c in out[: min(nb.value, cap)]]
 
+    def calibrate_valu(self, iterations: int = 20000):
+        """Issue rate of the vector ALU (jxrhip_calibrate_valu): dict of ms,
+        wave_instructions, simds and clock_khz of one timed launch."""
+        r = abi.ValuRate()
+        check(load().jxrhip_calibrate_valu(self._h, iterations, C.byref(r)), "calibrate_valu")
+        return {"ms": r.ms, "wave_instructions": r.wave_instructions, "simds": r.simds, "clock_khz": r.clock_khz}
+
     def set_bucket_tile(self, bucket_key: int, tx: int, ty: int):
         check(load().jxrhip_session_set_bucket_tile(self._h, bucket_key, tx, ty), "set_bucket_tile")
 

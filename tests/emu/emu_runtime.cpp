@@ -81,6 +81,7 @@ hipError_t hipGetDeviceProperties(hipDeviceProp_t *prop, int) {
   prop->multiProcessorCount = 4;
   prop->sharedMemPerBlock = 65536;
   prop->sharedMemPerBlockOptin = 65536;
+  prop->clockRate = 2400000;
   return hipSuccess;
 }
 hipError_t hipSetDevice(int) { return hipSuccess; }

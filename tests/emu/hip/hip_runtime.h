@@ -102,6 +102,7 @@ struct hipDeviceProp_t {
   int multiProcessorCount;
   size_t sharedMemPerBlock;
   size_t sharedMemPerBlockOptin;
+  int clockRate;  // kHz
 };
 
 hipError_t hipGetDeviceCount(int *n);

@@ -50,6 +50,11 @@ STRUCTS = {
     "jxrhip_image_desc": abi.ImageDesc,
     "jxrhip_diagnostics": abi.Diagnostics,
     "jxrhip_encode_params": abi.EncodeParams,
+    "jxrhip_tile_choice": abi.TileChoice,
+    "jxrhip_server_config": abi.ServerConfig,
+    "jxrhip_server_result": abi.ServerResult,
+    "jxrhip_server_stats": abi.ServerStats,
+    "jxrhip_valu_rate": abi.ValuRate,
 }
 
 
