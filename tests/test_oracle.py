@@ -1,6 +1,7 @@
-"""Pins the CPU oracle: known-answer vectors from the reference's own unit
-tests (cited per test), forward/inverse round trips of every operator on the
-reference's seeded LCG samples, closed forms, and brute force."""
+"""Pins the CPU oracle (the reference's own known-answer vectors are in
+tests/golden, checked by tests/test_golden.py): forward/inverse round trips
+of every operator on the reference's seeded LCG samples, closed forms, and
+brute force."""
 import ctypes as C
 import struct
 
@@ -25,22 +26,6 @@ def lcg_samples(seed, bits, count):
 
 
 # ---- core transform (crates/jxr-math/src/transform.rs:461-516) -----------
-def test_inverse_transform_matches_t835_signed_rounding_case():
-    v, e = O.op("jxro_inverse_core_transform", [50, -42, 9, 21, 126, 33, -6, -18, -24, 21, -3, -9, -6, -33, 6, 15])
-    assert not e
-    assert v.tolist() == [50, 50, 47, 49, 39, 38, 41, 41, -30, -44, 27, 36, -47, -55, -23, -20]
-
-
-def test_dc_only_block_is_uniform():
-    v, _ = O.op("jxro_inverse_core_transform", [16] + [0] * 15)
-    assert v.tolist() == [4] * 16
-
-
-def test_inverse_chroma_422_known_answer():
-    v, _ = O.op("jxro_inverse_chroma_422", [1, 2, 3, 4, 5, 6, 7, 8])
-    assert v.tolist() == [4, -3, -4, -2, 12, -2, -3, -1]
-
-
 def test_forward_transforms_concentrate_flat_input_in_dc():
     v, _ = O.op("jxro_forward_core_transform", [7] * 16)
     assert all(x == 0 for x in v[1:])
@@ -104,12 +89,8 @@ def test_overlap_filters_are_exact_inverses(bits):
 
 
 def test_overlap_known_answers():
-    assert O.op("jxro_inverse_overlap_2", [1, 2])[0].tolist() == [2, 3]
-    assert O.op("jxro_inverse_overlap_2x2", [1, 2, 3, 4])[0].tolist() == [3, 4, 4, 5]
     assert O.op("jxro_forward_overlap_2", [2, 3])[0].tolist() == [1, 2]
     assert O.op("jxro_forward_overlap_2x2", [3, 4, 4, 5])[0].tolist() == [1, 2, 3, 4]
-    # crates/jxr-native/src/reconstruct/overlap.rs:497-501
-    assert O.op("jxro_inverse_overlap_4", [1, 2, 3, 4])[0].tolist() == [2, 1, 4, 5]
     assert O.op("jxro_inverse_overlap_4", [0] * 4)[0].tolist() == [0] * 4
     assert O.op("jxro_inverse_overlap_4x4", [0] * 16)[0].tolist() == [0] * 16
     assert O.op("jxro_forward_overlap_2", [-(2**31), 2**31 - 1])[1]
@@ -124,28 +105,6 @@ def _plane_overlap(samples, w, h, cols, rows, hard, fn="jxro_overlap_plane"):
                                len(cb), rb.ctypes.data_as(C.POINTER(C.c_uint32)), len(rb), 1 if hard else 0)
     assert not err
     return s.tolist()
-
-
-def test_soft_and_hard_tile_planes_known_answers():
-    # crates/jxr-native/src/reconstruct/overlap.rs:504-529
-    src = [1, 1, 1, 1, 9, 9, 9, 9] * 4
-    soft = _plane_overlap(src, 8, 4, [0, 4, 8], [0, 4], False)
-    hard = _plane_overlap(src, 8, 4, [0, 4, 8], [0, 4], True)
-    assert soft == [1, 1, 3, 6, 8, 11, 11, 12, 1, 1, 3, 6, 8, 11, 13, 14] * 2
-    assert hard == [1, 1, 1, 1, 11, 12, 11, 12, 1, 1, 1, 1, 13, 14, 13, 14] * 2
-
-
-def test_full_plane_covers_two_sample_image_edges():
-    # crates/jxr-native/src/reconstruct/overlap.rs:532-563
-    rows = [
-        [-88, -88, -88, -88, -87, -87, -88, -88, -87, -87, -88, -88, -87, -87, -88, -88],
-        [-87, -87, -88, -88, -87, -87, -88, -88, -87, -87, -88, -88, -87, -87, -87, -87],
-        [-88, -88, -87, -87, -88, -88, -87, -87, -88, -88, -87, -87, -88, -88, -88, -88],
-        [-87, -87, -88, -88, -88, -88, -88, -88, -88, -88, -88, -88, -88, -88, -87, -87],
-    ]
-    order = [0, 1, 2, 2, 3, 3, 2, 2, 3, 3, 2, 2, 3, 3, 0, 1]
-    samples = [v for r in order for v in rows[r]]
-    assert _plane_overlap(samples, 16, 16, [0, 16], [0, 16], False) == [-128] * 256
 
 
 def test_subsampled_plane_round_trip_and_corner_order():
@@ -200,9 +159,6 @@ def up(prev, cur, nxt, centering):
 
 
 def test_upsampling_known_answers():
-    assert up(0, 8, 16, 0) == [8, 12]
-    assert up(0, 8, 16, 4) == [4, 8]
-    assert up(-9, -4, 3, 2) == [-5, -2]
     assert up(-(2**31), 2**31 - 1, -(2**31), 0) == [2**31 - 1, 0]
     for c in range(5):
         for s in (2**31 - 1, -(2**31)):

@@ -1,6 +1,6 @@
-"""The encoder's quality scale against the reference encoder's own known
-answers and properties (crates/jxr-native/src/encode/quality.rs:620-912):
-the calibrated anchors of 8-bit RGB, the chroma offsets by sampling, the
+"""The encoder's quality scale against the reference encoder's properties
+(crates/jxr-native/src/encode/quality.rs:620-912; its calibrated anchors
+and chroma offsets are golden vectors, tests/test_golden.py): the
 shift of sixteen parameters per doubling of the sample range, the whole
 steps of ranges below eight bits, and a scale that never quantizes more
 finely as the quality falls."""
@@ -23,30 +23,9 @@ def rgb8(q, color=RGB444):
     return list(luma) + list(chroma), ov
 
 
-@pytest.mark.parametrize("quality,params,overlap", [
-    (99, [5, 5, 5, 10, 10, 10], 0),
-    (90, [35, 35, 35, 51, 51, 51], 1),
-    (75, [50, 50, 50, 66, 66, 66], 1),
-    (50, [63, 63, 63, 78, 78, 78], 1),
-    (25, [70, 75, 75, 86, 91, 91], 1),
-    (0, [97, 113, 113, 112, 128, 128], 2),
-])
-def test_calibrated_anchors(quality, params, overlap):
-    """quality.rs:645-666"""
-    assert rgb8(quality) == (params, overlap)
-
-
 def test_quality_100_is_lossless_without_overlap():
     assert rgb8(100) == ([0] * 6, 0)
     assert Q(100, RGB420, True, 65535) == ((0, 0, 0), (0, 0, 0), 0)
-
-
-@pytest.mark.parametrize("color,expected", [(RGB444, [10, 32, 66]), (RGB422, [7, 24, 58]), (RGB420, [6, 20, 54])])
-def test_chroma_offsets_by_sampling(color, expected):
-    """difference_parameter of 5, 16 and 50 (quality.rs:771-783), reached
-    as the chroma of qualities 99, 96 and 75, whose difference levels are
-    those parameters with no DC relief."""
-    assert [Q(q, color, True, 255)[1][2] for q in (99, 96, 75)] == expected
 
 
 def test_gray_and_yuv_components_take_the_primary_parameters():
