@@ -55,3 +55,34 @@ def test_edge_shapes_in_one_batch(session, corpus):
     assert not np.any(stages)
     for (name, _, _, want), b in zip(corpus, bufs):
         assert np.array_equal(b.to_host(), want), name
+
+
+def test_surface_planes_beyond_4_gib(session):
+    """Output planes 2.5 and 5 GiB into the surface: every offset the output
+    stage forms is 64-bit, and nothing lands outside the planes."""
+    px = synth.gradient_noise(96, 64, 5, depth=16, sigma=8.0)
+    enc = EncodedImage(px, color=abi.COLOR_YUV420, overlap=2, scaled=True, qp=(120, 120, 124))
+    spec = workloads.rgb_out(16, True)
+    want = O.decode(image_desc(enc, spec))
+    plane = 96 * 64 * 2
+    offsets = [0, (5 << 29) + 4096, (5 << 30) + 512]
+    desc = image_desc(enc, spec)
+    for c, off in enumerate(offsets):
+        desc.surface.planes[c].byte_offset = off
+    desc.surface.byte_len = offsets[-1] + plane + 4096
+    plan = HipDecodePlan(desc, keepalive=enc)
+    dst = session.allocate_destination(desc.surface.byte_len)
+    try:
+        dst.fill(0x3C)
+        stage = session.submit_into(plan, dst.ptr, dst.nbytes).wait()
+        assert not stage
+        for c, off in enumerate(offsets):
+            got = dst.to_host(off, plane)
+            assert np.array_equal(got, want[c * plane:(c + 1) * plane]), f"plane {c}"
+            after = dst.to_host(off + plane, 4096)
+            assert np.all(after == 0x3C), f"bytes after plane {c}"
+            if off:
+                before = dst.to_host(off - 4096, 4096)
+                assert np.all(before == 0x3C), f"bytes before plane {c}"
+    finally:
+        dst.close()
