@@ -83,7 +83,9 @@ def cpu_baseline(items, plans, dense, offsets, budget_s=15.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from jxr_amd import image_desc
-    threads = max(1, min(os.cpu_count() or 1, 64))
+    # the host cores this process may use, at most the sixteen a one-GPU
+    # share of the host has
+    threads = max(1, min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16))
     descs = [image_desc(enc, spec) for enc, spec in items]
     # calibrate on the first image, then size the sample to the budget
     t0 = time.perf_counter()
@@ -136,7 +138,11 @@ def serve(session, seconds, cache, budget_gb):
     arrivals = [start + int(t * 1e9) for t, _, _ in sched]
     i = done = failed = 0
     next_log = time.perf_counter() + 5.0
+    deadline = time.perf_counter() + seconds + 120.0
     while done < n:
+        if time.perf_counter() > deadline:
+            server.close()
+            raise SystemExit(f"serving: {n - done} of {n} requests without a result two minutes after the stream")
         now = clock_ns()
         while i < n and arrivals[i] <= now:
             _, kind, k = sched[i]
