@@ -402,6 +402,15 @@ class HipBatch:
             raise JxrHipError(code, lib.jxrhip_last_error().decode(errors="replace"))
         return stages[: self.count]
 
+    def __del__(self):
+        # a batch dropped unwaited is waited for, then freed (as a dropped
+        # pending submission is in the reference, submission.rs:132-139)
+        if getattr(self, "_h", None):
+            try:
+                self.wait(raise_on_error=False)
+            except Exception:  # noqa: BLE001
+                pass
+
 
 class HipBatchGraph:
     """A captured persistent batch. launch() replays it and returns a
