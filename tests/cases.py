@@ -87,6 +87,8 @@ CASES = {
     "gray8_dc_only": dict(w=64, h=48, ov=1, quality=70, image="gray8", color=abi.COLOR_LUMA, out=GRAY8,
                           bands=abi.BANDS_DC_ONLY),
     "rgb8_no_hp": dict(w=80, h=64, ov=2, quality=70, bands=abi.BANDS_NO_HIGH_PASS, color=abi.COLOR_YUV420),
+    "rgb8_no_flexbits": dict(w=96, h=48, ov=1, quality=60, bands=abi.BANDS_NO_FLEXBITS, color=abi.COLOR_YUV422,
+                             tiles=(2, 2), vary=True),
     "rgbaf_separate_ov1": dict(w=96, h=64, ov=1, image="rgbaf", alpha=abi.ALPHA_SEPARATE,
                                out=dict(RGBAF, planar=True)),
     "rgbaf_integrated_ov2": dict(w=80, h=48, ov=2, image="rgbaf", alpha=abi.ALPHA_INTEGRATED, quality=90,
