@@ -151,6 +151,7 @@ class OutputSpec:
     centering: tuple = (0, 0)
     crop: Optional[tuple] = None  # (x, y, w, h) in the coded plane
     row_padding: int = 0          # extra bytes per row (strided surfaces)
+    components: int = 3           # colour channels of an N-component layout
 
     @property
     def bytes_per_sample(self):
@@ -158,6 +159,10 @@ class OutputSpec:
 
     @property
     def channels(self):
+        if self.layout == abi.LAYOUT_NCOMPONENT:
+            return self.components
+        if self.layout == abi.LAYOUT_NCOMPONENT_ALPHA:
+            return self.components + 1
         return _LAYOUT_CHANNELS[self.layout]
 
 
