@@ -47,6 +47,10 @@ def make(case):
         px = synth.hdr_rgba(w, h, seed)
     elif kind == "rgba8":
         px = synth.gradient_noise(w, h, seed, channels=4, depth=8)
+    elif kind == "graya8":
+        px = synth.gradient_noise(w, h, seed, channels=2, depth=8)
+    elif kind == "graya16":
+        px = synth.gradient_noise(w, h, seed, channels=2, depth=16)
     else:
         raise ValueError(kind)
     q = case.get("quality", 100)
@@ -98,6 +102,16 @@ CASES = {
     "rgba8_bgra_premultiply": dict(w=80, h=64, ov=1, image="rgba8", alpha=abi.ALPHA_INTEGRATED, quality=80,
                                    out=dict(storage=abi.STORE_U8, depth=abi.DEPTH_U8, layout=abi.LAYOUT_BGRA,
                                             output_color=abi.COLOR_RGB, premultiply=True)),
+    "graya8_luma_alpha": dict(w=80, h=48, ov=1, quality=75, image="graya8", color=abi.COLOR_LUMA,
+                              alpha=abi.ALPHA_SEPARATE,
+                              out=dict(storage=abi.STORE_U8, depth=abi.DEPTH_U8, layout=abi.LAYOUT_LUMA_ALPHA,
+                                       output_color=abi.COLOR_LUMA)),
+    "graya16_luma_alpha_integrated": dict(w=64, h=64, ov=2, quality=85, image="graya16", color=abi.COLOR_LUMA,
+                                          alpha=abi.ALPHA_INTEGRATED,
+                                          out=dict(storage=abi.STORE_U16, depth=abi.DEPTH_U16,
+                                                   layout=abi.LAYOUT_LUMA_ALPHA, output_color=abi.COLOR_LUMA)),
+    "rgb16_420_bgr": dict(w=96, h=80, ov=2, quality=70, image="rgb16", color=abi.COLOR_YUV420,
+                          out=dict(RGB16, layout=abi.LAYOUT_BGR)),
     "rgb8_bgrx": dict(w=64, h=64, ov=1, quality=80,
                       out=dict(storage=abi.STORE_U8, depth=abi.DEPTH_U8, layout=abi.LAYOUT_BGRX,
                                output_color=abi.COLOR_RGB)),
