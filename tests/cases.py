@@ -63,7 +63,8 @@ def make(case):
         hard_tiles=case.get("hard", False), scaled=case.get("scaled", q < 100), qp=qp,
         bands=case.get("bands", abi.BANDS_ALL), alpha_mode=case.get("alpha", abi.ALPHA_NONE),
         alpha_qp=qp[0], seed=seed, vary_quantizers=case.get("vary", False),
-        shift_bits=case.get("shift_bits", 0))
+        shift_bits=case.get("shift_bits", 0), mantissa_length=case.get("mantissa", 13),
+        exponent_bias=case.get("bias", 4))
     spec = OutputSpec(**case.get("out", RGB8))
     if "crop" in case:
         spec.crop = case["crop"]
@@ -99,6 +100,11 @@ CASES = {
                                  out=dict(RGBAF, planar=False)),
     "rgbaf_premultiply": dict(w=64, h=48, ov=1, image="rgbaf", alpha=abi.ALPHA_SEPARATE,
                               out=dict(RGBAF, premultiply=True)),
+    "rgbaf_mant10_bias1": dict(w=64, h=48, ov=1, image="rgbaf", alpha=abi.ALPHA_SEPARATE, mantissa=10, bias=1,
+                               out=dict(RGBAF, planar=True, mantissa_length=10, exponent_bias=1)),
+    "rgbaf_mant16_bias6_lossy": dict(w=80, h=48, ov=2, image="rgbaf", alpha=abi.ALPHA_INTEGRATED, quality=90,
+                                     mantissa=16, bias=6,
+                                     out=dict(RGBAF, planar=False, mantissa_length=16, exponent_bias=6)),
     "rgba8_bgra_premultiply": dict(w=80, h=64, ov=1, image="rgba8", alpha=abi.ALPHA_INTEGRATED, quality=80,
                                    out=dict(storage=abi.STORE_U8, depth=abi.DEPTH_U8, layout=abi.LAYOUT_BGRA,
                                             output_color=abi.COLOR_RGB, premultiply=True)),
