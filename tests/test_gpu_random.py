@@ -62,8 +62,9 @@ def random_case(rng):
     return enc, spec
 
 
-def test_random_sweep_in_one_batch():
-    rng = np.random.default_rng(20261204)
+@pytest.mark.parametrize("seed", [20261204, 1, 2, 3])
+def test_random_sweep_in_one_batch(seed):
+    rng = np.random.default_rng(seed)
     items = [random_case(rng) for _ in range(160)]
     s = HipDecoderSession(0)
     try:
