@@ -126,7 +126,7 @@ def cpu_baseline(items, plans, dense, offsets, budget_s=15.0):
                 p95_value=round(px / slow / 1e6, 3), passes=len(runs), parity_checked_images=checked)
 
 
-def serve(session, seconds, cache, budget_gb):
+def serve(session, seconds, cache, budget_gb, rate=None):
     """The serving stream (Poisson arrivals at 2000/s, the c5 mix) through
     the native serving loop; returns its summary. With `cache` the pool's
     coefficients stay resident on the device (uploaded once before the
@@ -142,7 +142,8 @@ def serve(session, seconds, cache, budget_gb):
             for p in v:
                 session.upload(p)
         session.synchronize()
-    sched = workloads.c5_schedule(seconds)
+    rate = rate or workloads.C5_RATE
+    sched = workloads.c5_schedule(seconds, rate=rate)
     n = len(sched)
     offered_px = sum(plans[kind][k].output_pixels for _, kind, k in sched)
     server = HipServer(session, hbm_budget_bytes=int(budget_gb * (1 << 30)), max_wave_images=256, max_waves=4)
@@ -174,7 +175,7 @@ def serve(session, seconds, cache, budget_gb):
     server.close()
     wall = (end - start) / 1e9
     return {
-        "workload": "c5: Poisson 2000 req/s; 70% RGB u8 256..1024 q75 4:4:4 ov1, 20% 4096x4096 gray u16 8x8 "
+        "workload": f"c5: Poisson {rate:g} req/s; 70% RGB u8 256..1024 q75 4:4:4 ov1, 20% 4096x4096 gray u16 8x8 "
                     "tiles ov2 q75, 10% 512x512 regions of 8192x8192 RGB u8 (512-px tiles)",
         "seconds_offered": seconds,
         "uploads": "resident (pool uploaded once)" if cache else "per request (PCIe inside the latency)",
@@ -201,10 +202,11 @@ def main():
     ap.add_argument("--tile", default=None, help="TXxTY macroblocks per work item")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--no-graph", action="store_true", help="submit the batch eagerly every step")
+    ap.add_argument("--serve-rate", type=float, default=0.0, help="arrivals per second (0: the c5 rate, 2000)")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="seconds of the c5 serving stream to run after the batch (0: none)")
     ap.add_argument("--serve-uploads", choices=["resident", "per-request"], default="resident")
-    ap.add_argument("--serve-budget-gb", type=float, default=0.0, help="admission budget (0: 3/4 of free HBM)")
+    ap.add_argument("--serve-budget-gb", type=float, default=0.0, help="admission budget (0: the server's default, 8 GiB or 3/4 of free HBM if less)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also copy 256 MiB device-to-device (HBM counter calibration)")
     args = ap.parse_args()
@@ -352,7 +354,8 @@ def main():
                                           "ms_per_step": round(ps_ms, 4)}
         out["batched_over_per_image_one_at_a_time"] = round(value / ps_value, 3)
     if args.serve_seconds > 0:
-        out["serving"] = serve(session, args.serve_seconds, args.serve_uploads == "resident", args.serve_budget_gb)
+        out["serving"] = serve(session, args.serve_seconds, args.serve_uploads == "resident", args.serve_budget_gb,
+                               args.serve_rate)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(items, plans, dense, offsets)
     print(json.dumps(out), flush=True)

@@ -457,7 +457,11 @@ void jxrhip_batch_graph_destroy(jxrhip_batch_graph *graph);
  * are admitted as memory frees up and their waves overlap on the device
  * (continuous batching). Results carry each request's status, its
  * arrival-to-resident latency and, when the request gave no destination, the
- * output image. The nearest callers in the reference are the batch decoder's
+ * output image. Such an image counts against the budget until the caller
+ * destroys it, so a caller that falls behind slows admission rather than
+ * exhausting device memory; when nothing is in flight one request is
+ * admitted regardless, so a caller that drains before polling still sees
+ * every request finish. The nearest callers in the reference are the batch decoder's
  * grouped submissions (crates/jxr/src/batch/cuda.rs:20-156, indexed
  * per-image errors); admission and continuous batching have no counterpart
  * there. Plans and destinations must outlive their requests; submit and poll
@@ -466,7 +470,8 @@ void jxrhip_batch_graph_destroy(jxrhip_batch_graph *graph);
 typedef struct jxrhip_server jxrhip_server;
 
 typedef struct jxrhip_server_config {
-  uint64_t hbm_budget_bytes; /* admitted footprint limit (0: 3/4 of free HBM) */
+  uint64_t hbm_budget_bytes; /* admitted footprint limit (0: 8 GiB, or 3/4 of
+                                free HBM if less)                              */
   uint32_t max_inflight;     /* admitted requests at once (0: 4096)           */
   uint32_t max_wave_images;  /* images per wave, up to 65536 (0: 256)         */
   uint32_t max_waves;        /* waves on the device at once, 1..4 (0: 4)      */

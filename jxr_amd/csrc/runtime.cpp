@@ -692,6 +692,9 @@ struct jxrhip_image {
   jxrhip_session *session = nullptr;
   DeviceBuffer buf;
   uint64_t bytes = 0;
+  // a serving loop's result: the bytes it counts against that loop's budget
+  // until the caller destroys the image
+  std::shared_ptr<std::atomic<uint64_t>> held;
 };
 
 struct jxrhip_submission {
@@ -1250,6 +1253,7 @@ extern "C" uint64_t jxrhip_image_bytes(const jxrhip_image *im) { return im ? im-
 extern "C" void jxrhip_image_destroy(jxrhip_image *im) {
   if (!im) return;
   jxrhip_session *s = im->session;
+  if (im->held) im->held->fetch_sub(im->buf.bytes);
   {
     std::lock_guard<std::mutex> g(s->lock);
     s->pool.put(im->buf);
@@ -1867,6 +1871,10 @@ struct jxrhip_server {
   std::deque<ServerWave *> waves;
   uint64_t inflight_bytes = 0;
   uint32_t inflight_count = 0;
+  // device bytes of result images the caller has not destroyed yet: they
+  // count against the budget too, so a caller that falls behind slows
+  // admission instead of exhausting device memory
+  std::shared_ptr<std::atomic<uint64_t>> held = std::make_shared<std::atomic<uint64_t>>(0);
 };
 
 extern "C" uint64_t jxrhip_server_clock_ns(void) {
@@ -1890,6 +1898,8 @@ static void server_finish(jxrhip_server *sv, ServerRequest *r, jxrhip_status sta
       im->session = s;
       im->buf = r->output;
       im->bytes = r->plan->output_bytes;
+      im->held = sv->held;
+      sv->held->fetch_add(im->buf.bytes);
       r->output = DeviceBuffer{};
       res.image = im;
     } else {
@@ -2023,7 +2033,12 @@ static void server_loop(jxrhip_server *sv) {
         progressed = true;
         continue;
       }
-      if (sv->inflight_count >= sv->cfg.max_inflight || sv->inflight_bytes + r->footprint > sv->cfg.hbm_budget_bytes)
+      // results not yet destroyed by the caller count against the budget;
+      // with nothing in flight one request is admitted regardless, so a
+      // caller that drains before it polls still sees every request finish
+      const uint64_t used = sv->inflight_bytes + sv->held->load();
+      if (sv->inflight_count >= sv->cfg.max_inflight ||
+          (sv->inflight_count > 0 && used + r->footprint > sv->cfg.hbm_budget_bytes))
         break;
       sv->pending.pop_front();
       r->admitted_ns = jxrhip_server_clock_ns();
@@ -2065,7 +2080,11 @@ extern "C" jxrhip_status jxrhip_server_create(jxrhip_session *s, const jxrhip_se
   if (!sv->cfg.hbm_budget_bytes) {
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-    sv->cfg.hbm_budget_bytes = free_b / 4 * 3;
+    // 8 GiB (or 3/4 of free memory, if less): enough work in flight to fill
+    // the device, and within what the buffer pool keeps for reuse, so that
+    // admitted outputs recycle pooled buffers instead of allocating and
+    // freeing device memory under load (which serialises the device)
+    sv->cfg.hbm_budget_bytes = std::min<uint64_t>(free_b / 4 * 3, uint64_t(8) << 30);
   }
   if (!sv->cfg.max_inflight) sv->cfg.max_inflight = 4096;
   if (!sv->cfg.max_wave_images) sv->cfg.max_wave_images = 256;
