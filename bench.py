@@ -126,16 +126,23 @@ def cpu_baseline(items, plans, dense, offsets, budget_s=15.0):
                 p95_value=round(px / slow / 1e6, 3), passes=len(runs), parity_checked_images=checked)
 
 
+_SERVE_POOLS = {}
+
+
 def serve(session, seconds, cache, budget_gb, rate=None):
-    """The serving stream (Poisson arrivals at 2000/s, the c5 mix) through
-    the native serving loop; returns its summary. With `cache` the pool's
-    coefficients stay resident on the device (uploaded once before the
-    stream); without, every request uploads its own."""
+    """The serving stream (Poisson arrivals, 2000/s unless `rate` says
+    otherwise, the c5 mix) through the native serving loop; returns its
+    summary. With `cache` the pool's coefficients stay resident on the device
+    (uploaded once before the stream); without, every request uploads its
+    own. The pool is built once per process."""
     from jxr_amd import HipDecodePlan, HipServer, workloads
     from jxr_amd.serving import clock_ns
     t0 = time.perf_counter()
-    pool = workloads.c5_pool(cache=cache)
-    plans = {k: [HipDecodePlan.from_encoded(e, sp) for e, sp in pool[k]] for k in ("small", "gray", "region")}
+    if cache not in _SERVE_POOLS:
+        pool = workloads.c5_pool(cache=cache)
+        _SERVE_POOLS[cache] = (pool, {k: [HipDecodePlan.from_encoded(e, sp) for e, sp in pool[k]]
+                                      for k in ("small", "gray", "region")})
+    pool, plans = _SERVE_POOLS[cache]
     log(f"[serve] pool of {sum(len(v) for v in plans.values())} images in {time.perf_counter() - t0:.1f} s")
     if cache:
         for v in plans.values():
@@ -206,6 +213,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="seconds of the c5 serving stream to run after the batch (0: none)")
     ap.add_argument("--serve-uploads", choices=["resident", "per-request"], default="resident")
+    ap.add_argument("--serve-loaded-rate", type=float, default=40000.0,
+                    help="arrivals per second of a second, loaded serving run (0: none)")
     ap.add_argument("--serve-budget-gb", type=float, default=0.0, help="admission budget (0: the server's default, 8 GiB or 3/4 of free HBM if less)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also copy 256 MiB device-to-device (HBM counter calibration)")
@@ -356,6 +365,11 @@ def main():
     if args.serve_seconds > 0:
         out["serving"] = serve(session, args.serve_seconds, args.serve_uploads == "resident", args.serve_budget_gb,
                                args.serve_rate)
+        if args.serve_loaded_rate > 0:
+            # the same mix offered twenty times faster: continuous batching
+            # with the device busy (waves of several images)
+            out["serving_loaded"] = serve(session, min(args.serve_seconds, 5.0), args.serve_uploads == "resident",
+                                          args.serve_budget_gb, args.serve_loaded_rate)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(items, plans, dense, offsets)
     print(json.dumps(out), flush=True)
