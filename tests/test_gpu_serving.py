@@ -122,3 +122,30 @@ def test_oversized_request_is_refused(session, requests):
         assert server.stats()["rejected"] == 1
     finally:
         server.close()
+
+
+def test_held_results_slow_admission_but_every_request_finishes(session, requests):
+    """Result images the caller keeps count against the budget: with room
+    for one and a half requests and a caller that polls only after draining,
+    the kept outputs fill the budget after a few requests, the loop goes on
+    one request at a time, and every request still finishes with the right
+    bytes."""
+    reqs, alone = requests
+    i = next(k for k, r in enumerate(reqs) if r[0] == "large0")
+    plan = reqs[i][1]
+    assert plan.output_bytes * 4 < plan.device_footprint * 3 // 2 < plan.output_bytes * 10
+    server = HipServer(session, hbm_budget_bytes=plan.device_footprint * 3 // 2, max_waves=1)
+    try:
+        for n in range(10):
+            server.submit(plan, tag=n)
+        server.drain()
+        results = server.poll(capacity=16)
+        assert sorted(r.tag for r in results) == list(range(10))
+        for r in results:
+            assert r.status == 0
+            assert np.array_equal(r.image.readback()[: alone[i].nbytes], alone[i])
+            r.image.close()
+        st = server.stats()
+        assert st["completed"] == 10 and st["failed"] == 0 and st["rejected"] == 0
+    finally:
+        server.close()
