@@ -1342,7 +1342,7 @@ static void release_prepared(jxrhip_session *s, PreparedBatch &pb) {
 // Called with the session lock held; on failure the caller releases `pb`.
 static jxrhip_status prepare_persistent(jxrhip_session *s, const jxrhip_plan *const *plans, uint32_t count,
                                         void *const *dsts, const uint32_t *force, PreparedBatch &pb,
-                                        hipStream_t upload_stream = nullptr) {
+                                        hipStream_t upload_stream = nullptr, Slot *pinned_slots = nullptr) {
   hipStream_t stream = upload_stream ? upload_stream : s->streams[0];
   pb.plans.assign(plans, plans + count);
   pb.dsts.assign(dsts, dsts + count);
@@ -1396,10 +1396,18 @@ static jxrhip_status prepare_persistent(jxrhip_session *s, const jxrhip_plan *co
   HIP_TRY(s->pool.get(std::max<size_t>(slots.size(), 1) * sizeof(Slot), pb.slots, s->diag.scratch_hits,
                       s->diag.scratch_misses),
           "slot buffer");
-  HIP_TRY(hipMemcpyAsync(pb.slots.ptr, slots.data(), slots.size() * sizeof(Slot), hipMemcpyHostToDevice, stream),
-          "slot upload");
-  // the host records are staged before this returns
-  HIP_TRY(hipStreamSynchronize(stream), "slot upload");
+  if (pinned_slots) {
+    // pinned staging of the caller's, left alone until the stream passes
+    // this copy: the copy is a plain DMA and the host does not wait for it
+    std::copy(slots.begin(), slots.end(), pinned_slots);
+    HIP_TRY(hipMemcpyAsync(pb.slots.ptr, pinned_slots, slots.size() * sizeof(Slot), hipMemcpyHostToDevice, stream),
+            "slot upload");
+  } else {
+    HIP_TRY(hipMemcpyAsync(pb.slots.ptr, slots.data(), slots.size() * sizeof(Slot), hipMemcpyHostToDevice, stream),
+            "slot upload");
+    // the host records are staged before this returns
+    HIP_TRY(hipStreamSynchronize(stream), "slot upload");
+  }
   uint64_t total_items = 0;
   for (const BucketLaunch &L : pb.launches) total_items += L.items;
   HIP_TRY(s->pool.get(std::max<uint64_t>(total_items, 1) * sizeof(uint32_t), pb.item_slots, s->diag.scratch_hits,
@@ -1848,10 +1856,12 @@ struct ServerWave {
 struct jxrhip_server {
   jxrhip_session *s = nullptr;
   jxrhip_server_config cfg{};
-  hipStream_t upload = nullptr;
   hipStream_t compute[kServerStreams] = {};
   hipEvent_t done[kServerStreams] = {};
   uint32_t *host_status[kServerStreams] = {};
+  // per-stream pinned staging for the wave's slot table; a stream's wave
+  // (and so its upload) has finished before the stream takes the next one
+  Slot *host_slots[kServerStreams] = {};
   bool slot_busy[kServerStreams] = {};
   std::thread worker;
   std::mutex m;                   // arrivals, results, outstanding, stop, statistics
@@ -1951,7 +1961,8 @@ static void server_launch_wave(jxrhip_server *sv, uint32_t slot) {
       plans.push_back(r->plan);
       dsts.push_back(r->dst ? r->dst : r->output.ptr);
     }
-    if (!st) st = prepare_persistent(s, plans.data(), (uint32_t)n, dsts.data(), nullptr, w->pb, sv->upload);
+    if (!st) st = prepare_persistent(s, plans.data(), (uint32_t)n, dsts.data(), nullptr, w->pb, sv->compute[slot],
+                                     sv->host_slots[slot]);
     if (!st) st = enqueue_persistent(s, w->pb, sv->compute[slot]);
     if (!st && hipMemcpyAsync(sv->host_status[slot], w->pb.statuses.ptr, sizeof(uint32_t) * n,
                               hipMemcpyDeviceToHost, sv->compute[slot]) != hipSuccess)
@@ -2096,16 +2107,17 @@ extern "C" jxrhip_status jxrhip_server_create(jxrhip_session *s, const jxrhip_se
       if (sv->compute[i]) (void)hipStreamDestroy(sv->compute[i]);
       if (sv->done[i]) (void)hipEventDestroy(sv->done[i]);
       if (sv->host_status[i]) (void)hipHostFree(sv->host_status[i]);
+      if (sv->host_slots[i]) (void)hipHostFree(sv->host_slots[i]);
+    if (sv->host_slots[i]) (void)hipHostFree(sv->host_slots[i]);
     }
-    if (sv->upload) (void)hipStreamDestroy(sv->upload);
     return st;
   };
-  if (hipStreamCreateWithFlags(&sv->upload, hipStreamNonBlocking) != hipSuccess)
-    return undo(fail(JXRHIP_ERR_DEVICE, "hipStreamCreate"));
   for (uint32_t i = 0; i < kServerStreams; i++) {
     if (hipStreamCreateWithFlags(&sv->compute[i], hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&sv->done[i], hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void **)&sv->host_status[i], sizeof(uint32_t) * sv->cfg.max_wave_images,
+                      hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&sv->host_slots[i], sizeof(Slot) * sv->cfg.max_wave_images,
                       hipHostMallocDefault) != hipSuccess)
       return undo(fail(JXRHIP_ERR_DEVICE, "server streams"));
   }
@@ -2216,8 +2228,8 @@ extern "C" void jxrhip_server_destroy(jxrhip_server *sv) {
     if (sv->compute[i]) (void)hipStreamDestroy(sv->compute[i]);
     if (sv->done[i]) (void)hipEventDestroy(sv->done[i]);
     if (sv->host_status[i]) (void)hipHostFree(sv->host_status[i]);
+    if (sv->host_slots[i]) (void)hipHostFree(sv->host_slots[i]);
   }
-  if (sv->upload) (void)hipStreamDestroy(sv->upload);
   {
     std::lock_guard<std::mutex> g(sv->s->lock);
     sv->s->live_servers--;
