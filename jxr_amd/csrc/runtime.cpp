@@ -1538,10 +1538,14 @@ extern "C" jxrhip_status jxrhip_submit_batch(jxrhip_session *s, const jxrhip_pla
     b->host_statuses = nullptr;
     return st;
   };
-  if (hipHostMalloc((void **)&b->host_statuses, sizeof(uint32_t) * std::max<uint32_t>(count, 1),
+  // one pinned block: the status words, then the staging of the slot table,
+  // which the batch keeps (as it keeps the statuses) until its wait
+  const size_t status_bytes = (sizeof(uint32_t) * std::max<uint32_t>(count, 1) + 255) & ~size_t(255);
+  if (hipHostMalloc((void **)&b->host_statuses, status_bytes + sizeof(Slot) * std::max<uint32_t>(count, 1),
                     hipHostMallocDefault) != hipSuccess)
     return undo(fail(JXRHIP_ERR_DEVICE, "hipHostMalloc"));
-  jxrhip_status st = prepare_persistent(s, plans, count, dsts, nullptr, b->pb);
+  Slot *slot_staging = reinterpret_cast<Slot *>(reinterpret_cast<uint8_t *>(b->host_statuses) + status_bytes);
+  jxrhip_status st = prepare_persistent(s, plans, count, dsts, nullptr, b->pb, nullptr, slot_staging);
   if (st) return undo(st);
   if (hipEventRecord(s->t0, stream) != hipSuccess) return undo(fail(JXRHIP_ERR_DEVICE, "hipEventRecord"));
   st = enqueue_persistent(s, b->pb);
