@@ -298,9 +298,14 @@ uint64_t jxrhip_plan_algorithmic_bytes(const jxrhip_plan *plan);
 
 /* ---- sessions (session.rs:16-268) --------------------------------------- */
 jxrhip_status jxrhip_session_create(int32_t device_ordinal, jxrhip_session **out);
-/* Waits for the session's streams and frees it. Refused (the session stays,
- * and jxrhip_last_error says why) while captured batches or servers of the
- * session exist: destroy those first. */
+/* Releases the caller's handle. Refused (the session stays, and
+ * jxrhip_last_error says why) while captured batches or servers of the
+ * session exist: destroy those first. Submissions, batches, result images
+ * and destinations of the session that are still alive keep it (its
+ * streams, pool and cache) until the last of them is released, as a
+ * resident image keeps its device context (resident.rs:8-12); the session
+ * then waits for its streams and frees itself. The handle must not be used
+ * for new work after this call. */
 void jxrhip_session_destroy(jxrhip_session *session);
 jxrhip_status jxrhip_session_diagnostics(jxrhip_session *session, jxrhip_diagnostics *out);
 /* Upload-cache byte budget (default 64 GiB of the device's HBM). */
@@ -330,7 +335,9 @@ void jxrhip_submission_destroy(jxrhip_submission *submission);
 /* Device destinations (CudaDecoderSession::allocate_destination /
  * allocate_batch_destination, session.rs:85-101): plain device memory of the
  * session's device, usable as submit_into / submit_batch targets. A dense
- * batch is one allocation with per-image offsets. */
+ * batch is one allocation with per-image offsets. jxrhip_free_destination
+ * takes only pointers this session allocated, and works after
+ * jxrhip_session_destroy (each live destination keeps the session). */
 jxrhip_status jxrhip_allocate_destination(jxrhip_session *session, uint64_t bytes, void **device_ptr);
 jxrhip_status jxrhip_free_destination(jxrhip_session *session, void *device_ptr);
 jxrhip_status jxrhip_fill_destination(jxrhip_session *session, void *device_ptr, uint8_t value,

@@ -214,7 +214,10 @@ static jxrhip_status build_plane(jxrhip_plan *plan, const jxrhip_arena *a, uint3
   uint64_t mo = cp.macroblock_offset, mc = cp.macroblock_count;
   if (mc == 0 || mo + mc > a->macroblock_count || mo + mc < mo)
     return fail(JXRHIP_ERR_INVALID_PLAN, "macroblock plane range");
-  if (cp.coefficient_offset + cp.coefficient_count > a->coefficient_count)
+  // checked addition, as check_range does (jxr-core plan.rs): a huge offset
+  // must not wrap into range
+  const uint64_t co = cp.coefficient_offset, cc = cp.coefficient_count;
+  if (co + cc < co || co + cc > a->coefficient_count)
     return fail(JXRHIP_ERR_INVALID_PLAN, "coefficient plane range");
   uint32_t x0 = UINT32_MAX, y0 = UINT32_MAX, x1 = 0, y1 = 0;
   for (uint64_t i = mo; i < mo + mc; i++) {
@@ -656,6 +659,11 @@ struct CacheEntry {
   uint64_t last_use = 0;
   const void *deps[2] = {nullptr, nullptr};  // coefficient buffers a descriptor points at
   uint32_t pins = 0;  // prepared batches and graphs that read it; never evicted while pinned
+  // coefficient uploads are asynchronous: `ready` is recorded on `stream`
+  // after the copy that filled the entry, and work on any other stream that
+  // reads the entry before the copy has finished waits for it on the device
+  hipEvent_t ready = nullptr;
+  hipStream_t stream = nullptr;
 };
 
 struct jxrhip_session {
@@ -675,6 +683,14 @@ struct jxrhip_session {
   uint32_t tile_x = 0, tile_y = 0, groups_per_cu = 0;
   std::map<uint64_t, std::pair<uint32_t, uint32_t>> tuned;  // bucket key -> tile shape
   uint32_t live_graphs = 0, live_servers = 0;  // objects that must be destroyed before the session
+  // owners of the session: the caller's handle, plus every submission,
+  // result image and batch not yet released. The last one frees it, so an
+  // image may outlive jxrhip_session_destroy (a resident image keeps its
+  // device context alive, resident.rs:8-12).
+  std::atomic<uint32_t> refs{1};
+  // device destinations from jxrhip_allocate_destination; each one is an
+  // owner too, until jxrhip_free_destination
+  std::unordered_map<void *, uint64_t> destinations;
   double last_ms = 0;
   uint32_t last_launches = 0;
   jxrhip_diagnostics diag{};
@@ -709,6 +725,32 @@ struct jxrhip_submission {
 };
 
 static void make_current(jxrhip_session *s) { (void)hipSetDevice(s->device); }
+
+static void session_free(jxrhip_session *s);
+static void session_ref(jxrhip_session *s) { s->refs.fetch_add(1, std::memory_order_relaxed); }
+static void session_unref(jxrhip_session *s) {
+  if (s->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) session_free(s);
+}
+
+// An event for one-shot device ordering (completion of a submission, an
+// upload): reused from the session's free list when one is there.
+static hipError_t take_event(jxrhip_session *s, hipEvent_t *e) {
+  if (!s->free_events.empty()) {
+    *e = s->free_events.back();
+    s->free_events.pop_back();
+    return hipSuccess;
+  }
+  return hipEventCreateWithFlags(e, hipEventDisableTiming);
+}
+
+// Returns a cache entry's memory to the pool (the caller has made sure no
+// queued work reads it) and its upload event to the free list.
+static void drop_entry(jxrhip_session *s, CacheEntry &ce) {
+  s->cache_bytes -= ce.buf.bytes;
+  s->pool.put(ce.buf);
+  if (ce.ready) s->free_events.push_back(ce.ready);
+  ce.ready = nullptr;
+}
 
 extern "C" jxrhip_status jxrhip_session_create(int32_t ordinal, jxrhip_session **out) {
   if (!out) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null output");
@@ -764,11 +806,17 @@ extern "C" void jxrhip_session_destroy(jxrhip_session *s) {
       return;
     }
   }
+  // submissions, images and batches still alive keep the session until the
+  // last of them is released
+  session_unref(s);
+}
+
+static void session_free(jxrhip_session *s) {
   make_current(s);
   for (uint32_t i = 0; i < kStreams; i++)
     if (s->streams[i]) (void)hipStreamSynchronize(s->streams[i]);
-  for (auto &kv : s->coef_cache) s->pool.put(kv.second.buf);
-  for (auto &kv : s->desc_cache) s->pool.put(kv.second.buf);
+  for (auto &kv : s->coef_cache) drop_entry(s, kv.second);
+  for (auto &kv : s->desc_cache) drop_entry(s, kv.second);
   if (s->status_dev) (void)hipFree(s->status_dev);
   if (s->status_host) (void)hipHostFree(s->status_host);
   for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
@@ -828,10 +876,11 @@ static void evict_until(jxrhip_session *s, uint64_t need) {
         }
     if (!victim_map) return;
     auto it = victim_map->find(victim);
-    // a buffer still read by queued work must not be recycled early
+    // a buffer still read by queued work must not be recycled early; work
+    // of a serving loop reads only pinned entries
     for (uint32_t i = 0; i < kStreams; i++) (void)hipStreamSynchronize(s->streams[i]);
-    s->cache_bytes -= it->second.buf.bytes;
-    s->pool.put(it->second.buf);
+    if (it->second.ready) (void)hipEventSynchronize(it->second.ready);
+    drop_entry(s, it->second);
     victim_map->erase(it);
     s->diag.upload_evictions++;
   }
@@ -844,9 +893,22 @@ static jxrhip_status device_coefficients(jxrhip_session *s, const jxrhip_arena *
   if (a->identity) {
     auto it = s->coef_cache.find(a->identity);
     if (it != s->coef_cache.end()) {
-      it->second.last_use = ++s->tick;
+      CacheEntry &ce = it->second;
+      if (ce.ready) {
+        // the copy that filled the entry may still be running on its stream
+        const hipError_t q = hipEventQuery(ce.ready);
+        if (q == hipSuccess) {
+          s->free_events.push_back(ce.ready);
+          ce.ready = nullptr;
+        } else if (q == hipErrorNotReady) {
+          if (ce.stream != stream) HIP_TRY(hipStreamWaitEvent(stream, ce.ready, 0), "upload dependency");
+        } else {
+          return hip_fail(q, "upload event");
+        }
+      }
+      ce.last_use = ++s->tick;
       s->diag.upload_hits++;
-      *out = (const int32_t *)it->second.buf.ptr;
+      *out = (const int32_t *)ce.buf.ptr;
       return JXRHIP_OK;
     }
   }
@@ -856,8 +918,17 @@ static jxrhip_status device_coefficients(jxrhip_session *s, const jxrhip_arena *
   HIP_TRY(s->pool.get(bytes + 64, b, s->diag.scratch_hits, s->diag.scratch_misses), "coefficient buffer");
   HIP_TRY(hipMemcpyAsync(b.ptr, a->coefficients, bytes, hipMemcpyHostToDevice, stream), "coefficient upload");
   if (a->identity) {
-    s->cache_bytes += b.bytes;
     CacheEntry ce{b, ++s->tick, {nullptr, nullptr}};
+    hipError_t e = take_event(s, &ce.ready);
+    if (e == hipSuccess) e = hipEventRecord(ce.ready, stream);
+    if (e != hipSuccess) {
+      if (ce.ready) s->free_events.push_back(ce.ready);
+      (void)hipStreamSynchronize(stream);
+      s->pool.put(b);
+      return hip_fail(e, "upload event");
+    }
+    ce.stream = stream;
+    s->cache_bytes += b.bytes;
     s->coef_cache[a->identity] = ce;
   } else if (transient_owner) {
     transient_owner->transient.push_back(b);
@@ -933,8 +1004,7 @@ static jxrhip_status device_image(jxrhip_session *s, const jxrhip_plan *p, hipSt
     it = s->desc_cache.find(p->id);
     uint32_t pins = 0;
     if (it != s->desc_cache.end()) {
-      s->cache_bytes -= it->second.buf.bytes;
-      s->pool.put(it->second.buf);
+      drop_entry(s, it->second);
       pins = it->second.pins;
     }
     s->cache_bytes += b.bytes;
@@ -973,8 +1043,7 @@ extern "C" jxrhip_status jxrhip_session_clear_cache(jxrhip_session *s) {
         ++it;
         continue;
       }
-      s->cache_bytes -= it->second.buf.bytes;
-      s->pool.put(it->second.buf);
+      drop_entry(s, it->second);
       it = m->erase(it);
     }
   s->pool.trim(0);
@@ -1058,27 +1127,35 @@ static jxrhip_status submit_impl(jxrhip_session *s, const jxrhip_plan *p, void *
   sub->session = s;
   sub->stream = s->streams[s->next_stream++ % kStreams];
   if (s->free_status.empty()) return fail(JXRHIP_ERR_BUDGET, "too many submissions in flight");
-  if (!s->free_events.empty()) {
-    sub->done = s->free_events.back();
-    s->free_events.pop_back();
-  } else {
-    HIP_TRY(hipEventCreateWithFlags(&sub->done, hipEventDisableTiming), "hipEventCreate");
-  }
+  // a failure once work may be queued waits for the stream, then returns
+  // the status word, the event and every buffer the submission took
+  auto undo = [&](jxrhip_status st) {
+    (void)hipStreamSynchronize(sub->stream);
+    for (auto &b : sub->transient) s->pool.put(b);
+    sub->transient.clear();
+    if (sub->output.ptr) s->pool.put(sub->output);
+    if (sub->slot != UINT32_MAX) s->free_status.push_back(sub->slot);
+    if (sub->done) s->free_events.push_back(sub->done);
+    return st;
+  };
+  hipError_t e = take_event(s, &sub->done);
+  if (e != hipSuccess) return hip_fail(e, "hipEventCreate");
   sub->slot = s->free_status.back();
   s->free_status.pop_back();
   uint32_t *status_word = s->status_dev + sub->slot;
   s->status_host[sub->slot] = 0;
   if (!dst) {
-    HIP_TRY(s->pool.get(p->output_bytes, sub->output, s->diag.scratch_hits, s->diag.scratch_misses), "output");
+    e = s->pool.get(p->output_bytes, sub->output, s->diag.scratch_hits, s->diag.scratch_misses);
+    if (e != hipSuccess) return undo(hip_fail(e, "output"));
     dst = sub->output.ptr;
     sub->output_bytes = p->output_bytes;
   }
   const DevImage *dimg;
   jxrhip_status st = device_image(s, p, sub->stream, &dimg, sub.get());
-  if (st) return st;
+  if (st) return undo(st);
   uint32_t tx, ty, words;
   choose_tile(s, p, tx, ty, words);
-  if ((size_t)words * 4 > s->max_lds) return fail(JXRHIP_ERR_INVALID_PLAN, "tile does not fit in LDS");
+  if ((size_t)words * 4 > s->max_lds) return undo(fail(JXRHIP_ERR_INVALID_PLAN, "tile does not fit in LDS"));
   KernelArgs args{};
   args.inline_slot.img = dimg;
   args.inline_slot.out = (uint8_t *)dst;
@@ -1092,16 +1169,19 @@ static jxrhip_status submit_impl(jxrhip_session *s, const jxrhip_plan *p, void *
   args.storage = p->dev.storage;
   const uint32_t tiles = args.inline_slot.tiles_x * ((p->mby + ty - 1) / ty);
   args.item_count = tiles;
-  HIP_TRY(hipMemsetAsync(status_word, 0, 4, sub->stream), "status reset");
-  HIP_TRY(clear_bit_rows(p, dst, sub->stream), "one-bit rows");
+  if ((e = hipMemsetAsync(status_word, 0, 4, sub->stream)) != hipSuccess) return undo(hip_fail(e, "status reset"));
+  if ((e = clear_bit_rows(p, dst, sub->stream)) != hipSuccess) return undo(hip_fail(e, "one-bit rows"));
   // one item per workgroup; a multiple of 8 workgroups lets the kernel give
   // each XCD a contiguous share of the tiles
-  HIP_TRY(launch_tiles(args, (tiles + 7u) & ~7u, sub->stream), "reconstruction launch");
-  HIP_TRY(hipMemcpyAsync(s->status_host + sub->slot, status_word, 4, hipMemcpyDeviceToHost, sub->stream),
-          "status readback");
-  HIP_TRY(hipEventRecord(sub->done, sub->stream), "hipEventRecord");
+  if ((e = launch_tiles(args, (tiles + 7u) & ~7u, sub->stream)) != hipSuccess)
+    return undo(hip_fail(e, "reconstruction launch"));
+  if ((e = hipMemcpyAsync(s->status_host + sub->slot, status_word, 4, hipMemcpyDeviceToHost, sub->stream)) !=
+      hipSuccess)
+    return undo(hip_fail(e, "status readback"));
+  if ((e = hipEventRecord(sub->done, sub->stream)) != hipSuccess) return undo(hip_fail(e, "hipEventRecord"));
   s->diag.submissions++;
   s->diag.kernel_launches++;
+  session_ref(s);
   *out = sub.release();
   return JXRHIP_OK;
 }
@@ -1122,14 +1202,17 @@ extern "C" int32_t jxrhip_submission_is_complete(const jxrhip_submission *sub) {
 
 static void release_submission(jxrhip_submission *sub) {
   jxrhip_session *s = sub->session;
-  std::lock_guard<std::mutex> g(s->lock);
-  make_current(s);
-  for (auto &b : sub->transient) s->pool.put(b);
-  sub->transient.clear();
-  if (sub->output.ptr) s->pool.put(sub->output);
-  if (sub->slot != UINT32_MAX) s->free_status.push_back(sub->slot);
-  if (sub->done) s->free_events.push_back(sub->done);
+  {
+    std::lock_guard<std::mutex> g(s->lock);
+    make_current(s);
+    for (auto &b : sub->transient) s->pool.put(b);
+    sub->transient.clear();
+    if (sub->output.ptr) s->pool.put(sub->output);
+    if (sub->slot != UINT32_MAX) s->free_status.push_back(sub->slot);
+    if (sub->done) s->free_events.push_back(sub->done);
+  }
   delete sub;
+  session_unref(s);
 }
 
 extern "C" jxrhip_status jxrhip_submission_wait(jxrhip_submission *sub, jxrhip_image **image, uint32_t *stage) {
@@ -1158,6 +1241,7 @@ extern "C" jxrhip_status jxrhip_submission_wait(jxrhip_submission *sub, jxrhip_i
     im->buf = sub->output;
     im->bytes = sub->output_bytes;
     sub->output = DeviceBuffer{};
+    session_ref(im->session);
     *image = im;
   }
   release_submission(sub);
@@ -1176,15 +1260,33 @@ extern "C" jxrhip_status jxrhip_allocate_destination(jxrhip_session *s, uint64_t
   *ptr = nullptr;
   make_current(s);
   HIP_TRY(hipMalloc(ptr, bytes ? bytes : 1), "hipMalloc");
+  {
+    std::lock_guard<std::mutex> g(s->lock);
+    s->destinations[*ptr] = bytes;
+  }
+  session_ref(s);
   return JXRHIP_OK;
 }
 extern "C" jxrhip_status jxrhip_free_destination(jxrhip_session *s, void *ptr) {
   if (!s) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null session");
   if (!ptr) return JXRHIP_OK;
   make_current(s);
-  for (uint32_t i = 0; i < kStreams; i++) HIP_TRY(hipStreamSynchronize(s->streams[i]), "hipStreamSynchronize");
-  HIP_TRY(hipFree(ptr), "hipFree");
-  return JXRHIP_OK;
+  {
+    std::lock_guard<std::mutex> g(s->lock);
+    auto it = s->destinations.find(ptr);
+    if (it == s->destinations.end())
+      return fail(JXRHIP_ERR_INVALID_ARGUMENT, "not a destination allocated by this session");
+    s->destinations.erase(it);
+  }
+  jxrhip_status st = JXRHIP_OK;
+  for (uint32_t i = 0; i < kStreams && !st; i++) {
+    const hipError_t e = hipStreamSynchronize(s->streams[i]);
+    if (e != hipSuccess) st = hip_fail(e, "hipStreamSynchronize");
+  }
+  const hipError_t e = hipFree(ptr);
+  if (e != hipSuccess && !st) st = hip_fail(e, "hipFree");
+  session_unref(s);
+  return st;
 }
 extern "C" jxrhip_status jxrhip_fill_destination(jxrhip_session *s, void *ptr, uint8_t value, uint64_t bytes) {
   if (!s || !ptr) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
@@ -1256,9 +1358,11 @@ extern "C" void jxrhip_image_destroy(jxrhip_image *im) {
   if (im->held) im->held->fetch_sub(im->buf.bytes);
   {
     std::lock_guard<std::mutex> g(s->lock);
+    make_current(s);
     s->pool.put(im->buf);
   }
   delete im;
+  session_unref(s);
 }
 
 extern "C" jxrhip_status jxrhip_readback(jxrhip_session *s, const jxrhip_image *im, void *host, uint64_t bytes) {
@@ -1514,28 +1618,40 @@ extern "C" jxrhip_status jxrhip_submit_batch(jxrhip_session *s, const jxrhip_pla
       }
       b->subs.push_back(sub);
     }
-    std::lock_guard<std::mutex> g(s->lock);
-    for (uint32_t i = 1; i < kStreams; i++) {
-      hipEvent_t e;
-      HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-      HIP_TRY(hipEventRecord(e, s->streams[i]), "hipEventRecord");
-      HIP_TRY(hipStreamWaitEvent(s->streams[0], e, 0), "hipStreamWaitEvent");
-      (void)hipEventDestroy(e);
+    // the end event on streams[0] waits for the other streams' images
+    hipError_t e = hipSuccess;
+    {
+      std::lock_guard<std::mutex> g(s->lock);
+      for (uint32_t i = 1; i < kStreams && e == hipSuccess; i++) {
+        e = hipEventRecord(s->join[i], s->streams[i]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s->streams[0], s->join[i], 0);
+      }
+      if (e == hipSuccess) e = hipEventRecord(s->t1, s->streams[0]);
+      if (e == hipSuccess) s->last_launches = count;
     }
-    HIP_TRY(hipEventRecord(s->t1, s->streams[0]), "hipEventRecord");
+    if (e != hipSuccess) {
+      const jxrhip_status st = hip_fail(e, "batch end event");
+      for (auto *x : b->subs) jxrhip_submission_destroy(x);
+      return st;
+    }
     b->timed = true;
-    s->last_launches = count;
+    session_ref(s);
     *out = b.release();
     return JXRHIP_OK;
   }
   if (mode != JXRHIP_BATCH_PERSISTENT) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "batch mode");
   std::lock_guard<std::mutex> g(s->lock);
   hipStream_t stream = s->streams[0];
+  // every failure once the pinned block exists: wait for the stream, then
+  // release the prepared batch (pool buffers and cache pins), the pinned
+  // block and the completion event
   auto undo = [&](jxrhip_status st) {
     (void)hipStreamSynchronize(stream);
     release_prepared(s, b->pb);
     if (b->host_statuses) (void)hipHostFree(b->host_statuses);
     b->host_statuses = nullptr;
+    if (b->done) (void)hipEventDestroy(b->done);
+    b->done = nullptr;
     return st;
   };
   // one pinned block: the status words, then the staging of the slot table,
@@ -1550,14 +1666,18 @@ extern "C" jxrhip_status jxrhip_submit_batch(jxrhip_session *s, const jxrhip_pla
   if (hipEventRecord(s->t0, stream) != hipSuccess) return undo(fail(JXRHIP_ERR_DEVICE, "hipEventRecord"));
   st = enqueue_persistent(s, b->pb);
   if (st) return undo(st);
-  HIP_TRY(hipEventRecord(s->t1, stream), "hipEventRecord");
-  HIP_TRY(hipMemcpyAsync(b->host_statuses, b->pb.statuses.ptr, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, stream),
-          "status readback");
-  HIP_TRY(hipEventCreateWithFlags(&b->done, hipEventDisableTiming), "hipEventCreate");
-  HIP_TRY(hipEventRecord(b->done, stream), "hipEventRecord");
+  hipError_t e = hipEventRecord(s->t1, stream);
+  if (e != hipSuccess) return undo(hip_fail(e, "hipEventRecord"));
+  e = hipMemcpyAsync(b->host_statuses, b->pb.statuses.ptr, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, stream);
+  if (e != hipSuccess) return undo(hip_fail(e, "status readback"));
+  e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+  if (e != hipSuccess) return undo(hip_fail(e, "hipEventCreate"));
+  e = hipEventRecord(b->done, stream);
+  if (e != hipSuccess) return undo(hip_fail(e, "hipEventRecord"));
   b->timed = true;
   s->last_launches = (uint32_t)b->pb.launches.size();
   s->diag.submissions += count;
+  session_ref(s);
   *out = b.release();
   return JXRHIP_OK;
 }
@@ -1616,6 +1736,7 @@ extern "C" jxrhip_status jxrhip_batch_wait(jxrhip_batch *b, uint32_t *stages) {
     if (b->done) (void)hipEventDestroy(b->done);
   }
   delete b;
+  session_unref(s);
   if (result) g_error = first_error.empty() ? g_error : first_error;
   return result;
 }
@@ -1703,6 +1824,7 @@ extern "C" jxrhip_status jxrhip_batch_graph_launch(jxrhip_batch_graph *gr, jxrhi
   s->last_launches = (uint32_t)gr->pb.launches.size();
   s->diag.kernel_launches += gr->pb.launches.size();
   s->diag.submissions += count;
+  session_ref(s);
   *out = b.release();
   return JXRHIP_OK;
 }
@@ -1915,6 +2037,7 @@ static void server_finish(jxrhip_server *sv, ServerRequest *r, jxrhip_status sta
       im->held = sv->held;
       sv->held->fetch_add(im->buf.bytes);
       r->output = DeviceBuffer{};
+      session_ref(s);
       res.image = im;
     } else {
       res.status = JXRHIP_ERR_INVALID_ARGUMENT;
@@ -2106,13 +2229,17 @@ extern "C" jxrhip_status jxrhip_server_create(jxrhip_session *s, const jxrhip_se
   if (!sv->cfg.max_waves) sv->cfg.max_waves = kServerStreams;
   if (sv->cfg.max_waves > kServerStreams) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "max_waves above 4");
   if (sv->cfg.max_wave_images > 65536) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "max_wave_images above 65536");
+  // idempotent: every handle is cleared once released
   auto undo = [&](jxrhip_status st) {
     for (uint32_t i = 0; i < kServerStreams; i++) {
       if (sv->compute[i]) (void)hipStreamDestroy(sv->compute[i]);
       if (sv->done[i]) (void)hipEventDestroy(sv->done[i]);
       if (sv->host_status[i]) (void)hipHostFree(sv->host_status[i]);
       if (sv->host_slots[i]) (void)hipHostFree(sv->host_slots[i]);
-    if (sv->host_slots[i]) (void)hipHostFree(sv->host_slots[i]);
+      sv->compute[i] = nullptr;
+      sv->done[i] = nullptr;
+      sv->host_status[i] = nullptr;
+      sv->host_slots[i] = nullptr;
     }
     return st;
   };

@@ -333,9 +333,12 @@ def plan_hip_route(reconstructed_coefficients: int, strict: bool, available: boo
 
 
 class ResidentHipImage:
-    def __init__(self, session, handle):
+    def __init__(self, session, handle, session_handle=None):
         self._s = session
         self._h = handle
+        # the image keeps the native session alive, also after the session
+        # object was closed
+        self._sh = session_handle if session_handle is not None else session.handle
 
     @property
     def device_ptr(self) -> int:
@@ -347,7 +350,7 @@ class ResidentHipImage:
 
     def readback(self) -> np.ndarray:
         out = np.empty(self.nbytes, np.uint8)
-        check(load().jxrhip_readback(self._s.handle, self._h, out.ctypes.data, out.nbytes), "readback")
+        check(load().jxrhip_readback(self._sh, self._h, out.ctypes.data, out.nbytes), "readback")
         return out
 
     def close(self):
@@ -366,6 +369,7 @@ class HipSubmission:
     def __init__(self, session, handle):
         self._s = session
         self._h = handle
+        self._sh = session.handle
 
     def is_complete(self) -> bool:
         return bool(load().jxrhip_submission_is_complete(self._h))
@@ -378,7 +382,7 @@ class HipSubmission:
         code = lib.jxrhip_submission_wait(h, C.byref(img), C.byref(stage))
         if code != abi.OK:
             raise JxrHipError(code, lib.jxrhip_last_error().decode(errors="replace"), stage.value)
-        return ResidentHipImage(self._s, img) if img.value else None
+        return ResidentHipImage(self._s, img, self._sh) if img.value else None
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -458,24 +462,27 @@ class DeviceBuffer:
         p = C.c_void_p()
         check(load().jxrhip_allocate_destination(session.handle, self.nbytes, C.byref(p)), "allocate")
         self._p = p
+        # the destination keeps the native session alive until it is freed,
+        # also after the session object was closed
+        self._sh = session.handle
 
     @property
     def ptr(self) -> int:
         return self._p.value or 0
 
     def fill(self, value: int):
-        check(load().jxrhip_fill_destination(self._s.handle, self._p, value & 0xFF, self.nbytes), "fill")
+        check(load().jxrhip_fill_destination(self._sh, self._p, value & 0xFF, self.nbytes), "fill")
 
     def to_host(self, offset: int = 0, nbytes: Optional[int] = None) -> np.ndarray:
         n = self.nbytes - offset if nbytes is None else nbytes
         out = np.empty(n, np.uint8)
-        check(load().jxrhip_copy_to_host(self._s.handle, C.c_void_p(self.ptr + offset), out.ctypes.data, n),
+        check(load().jxrhip_copy_to_host(self._sh, C.c_void_p(self.ptr + offset), out.ctypes.data, n),
               "copy_to_host")
         return out
 
     def close(self):
         if getattr(self, "_p", None) and self._p.value:
-            load().jxrhip_free_destination(self._s.handle, self._p)
+            load().jxrhip_free_destination(self._sh, self._p)
             self._p = None
 
     def __del__(self):

@@ -759,8 +759,12 @@ static int plane_setup(plane_t *p, const jxrhip_arena *a, uint32_t index, int sc
   if (!((p->bc == 4 && p->br == 4) || (p->bc == 2 && p->br == 4) || (p->bc == 2 && p->br == 2)))
     FAIL(JXRHIP_ERR_INVALID_PLAN, "unsupported component macroblock geometry");
   uint64_t mo = p->cp.macroblock_offset, mc = p->cp.macroblock_count;
-  if (mc == 0 || mo + mc > a->macroblock_count)
+  /* checked additions, as check_range (jxr-core/src/plan.rs:193-207) */
+  if (mc == 0 || mo + mc < mo || mo + mc > a->macroblock_count)
     FAIL(JXRHIP_ERR_INVALID_PLAN, "macroblock plane range");
+  uint64_t co = p->cp.coefficient_offset, cc = p->cp.coefficient_count;
+  if (co + cc < co || co + cc > a->coefficient_count)
+    FAIL(JXRHIP_ERR_INVALID_PLAN, "coefficient plane range");
   uint32_t x0 = UINT32_MAX, y0 = UINT32_MAX, x1 = 0, y1 = 0;
   for (uint64_t i = mo; i < mo + mc; i++) {
     if (a->coded_x[i] < x0) x0 = a->coded_x[i];

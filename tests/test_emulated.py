@@ -18,3 +18,14 @@ def test_emulated_hip_path_matches_oracle_under_asan():
                          capture_output=True, text=True, env=env, timeout=900)
     assert res.returncode == 0, res.stdout[-3000:] + res.stderr[-3000:]
     assert "0 failures" in res.stdout
+
+
+def test_emulated_session_lifetime_and_reuploads_under_asan():
+    """Owners that outlive the session handle, and serving re-uploads of
+    evicted arenas over two compute streams (tests/emu_lifetime.py)."""
+    emu_lib.build(asan=True)
+    env = dict(os.environ, LD_PRELOAD=emu_lib.asan_runtime(), ASAN_OPTIONS="detect_leaks=0")
+    res = subprocess.run([sys.executable, os.path.join(HERE, "emu_lifetime.py"), "--asan"],
+                         capture_output=True, text=True, env=env, timeout=600)
+    assert res.returncode == 0, res.stdout[-3000:] + res.stderr[-3000:]
+    assert "0 failures" in res.stdout

@@ -142,3 +142,17 @@ def test_planes_must_share_one_window():
     a.cx[6:12] += 1  # second plane's window shifted one macroblock right
     d = desc_for(a, abi.COLOR_YUV444, 3, OutputSpec(), (0, 0, 48, 32))
     refused(d)
+
+
+def test_coefficient_plane_range_that_wraps_is_refused_by_both():
+    # check_range uses a checked addition (crates/jxr-core/src/plan.rs:193-207):
+    # an offset so large that offset + count wraps past 2^64 is out of range
+    a = luma_arena()
+    a.planes[0].coefficient_offset = (1 << 64) - 2
+    a.planes[0].coefficient_count = 4
+    assert "coefficient plane range" in refused(luma_desc(a))
+    assert O.decode_status(luma_desc(a))[0] == abi.ERR_INVALID_PLAN
+    a = luma_arena()
+    a.planes[0].coefficient_count = a.coef.size + 1  # past the arena, no wrap
+    assert "coefficient plane range" in refused(luma_desc(a))
+    assert O.decode_status(luma_desc(a))[0] == abi.ERR_INVALID_PLAN
