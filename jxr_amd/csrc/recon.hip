@@ -1396,9 +1396,11 @@ __device__ __forceinline__ void packed_group(const OutCtx &o, JXR_GLOBAL uint8_t
         const int t0 = ar.neg(u[i]);
         const int g = ar.sub(l[i], t0 >> 1);
         const int r = ar.sub(ar.add(t0, g), (v[i] >> 1) + (v[i] & 1));
-        rgb[0] = r;
+        const int b = ar.add(v[i], r);
+        // an image with red and blue swapped (packed words only)
+        rgb[0] = im.reverse ? b : r;
         rgb[1] = g;
-        rgb[2] = ar.add(v[i], r);
+        rgb[2] = im.reverse ? r : b;
       } else {
         rgb[0] = rgb[1] = rgb[2] = l[i];
       }

@@ -413,6 +413,10 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
              dep == JXRHIP_DEPTH_RGB565 || dep == JXRHIP_DEPTH_RGB101010) {
     return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "bit depth does not match the storage kind");
   }
+  // packed RGB words of an image whose red and blue are swapped store them
+  // swapped back (output_format/color.rs:24-26); the packed layout is RGB,
+  // so `reverse` is free to carry it
+  if ((dv.out_kind == kOutPacked16 || dv.out_kind == kOutRgb101010) && !f.red_blue_not_swapped) dv.reverse = 1;
   if (dv.out_kind == kOutBits) {
     dv.bytes = 1;
   } else if (dv.out_kind == kOutPacked16 || dv.out_kind == kOutRgb101010 || dv.out_kind == kOutRgbe) {

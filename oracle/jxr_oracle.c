@@ -1228,18 +1228,15 @@ static int inverse_yuvk(const int32_t in[4], int32_t out[4]) {
   return err;
 }
 
-/* Converted colour components of one pixel (packing.rs:465-505). */
-static int color_at(const out_ctx *o, uint32_t x, uint32_t y, int32_t out[4]) {
-  const jxrhip_image_desc *d = o->d;
-  const image_t *im = o->im;
-  int32_t in[4] = {0, 0, 0, 0};
-  for (uint32_t c = 0; c < im->colors && c < 4; c++) {
-    if (o->upsample && (c == 1 || c == 2))
-      in[c] = chroma_at(o, &im->p[c], x, y);
-    else
-      in[c] = plane_at(&im->p[c], x, y);
-  }
-  uint32_t ic = d->color_format, oc = d->output.output_color;
+/* Internal-to-output colour of one pixel (output_format/color.rs:9-47):
+ * internal is the reconstructed colour at full resolution (a 4:2:0 or 4:2:2
+ * image's upsampled chroma counts as 4:4:4). Packed RGB words swap red and
+ * blue when the image does not say they are not swapped. N-component
+ * images keep their components as they are. 0 ok, 1 overflow,
+ * 2 unsupported combination. */
+int jxro_convert(uint32_t internal, uint32_t output, const int32_t in[4], int32_t packed_rgb,
+                 int32_t red_blue_not_swapped, int32_t out[4]) {
+  uint32_t ic = internal, oc = output;
   int yuv = ic == JXRHIP_COLOR_YUV420 || ic == JXRHIP_COLOR_YUV422 || ic == JXRHIP_COLOR_YUV444;
   out[0] = out[1] = out[2] = out[3] = 0;
   if (ic == JXRHIP_COLOR_LUMA && oc == JXRHIP_COLOR_LUMA) {
@@ -1248,6 +1245,11 @@ static int color_at(const out_ctx *o, uint32_t x, uint32_t y, int32_t out[4]) {
     out[0] = out[1] = out[2] = in[0];
   } else if (yuv && (oc == JXRHIP_COLOR_RGB || oc == JXRHIP_COLOR_RGBE)) {
     if (inverse_yuv(in[0], in[1], in[2], out)) return 1;
+    if (packed_rgb && !red_blue_not_swapped) {
+      int32_t t = out[0];
+      out[0] = out[2];
+      out[2] = t;
+    }
   } else if (ic == JXRHIP_COLOR_YUVK && oc == JXRHIP_COLOR_CMYK) {
     if (inverse_yuvk(in, out)) return 1;
   } else if (ic == JXRHIP_COLOR_YUVK && oc == JXRHIP_COLOR_CMYK_DIRECT) {
@@ -1261,6 +1263,22 @@ static int color_at(const out_ctx *o, uint32_t x, uint32_t y, int32_t out[4]) {
     return 2;
   }
   return 0;
+}
+
+/* Converted colour components of one pixel (packing.rs:465-505). */
+static int color_at(const out_ctx *o, uint32_t x, uint32_t y, int32_t out[4]) {
+  const jxrhip_image_desc *d = o->d;
+  const image_t *im = o->im;
+  int32_t in[4] = {0, 0, 0, 0};
+  for (uint32_t c = 0; c < im->colors && c < 4; c++) {
+    if (o->upsample && (c == 1 || c == 2))
+      in[c] = chroma_at(o, &im->p[c], x, y);
+    else
+      in[c] = plane_at(&im->p[c], x, y);
+  }
+  const int packed = o->kind == OUT_PACKED16 || o->kind == OUT_RGB101010;
+  return jxro_convert(d->color_format, d->output.output_color, in, packed, (int32_t)d->output.red_blue_not_swapped,
+                      out);
 }
 
 static void store_le(uint8_t *dst, uint64_t v, uint32_t bytes) {

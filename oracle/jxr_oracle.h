@@ -68,6 +68,11 @@ int jxro_float32_bits(int32_t sample, uint32_t scaled, uint32_t mantissa_length,
                       int32_t exponent_bias, uint32_t *bits);
 int jxro_float16_bits(int32_t sample, uint32_t scaled, uint16_t *bits);
 
+/* Internal-to-output colour of one pixel (output_format/color.rs:9-47):
+ * JXRHIP_COLOR_* codes; 0 ok, 1 overflow, 2 unsupported combination. */
+int jxro_convert(uint32_t internal, uint32_t output, const int32_t in[4], int32_t packed_rgb,
+                 int32_t red_blue_not_swapped, int32_t out[4]);
+
 /* Postfilter over a whole plane with the given tile boundaries (sample
  * units, including 0 and the extent), as reconstruct/overlap.rs:124-161. */
 int jxro_overlap_plane(int32_t *samples, uint32_t width, uint32_t height,

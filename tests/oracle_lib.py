@@ -48,6 +48,8 @@ def lib():
         fn = getattr(L, name)
         fn.restype = C.c_int
         fn.argtypes = [i32p, C.c_uint32, C.c_uint32, u32p, C.c_uint32, u32p, C.c_uint32, C.c_int32]
+    L.jxro_convert.restype = C.c_int
+    L.jxro_convert.argtypes = [C.c_uint32, C.c_uint32, i32p, C.c_int32, C.c_int32, i32p]
     L.jxro_decode.restype = C.c_int
     L.jxro_decode.argtypes = [C.POINTER(abi.ImageDesc), C.c_void_p, C.c_uint64, u32p, C.c_char_p, C.c_size_t]
     L.jxro_reconstruct_plane.restype = C.c_int
@@ -115,3 +117,13 @@ def decode_many(descs, threads, outs=None):
     if code:
         raise OracleError(code, 0, "decode_many")
     return outs
+
+
+def convert(internal, output, values, packed_rgb=False, red_blue_not_swapped=True):
+    """One pixel's colour conversion; returns (components, code)."""
+    inp = np.zeros(4, np.int32)
+    inp[: len(values)] = values
+    out = np.zeros(4, np.int32)
+    code = lib().jxro_convert(internal, output, inp.ctypes.data_as(C.POINTER(C.c_int32)), 1 if packed_rgb else 0,
+                              1 if red_blue_not_swapped else 0, out.ctypes.data_as(C.POINTER(C.c_int32)))
+    return out.tolist(), code
