@@ -1266,22 +1266,25 @@ __device__ __forceinline__ void output_group(const OutCtx &o, JXR_GLOBAL uint8_t
       for (int c = 0; c < 6; c++) st[c][i] = px[c];
     }
     const uint32_t oy = (uint32_t)(y - (int)im.crop_y);
-    if (im.planar || im.channels == 1) {
+    // a plane group of an N-component image stores store_channels channels
+    // from channel_base on; every other image stores all of its channels
+    const uint32_t nst = im.store_channels;
+    if (im.planar || im.pixel_channels == 1) {
 #pragma unroll
       for (uint32_t c = 0; c < 6; c++) {
-        if (c >= im.channels) continue;
+        if (c >= nst) continue;
         const uint32_t pl = im.planar ? c : 0;
         JXR_GLOBAL uint8_t *row = out + im.plane_offset[pl] + (uint64_t)oy * im.row_stride[pl];
         store_run<B>(row + (int64_t)ox0 * B, st[c], valid);
       }
     } else {
       JXR_GLOBAL uint8_t *row = out + im.plane_offset[0] + (uint64_t)oy * im.row_stride[0];
-      const uint32_t pitch = im.channels * B;
-      JXR_GLOBAL uint8_t *px0 = row + (int64_t)ox0 * pitch;
-      const bool aligned = valid == 0xFu && (reinterpret_cast<uintptr_t>(px0) & 3u) == 0;
-      if (aligned && im.channels == 3) {
+      const uint32_t pitch = im.pixel_channels * B;
+      JXR_GLOBAL uint8_t *px0 = row + (int64_t)ox0 * pitch + im.channel_base * B;
+      const bool aligned = valid == 0xFu && (reinterpret_cast<uintptr_t>(px0) & 3u) == 0 && nst == im.pixel_channels;
+      if (aligned && nst == 3) {
         store_interleaved<B, 3>(px0, st);
-      } else if (aligned && im.channels == 4) {
+      } else if (aligned && nst == 4) {
         store_interleaved<B, 4>(px0, st);
       } else {
 #pragma unroll
@@ -1289,7 +1292,7 @@ __device__ __forceinline__ void output_group(const OutCtx &o, JXR_GLOBAL uint8_t
           if (!(valid & (1u << i))) continue;
 #pragma unroll
           for (uint32_t c = 0; c < 6; c++)
-            if (c < im.channels) store_sample<B>(px0 + i * pitch + c * B, st[c][i]);
+            if (c < nst) store_sample<B>(px0 + i * pitch + c * B, st[c][i]);
         }
       }
     }

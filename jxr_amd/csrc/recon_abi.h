@@ -6,7 +6,10 @@
 
 namespace jxr {
 
-constexpr uint32_t kMaxPlanes = 5;     // Y, U, V (or CMYK/K), alpha
+constexpr uint32_t kMaxPlanes = 5;     // planes of one work item: Y, U, V (or CMYK/K), alpha
+// Colour planes of an N-component image (DecodeLimits::max_components,
+// crates/jxr-core/src/request.rs:85); more than four run as plane groups.
+constexpr uint32_t kMaxColors = 64;
 constexpr uint32_t kThreads = 256;     // threads per workgroup (4 wave64)
 // Waves each SIMD must hold: four workgroups per compute unit (LDS allows
 // four 4:2:0 tiles of 4x4 macroblocks), so at most 128 VGPRs per lane.
@@ -104,6 +107,12 @@ struct alignas(64) DevImage {
   uint32_t out_kind;       // OutKind
   uint32_t out_depth;      // JXRHIP_DEPTH_* (packed field widths, one-bit polarity)
   uint32_t chroma_sub_y;   // planar YUV: 2 for 4:2:0 chroma rows, 1 otherwise
+  // plane groups of an N-component image: this descriptor's channels are
+  // channel_base .. channel_base + store_channels - 1 of a pixel of
+  // pixel_channels samples (0, channels and channels for every other image)
+  uint32_t pixel_channels;
+  uint32_t channel_base;
+  uint32_t store_channels;
   uint64_t plane_offset[kMaxPlanes];       // output surface planes
   uint64_t row_stride[kMaxPlanes];
   ChanScale color_scale[8];

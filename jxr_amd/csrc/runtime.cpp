@@ -114,7 +114,12 @@ struct jxrhip_plan {
   const jxrhip_arena *arenas[2] = {nullptr, nullptr};
   uint32_t arena_count = 0;
   std::vector<PlanPlane> planes;
-  DevImage dev{};  // device pointers filled per session at upload
+  DevImage dev{};  // the whole image's output policy; device pointers filled per session at upload
+  // Plane groups: one descriptor per group of planes a work item holds in
+  // LDS at once. Every image is one group, except N-component images of
+  // more than four colour planes (see jxrhip_plan_create).
+  std::vector<DevImage> groups;
+  std::vector<std::vector<uint32_t>> group_planes;  // plan plane of each group plane
   uint32_t mbx = 0, mby = 0;
   uint64_t output_bytes = 0, recon_coefs = 0, pixels = 0, alg_bytes = 0;
   uint64_t content_bytes = 0;  // bytes of the surface the output stage writes
@@ -302,7 +307,7 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
                           : (cf == JXRHIP_COLOR_CMYK || cf == JXRHIP_COLOR_CMYK_DIRECT || cf == JXRHIP_COLOR_YUVK)
                               ? 4
                               : colors;
-  if (colors == 0 || colors > 4 || colors != expect)
+  if (colors == 0 || colors > kMaxColors || colors != expect)
     return fail(JXRHIP_ERR_INVALID_PLAN, "primary component count does not match the colour format");
   const bool integrated = d->alpha_mode == JXRHIP_ALPHA_INTEGRATED;
   if (d->primary->plane_count != colors + (integrated ? 1 : 0))
@@ -337,7 +342,6 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
                                                            : (p.bc == 4 && p.br == 4);
     if (!ok) return fail(JXRHIP_ERR_INVALID_PLAN, "component macroblock geometry does not match the colour format");
   }
-  if (plan->planes.size() > kMaxPlanes) return fail(JXRHIP_ERR_INVALID_PLAN, "too many planes");
 
   // ---- output policy (device_plan/output.rs:102-295) ----
   const jxrhip_output_format &f = d->output;
@@ -573,11 +577,60 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
     const uint64_t out_px_bytes = plan->content_bytes;
     plan->alg_bytes = stored * 4 + mbs * sizeof(MbRec) + out_px_bytes;
   }
-  plan->descriptor_bytes += sizeof(DevImage);
-  // bucket key: plane geometries and the storage kind (one kernel
-  // instantiation per storage kind)
-  uint64_t key = plan->planes.size();
-  for (const PlanPlane &p : plan->planes) key = key * 16 + p.bc * 4 + p.br;
+  // ---- plane groups ----
+  // A work item holds up to kMaxPlanes planes of its tile in LDS, and its
+  // output stage reads up to four colour planes. An N-component image with
+  // more colour planes runs as groups of up to four colour
+  // planes, each a descriptor of its own that writes its channels of the
+  // interleaved pixel; the group with the last colour planes also stores
+  // alpha, and with premultiplied output every group carries the alpha
+  // plane, since each multiplies its channels by it. On a planar surface
+  // (at most JXRHIP_MAX_SURFACE_PLANES channels) a group writes its planes.
+  dv.pixel_channels = dv.channels;
+  dv.channel_base = 0;
+  dv.store_channels = dv.channels;
+  if (colors <= 4) {  // at most four colour planes and alpha: planes.size() <= kMaxPlanes
+    plan->groups.push_back(dv);
+    plan->group_planes.emplace_back();
+    for (uint32_t i = 0; i < plan->planes.size(); i++) plan->group_planes.back().push_back(i);
+  } else {
+    if (dv.conv != kConvDirect || dv.out_kind != kOutTyped) return fail(JXRHIP_ERR_INVALID_PLAN, "too many planes");
+    for (uint32_t g0 = 0; g0 < colors; g0 += 4) {
+      const uint32_t n = std::min<uint32_t>(4, colors - g0);
+      const bool last = g0 + n == colors;
+      DevImage gd = dv;
+      std::vector<uint32_t> idx;
+      for (uint32_t k = 0; k < n; k++) idx.push_back(g0 + k);
+      gd.colors = n;
+      gd.primary_count = n;
+      gd.has_alpha = 0;
+      gd.alpha_plane = -1;
+      if (dv.has_alpha && (last || dv.premultiply)) {
+        gd.alpha_plane = (int32_t)idx.size();
+        idx.push_back((uint32_t)alpha_plane);
+        gd.has_alpha = 1;
+      }
+      gd.plane_count = (uint32_t)idx.size();
+      gd.channels = n + gd.has_alpha;
+      gd.store_channels = n + (dv.has_alpha && last ? 1u : 0u);
+      gd.channel_base = g0;
+      if (dv.planar) {
+        // planar: the group's channels are surface planes g0 .. g0 + n (+ alpha)
+        for (uint32_t c = 0; c < kMaxPlanes; c++) {
+          gd.plane_offset[c] = g0 + c < kMaxPlanes ? dv.plane_offset[g0 + c] : 0;
+          gd.row_stride[c] = g0 + c < kMaxPlanes ? dv.row_stride[g0 + c] : 0;
+        }
+        gd.channel_base = 0;
+      }
+      plan->groups.push_back(gd);
+      plan->group_planes.push_back(std::move(idx));
+    }
+  }
+  plan->descriptor_bytes += sizeof(DevImage) * plan->groups.size();
+  // bucket key: plane geometries (of the first group, the largest) and the
+  // storage kind (one kernel instantiation per storage kind)
+  uint64_t key = plan->group_planes[0].size();
+  for (uint32_t i : plan->group_planes[0]) key = key * 16 + plan->planes[i].bc * 4 + plan->planes[i].br;
   plan->geometry_key = key * 16 + dv.storage;
   *out = plan.release();
   return JXRHIP_OK;
@@ -965,19 +1018,20 @@ static jxrhip_status device_image(jxrhip_session *s, const jxrhip_plan *p, hipSt
       return JXRHIP_OK;
     }
   }
-  // layout: DevImage | recs of each plane | masks
-  size_t bytes = sizeof(DevImage);
+  // layout: DevImage of each plane group | recs of each plane | masks
+  const size_t ngroups = p->groups.size();
+  size_t bytes = sizeof(DevImage) * ngroups;
   for (const PlanPlane &pp : p->planes)
     bytes += pp.recs.size() * sizeof(MbRec) + (pp.colmask.size() + pp.rowmask.size()) * 4 + 64;
   std::vector<uint8_t> host(bytes, 0);
   DeviceBuffer b;
   if (cacheable) evict_until(s, bytes);
   HIP_TRY(s->pool.get(bytes, b, s->diag.scratch_hits, s->diag.scratch_misses), "descriptor buffer");
-  DevImage dv = p->dev;
-  size_t off = sizeof(DevImage);
+  std::vector<DevPlane> dp(p->planes.size());
+  size_t off = sizeof(DevImage) * ngroups;
   for (size_t i = 0; i < p->planes.size(); i++) {
     const PlanPlane &pp = p->planes[i];
-    DevPlane &d = dv.planes[i];
+    DevPlane &d = dp[i];
     d.coef = coef[pp.arena];
     off = (off + 63) & ~size_t(63);
     std::memcpy(host.data() + off, pp.recs.data(), pp.recs.size() * sizeof(MbRec));
@@ -999,7 +1053,11 @@ static jxrhip_status device_image(jxrhip_session *s, const jxrhip_plan *p, hipSt
     d.overlap = pp.overlap;
     d.alpha = pp.alpha ? 1 : 0;
   }
-  std::memcpy(host.data(), &dv, sizeof dv);
+  for (size_t g = 0; g < ngroups; g++) {
+    DevImage dv = p->groups[g];
+    for (size_t k = 0; k < p->group_planes[g].size(); k++) dv.planes[k] = dp[p->group_planes[g][k]];
+    std::memcpy(host.data() + g * sizeof(DevImage), &dv, sizeof dv);
+  }
   HIP_TRY(hipMemcpyAsync(b.ptr, host.data(), bytes, hipMemcpyHostToDevice, stream), "descriptor upload");
   // pageable copies are staged before the call returns, so `host` may go
   HIP_TRY(hipStreamSynchronize(stream), "descriptor upload");
@@ -1057,11 +1115,17 @@ extern "C" jxrhip_status jxrhip_session_clear_cache(jxrhip_session *s) {
 // LDS words of a tile of tx x ty macroblocks: a sample region and a low
 // region for every plane.
 static uint32_t tile_words(const jxrhip_plan *p, uint32_t tx, uint32_t ty) {
-  uint32_t w = 0;
-  for (const PlanPlane &pp : p->planes)
-    w += sample_region_words(pp.bc, pp.br, tx, ty) + low_region_words(pp.bc, pp.br, pp.overlap, tx, ty) +
-         mb_table_words(tx, ty);
-  return w;
+  uint32_t most = 0;
+  for (const auto &gp : p->group_planes) {
+    uint32_t w = 0;
+    for (uint32_t i : gp) {
+      const PlanPlane &pp = p->planes[i];
+      w += sample_region_words(pp.bc, pp.br, tx, ty) + low_region_words(pp.bc, pp.br, pp.overlap, tx, ty) +
+           mb_table_words(tx, ty);
+    }
+    most = std::max(most, w);
+  }
+  return most;
 }
 
 static constexpr uint32_t kDefaultTileBudgetWords = 48 * 1024 / 4;
@@ -1161,7 +1225,6 @@ static jxrhip_status submit_impl(jxrhip_session *s, const jxrhip_plan *p, void *
   choose_tile(s, p, tx, ty, words);
   if ((size_t)words * 4 > s->max_lds) return undo(fail(JXRHIP_ERR_INVALID_PLAN, "tile does not fit in LDS"));
   KernelArgs args{};
-  args.inline_slot.img = dimg;
   args.inline_slot.out = (uint8_t *)dst;
   args.inline_slot.status = status_word;
   args.inline_slot.tiles_x = (p->mbx + tx - 1) / tx;
@@ -1176,15 +1239,18 @@ static jxrhip_status submit_impl(jxrhip_session *s, const jxrhip_plan *p, void *
   if ((e = hipMemsetAsync(status_word, 0, 4, sub->stream)) != hipSuccess) return undo(hip_fail(e, "status reset"));
   if ((e = clear_bit_rows(p, dst, sub->stream)) != hipSuccess) return undo(hip_fail(e, "one-bit rows"));
   // one item per workgroup; a multiple of 8 workgroups lets the kernel give
-  // each XCD a contiguous share of the tiles
-  if ((e = launch_tiles(args, (tiles + 7u) & ~7u, sub->stream)) != hipSuccess)
-    return undo(hip_fail(e, "reconstruction launch"));
+  // each XCD a contiguous share of the tiles. One launch per plane group.
+  for (size_t g = 0; g < p->groups.size(); g++) {
+    args.inline_slot.img = dimg + g;
+    if ((e = launch_tiles(args, (tiles + 7u) & ~7u, sub->stream)) != hipSuccess)
+      return undo(hip_fail(e, "reconstruction launch"));
+    s->diag.kernel_launches++;
+  }
   if ((e = hipMemcpyAsync(s->status_host + sub->slot, status_word, 4, hipMemcpyDeviceToHost, sub->stream)) !=
       hipSuccess)
     return undo(hip_fail(e, "status readback"));
   if ((e = hipEventRecord(sub->done, sub->stream)) != hipSuccess) return undo(hip_fail(e, "hipEventRecord"));
   s->diag.submissions++;
-  s->diag.kernel_launches++;
   session_ref(s);
   *out = sub.release();
   return JXRHIP_OK;
@@ -1450,7 +1516,8 @@ static void release_prepared(jxrhip_session *s, PreparedBatch &pb) {
 // Called with the session lock held; on failure the caller releases `pb`.
 static jxrhip_status prepare_persistent(jxrhip_session *s, const jxrhip_plan *const *plans, uint32_t count,
                                         void *const *dsts, const uint32_t *force, PreparedBatch &pb,
-                                        hipStream_t upload_stream = nullptr, Slot *pinned_slots = nullptr) {
+                                        hipStream_t upload_stream = nullptr, Slot *pinned_slots = nullptr,
+                                        size_t pinned_capacity = 0) {
   hipStream_t stream = upload_stream ? upload_stream : s->streams[0];
   pb.plans.assign(plans, plans + count);
   pb.dsts.assign(dsts, dsts + count);
@@ -1479,19 +1546,23 @@ static jxrhip_status prepare_persistent(jxrhip_session *s, const jxrhip_plan *co
     // can differ with the overlap mode
     for (uint32_t i : kv.second) words = std::max(words, tile_words(plans[i], tx, ty));
     if ((size_t)words * 4 > s->max_lds) return fail(JXRHIP_ERR_INVALID_PLAN, "tile does not fit in LDS");
-    BucketLaunch L{slots.size(), (uint32_t)kv.second.size(), 0, tx, ty, words, plans[kv.second[0]]->dev.storage, 0};
+    BucketLaunch L{slots.size(), 0, 0, tx, ty, words, plans[kv.second[0]]->dev.storage, 0};
     for (uint32_t i : kv.second) {
-      Slot sl{};
-      sl.img = dimgs[i];
-      sl.out = (uint8_t *)dsts[i];
-      sl.status = (uint32_t *)pb.statuses.ptr + i;
-      sl.tiles_x = (plans[i]->mbx + tx - 1) / tx;
-      sl.tile_start = L.items;
-      const uint64_t items = (uint64_t)L.items + (uint64_t)sl.tiles_x * ((plans[i]->mby + ty - 1) / ty);
-      if (items > UINT32_MAX / 2) return fail(JXRHIP_ERR_BUDGET, "too many work items in one shape bucket");
-      L.items = (uint32_t)items;
-      slots.push_back(sl);
+      // one slot per plane group; the groups of an image share its status word
+      for (size_t g = 0; g < plans[i]->groups.size(); g++) {
+        Slot sl{};
+        sl.img = dimgs[i] + g;
+        sl.out = (uint8_t *)dsts[i];
+        sl.status = (uint32_t *)pb.statuses.ptr + i;
+        sl.tiles_x = (plans[i]->mbx + tx - 1) / tx;
+        sl.tile_start = L.items;
+        const uint64_t items = (uint64_t)L.items + (uint64_t)sl.tiles_x * ((plans[i]->mby + ty - 1) / ty);
+        if (items > UINT32_MAX / 2) return fail(JXRHIP_ERR_BUDGET, "too many work items in one shape bucket");
+        L.items = (uint32_t)items;
+        slots.push_back(sl);
+      }
     }
+    L.n = (uint32_t)(slots.size() - L.first);
     // workgroups a compute unit holds at once (LDS, registers), unless the
     // session fixes it; a multiple of 8 workgroups for the XCD split
     int resident = 0;
@@ -1504,7 +1575,7 @@ static jxrhip_status prepare_persistent(jxrhip_session *s, const jxrhip_plan *co
   HIP_TRY(s->pool.get(std::max<size_t>(slots.size(), 1) * sizeof(Slot), pb.slots, s->diag.scratch_hits,
                       s->diag.scratch_misses),
           "slot buffer");
-  if (pinned_slots) {
+  if (pinned_slots && slots.size() <= pinned_capacity) {
     // pinned staging of the caller's, left alone until the stream passes
     // this copy: the copy is a plain DMA and the host does not wait for it
     std::copy(slots.begin(), slots.end(), pinned_slots);
@@ -1661,11 +1732,14 @@ extern "C" jxrhip_status jxrhip_submit_batch(jxrhip_session *s, const jxrhip_pla
   // one pinned block: the status words, then the staging of the slot table,
   // which the batch keeps (as it keeps the statuses) until its wait
   const size_t status_bytes = (sizeof(uint32_t) * std::max<uint32_t>(count, 1) + 255) & ~size_t(255);
-  if (hipHostMalloc((void **)&b->host_statuses, status_bytes + sizeof(Slot) * std::max<uint32_t>(count, 1),
-                    hipHostMallocDefault) != hipSuccess)
+  size_t slot_count = 0;  // one slot per plane group
+  for (uint32_t i = 0; i < count; i++) slot_count += plans[i]->groups.size();
+  slot_count = std::max<size_t>(slot_count, 1);
+  if (hipHostMalloc((void **)&b->host_statuses, status_bytes + sizeof(Slot) * slot_count, hipHostMallocDefault) !=
+      hipSuccess)
     return undo(fail(JXRHIP_ERR_DEVICE, "hipHostMalloc"));
   Slot *slot_staging = reinterpret_cast<Slot *>(reinterpret_cast<uint8_t *>(b->host_statuses) + status_bytes);
-  jxrhip_status st = prepare_persistent(s, plans, count, dsts, nullptr, b->pb, nullptr, slot_staging);
+  jxrhip_status st = prepare_persistent(s, plans, count, dsts, nullptr, b->pb, nullptr, slot_staging, slot_count);
   if (st) return undo(st);
   if (hipEventRecord(s->t0, stream) != hipSuccess) return undo(fail(JXRHIP_ERR_DEVICE, "hipEventRecord"));
   st = enqueue_persistent(s, b->pb);
@@ -2093,7 +2167,7 @@ static void server_launch_wave(jxrhip_server *sv, uint32_t slot) {
       dsts.push_back(r->dst ? r->dst : r->output.ptr);
     }
     if (!st) st = prepare_persistent(s, plans.data(), (uint32_t)n, dsts.data(), nullptr, w->pb, sv->compute[slot],
-                                     sv->host_slots[slot]);
+                                     sv->host_slots[slot], sv->cfg.max_wave_images);
     if (!st) st = enqueue_persistent(s, w->pb, sv->compute[slot]);
     if (!st && hipMemcpyAsync(sv->host_status[slot], w->pb.statuses.ptr, sizeof(uint32_t) * n,
                               hipMemcpyDeviceToHost, sv->compute[slot]) != hipSuccess)

@@ -1067,7 +1067,10 @@ static int reconstruct(plane_t *p, uint32_t *stage) {
 /* ---------------------------------------------------------------------- */
 /* image: planes, output formatting                                       */
 /* ---------------------------------------------------------------------- */
-#define MAXP 6
+/* colour planes of an N-component image (DecodeLimits::max_components,
+ * crates/jxr-core/src/request.rs:85), plus alpha */
+#define MAX_COLORS 64
+#define MAXP (MAX_COLORS + 1)
 typedef struct {
   plane_t p[MAXP];
   uint32_t count;      /* planes reconstructed                          */
@@ -1093,7 +1096,7 @@ static int image_setup(image_t *im, const jxrhip_image_desc *d, char *msg, size_
                         ? 3
                     : (cf == JXRHIP_COLOR_CMYK || cf == JXRHIP_COLOR_CMYK_DIRECT || cf == JXRHIP_COLOR_YUVK) ? 4
                     : colors;
-  if (colors != expect || colors == 0 || colors > 4)
+  if (colors != expect || colors == 0 || colors > MAX_COLORS)
     FAIL(JXRHIP_ERR_INVALID_PLAN, "primary component count does not match the colour format");
   im->colors = colors;
   uint32_t integrated = d->alpha_mode == JXRHIP_ALPHA_INTEGRATED;
@@ -1146,7 +1149,7 @@ typedef struct {
   int reverse;
   uint32_t channels;
   uint32_t bytes;
-  channel_scale color_scale[8];
+  channel_scale color_scale[MAXP];
   channel_scale alpha_scale;
   int32_t u16_max;
   int upsample; /* 0 none, 1 = 4:2:0, 2 = 4:2:2 */
@@ -1372,7 +1375,7 @@ static int output_setup(out_ctx *o, const jxrhip_image_desc *d, image_t *im, cha
   o->upsample = o->kind == OUT_YUV ? 0 : cf == JXRHIP_COLOR_YUV420 ? 1 : cf == JXRHIP_COLOR_YUV422 ? 2 : 0;
   if (o->upsample && (f->chroma_centering_x > 4 || (o->upsample == 1 && f->chroma_centering_y > 4)))
     FAIL(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "unknown chroma centering has no exact reconstruction filter");
-  for (uint32_t c = 0; c < 8; c++) {
+  for (uint32_t c = 0; c < MAXP; c++) {
     o->color_scale[c] = resolve_scale(c, f->output_color, dep, f->shift_bits, f->scaled);
     if ((l == JXRHIP_LAYOUT_RGBX || l == JXRHIP_LAYOUT_BGRX) && c == 3) o->color_scale[c].padding = 1;
   }
@@ -1387,7 +1390,8 @@ static int output_setup(out_ctx *o, const jxrhip_image_desc *d, image_t *im, cha
   if ((o->kind == OUT_BITS || o->kind == OUT_PACKED16 || o->kind == OUT_RGB101010 || o->kind == OUT_RGBE) &&
       s->planar)
     FAIL(JXRHIP_ERR_INVALID_PLAN, "packed and one-bit outputs are interleaved");
-  if (s->plane_count != need) FAIL(JXRHIP_ERR_INVALID_PLAN, "surface plane count");
+  if (s->plane_count != need || need > JXRHIP_MAX_SURFACE_PLANES)
+    FAIL(JXRHIP_ERR_INVALID_PLAN, "surface plane count");
   for (uint32_t i = 0; i < need; i++) {
     const jxrhip_surface_plane *sp = &s->planes[i];
     uint32_t ch = s->planar ? 1 : o->channels;
@@ -1564,7 +1568,7 @@ static int format_output(const out_ctx *o, uint8_t *out) {
   for (uint32_t y = 0; y < f->crop_height; y++)
     for (uint32_t x = 0; x < f->crop_width; x++) {
       uint32_t sx = f->crop_x + x, sy = f->crop_y + y;
-      int32_t ordered[6] = {0, 0, 0, 0, 0, 0};
+      int32_t ordered[MAXP + 1] = {0};
       if (ncomp) {
         for (uint32_t c = 0; c < o->primary_count; c++) ordered[c] = plane_at(&im->p[c], sx, sy);
       } else {
@@ -1581,7 +1585,7 @@ static int format_output(const out_ctx *o, uint8_t *out) {
         }
       }
       if (o->has_alpha) ordered[o->primary_count] = plane_at(&im->p[im->alpha_index], sx, sy);
-      uint64_t stored[6];
+      uint64_t stored[MAXP + 1];
       for (uint32_t c = 0; c < o->channels; c++) {
         int alpha = o->has_alpha && c == o->primary_count;
         const channel_scale *cs = alpha ? &o->alpha_scale : &o->color_scale[c];

@@ -106,6 +106,4 @@ def formatted_row_desc(v):
 
 @pytest.mark.parametrize("v", GOLDEN["formatted_rows"], ids=lambda v: v["source"].split("(")[-1].rstrip(")"))
 def test_formatted_row_vectors(v):
-    if len(v["components"]) > 4:
-        pytest.skip("N-component images of more than four planes are not decoded yet")
     assert O.decode(formatted_row_desc(v)).tolist() == v["expected_u8"], v["source"]

@@ -156,3 +156,20 @@ def test_coefficient_plane_range_that_wraps_is_refused_by_both():
     a.planes[0].coefficient_count = a.coef.size + 1  # past the arena, no wrap
     assert "coefficient plane range" in refused(luma_desc(a))
     assert O.decode_status(luma_desc(a))[0] == abi.ERR_INVALID_PLAN
+
+
+def test_n_component_images_up_to_the_reference_limit():
+    # DecodeLimits::max_components = 64 (crates/jxr-core/src/request.rs:85);
+    # more than four colour planes run as plane groups on the device
+    for n, ok in ((64, True), (65, False)):
+        a = ManualArena([np.zeros((1, 2), np.int64)] * n)
+        spec = OutputSpec(storage=abi.STORE_U8, depth=abi.DEPTH_U8, layout=abi.LAYOUT_NCOMPONENT,
+                          output_color=abi.COLOR_NCOMPONENT, components=n)
+        d = desc_for(a, abi.COLOR_NCOMPONENT, n, spec, (0, 0, 32, 16))
+        if ok:
+            plan = HipDecodePlan(d, keepalive=a)
+            assert plan.output_bytes == 32 * 16 * n
+            assert (O.decode(d) == 128).all()
+        else:
+            refused(d)
+            assert O.decode_status(d)[0] == abi.ERR_INVALID_PLAN
