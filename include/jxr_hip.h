@@ -394,6 +394,26 @@ jxrhip_status jxrhip_batch_wait(jxrhip_batch *batch, uint32_t *stages);
 jxrhip_status jxrhip_last_batch_timing(jxrhip_session *session, double *milliseconds,
                                        uint32_t *launches);
 
+/* Allocation-free persistent batch. No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture
+ * set-up, tight serving loops, harnesses that forbid hipMalloc). Decodes
+ * like jxrhip_submit_batch(..., JXRHIP_BATCH_PERSISTENT) + jxrhip_batch_wait
+ * and returns when the images are done, writing each image's stage code
+ * (0 = ok) to stages[count] when non-NULL. Its scratch (status words, share
+ * counters, slot records, item table) is a static device array of 4 MiB
+ * that every device the library runs on holds until the process ends; the
+ * call allocates no device or pinned memory. So every plan's coefficients
+ * and descriptors must be resident already (jxrhip_session_upload, or an
+ * earlier submission), else JXRHIP_ERR_INVALID_ARGUMENT. Limits: 4096
+ * images, 8192 plane groups, 64 shape buckets and 962,560 work items per
+ * call; beyond them JXRHIP_ERR_BUDGET. The array is shared by every call on
+ * a device and keeps the last call's contents (each call clears what must
+ * start at zero). Calls that overlapped on one device would race for it, so
+ * the library runs these calls one at a time in a process: a call made while
+ * another runs waits for it. */
+jxrhip_status jxrhip_decode_batch_static(jxrhip_session *session, const jxrhip_plan *const *plans,
+                                         uint32_t count, void *const *device_destinations, uint32_t *stages);
+
 /* ---- tuning ------------------------------------------------------------ */
 /* Preferred macroblock tile of one work item (0 = automatic). A shape whose
  * LDS does not fit a workgroup is halved along its longer side until it

@@ -312,6 +312,7 @@ SIGNATURES = {
     "jxrhip_readback": (st, [vp, vp, vp, C.c_uint64]),
     "jxrhip_decode_to_host": (st, [vp, vp, vp, C.c_uint64, u32p]),
     "jxrhip_submit_batch": (st, [vp, C.POINTER(vp), C.c_uint32, C.POINTER(vp), C.c_uint32, pp]),
+    "jxrhip_decode_batch_static": (st, [vp, C.POINTER(vp), C.c_uint32, C.POINTER(vp), u32p]),
     "jxrhip_batch_is_complete": (C.c_int32, [vp]),
     "jxrhip_batch_wait": (st, [vp, u32p]),
     "jxrhip_last_batch_timing": (st, [vp, C.POINTER(C.c_double), u32p]),

@@ -36,6 +36,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/jxr_hip.h"
 #include "recon_abi.h"
 
@@ -2072,6 +2074,32 @@ __global__ void __launch_bounds__(kThreads, kMinWavesPerSimd) jxr_reconstruct_ti
     __syncthreads();
     item = shared_item;
   }
+}
+
+// ---- scratch of the allocation-free batch entry point -------------------------
+// A file-scope device array, so that jxrhip_decode_batch_static allocates
+// nothing: every device the library runs on holds its own copy until the
+// process ends. Host code never passes `scratch` itself to a kernel (in host
+// code the name stands for a host-side shadow): it asks
+// static_scratch_address for the device's address once and uses that.
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+// Device address of `scratch` on the current device (`device`), looked up
+// with hipGetSymbolAddress the first time for each device and kept.
+hipError_t static_scratch_address(int device, uint8_t **out) {
+  constexpr int kDevices = 64;
+  static std::mutex m;
+  static uint8_t *addr[kDevices] = {};
+  if (device < 0 || device >= kDevices) return hipErrorInvalidValue;
+  std::lock_guard<std::mutex> g(m);
+  if (!addr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = static_cast<uint8_t *>(p);
+  }
+  *out = addr[device];
+  return hipSuccess;
 }
 
 // ---- host-side launch wrappers ----------------------------------------------

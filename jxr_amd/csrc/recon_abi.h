@@ -141,6 +141,20 @@ struct KernelArgs {
   uint32_t storage;      // JXRHIP_STORE_* of every image of the launch
 };
 
+// Scratch of the allocation-free batch entry point, a static device array of
+// kStaticScratchBytes per device: status words, the share counters of each
+// launch (eight 128-byte lines), slot records, then the item table.
+constexpr uint32_t kStaticMaxImages = 4096;
+constexpr uint32_t kStaticMaxSlots = 8192;    // one per plane group of an image
+constexpr uint32_t kStaticMaxLaunches = 64;   // shape buckets of one call
+constexpr size_t kStaticScratchBytes = size_t(4) << 20;
+constexpr size_t kStaticStatusOffset = 0;
+constexpr size_t kStaticCounterOffset = kStaticStatusOffset + kStaticMaxImages * 4;
+constexpr size_t kStaticSlotOffset = kStaticCounterOffset + kStaticMaxLaunches * 8 * 128;
+constexpr size_t kStaticItemOffset = kStaticSlotOffset + kStaticMaxSlots * sizeof(Slot);
+constexpr uint32_t kStaticMaxItems = (uint32_t)((kStaticScratchBytes - kStaticItemOffset) / 4);
+static_assert(kStaticSlotOffset % 256 == 0 && kStaticItemOffset % 256 == 0, "scratch regions stay aligned");
+
 // Halo of the low-pass region in macroblocks: one macroblock holds the DC of
 // every block in the one-block halo of the second level; the subsampled
 // first-level overlap (mode 2) reads a corner's neighbour one pair further.

@@ -44,6 +44,7 @@ hipError_t kernel_occupancy(uint32_t storage, size_t dynamic_lds, int *groups_pe
 hipError_t set_kernel_lds_limit(size_t bytes);
 hipError_t launch_valu_chains(uint32_t *out, uint32_t groups, uint32_t iterations, hipStream_t stream);
 uint32_t valu_per_iteration();
+hipError_t static_scratch_address(int device, uint8_t **out);
 }  // namespace jxr
 
 using namespace jxr;
@@ -755,6 +756,8 @@ struct jxrhip_session {
   // mirror) and reusable completion events, so a submission allocates nothing
   uint32_t *status_dev = nullptr;
   uint32_t *status_host = nullptr;
+  // the device's static scratch array (allocation-free batches)
+  uint8_t *scratch = nullptr;
   std::vector<uint32_t> free_status;
   std::vector<hipEvent_t> free_events;
 };
@@ -840,6 +843,7 @@ extern "C" jxrhip_status jxrhip_session_create(int32_t ordinal, jxrhip_session *
     (void)hipGetLastError();
     s->max_lds = 65536;
   }
+  HIP_TRY(static_scratch_address(ordinal, &s->scratch), "static scratch address");
   HIP_TRY(hipMalloc((void **)&s->status_dev, kStatusSlots * sizeof(uint32_t)), "status words");
   HIP_TRY(hipHostMalloc((void **)&s->status_host, kStatusSlots * sizeof(uint32_t), hipHostMallocDefault),
           "status words");
@@ -943,6 +947,20 @@ static void evict_until(jxrhip_session *s, uint64_t need) {
   }
 }
 
+// Orders work queued next on `stream` after the copy that filled a cached
+// coefficient entry, if that copy may still be running on another stream.
+static hipError_t await_upload(jxrhip_session *s, CacheEntry &ce, hipStream_t stream) {
+  if (!ce.ready) return hipSuccess;
+  const hipError_t q = hipEventQuery(ce.ready);
+  if (q == hipSuccess) {
+    s->free_events.push_back(ce.ready);
+    ce.ready = nullptr;
+    return hipSuccess;
+  }
+  if (q != hipErrorNotReady) return q;
+  return ce.stream == stream ? hipSuccess : hipStreamWaitEvent(stream, ce.ready, 0);
+}
+
 // Device-resident coefficients of arena `a` (cached by identity when set).
 static jxrhip_status device_coefficients(jxrhip_session *s, const jxrhip_arena *a, hipStream_t stream,
                                          const int32_t **out, jxrhip_submission *transient_owner) {
@@ -951,18 +969,7 @@ static jxrhip_status device_coefficients(jxrhip_session *s, const jxrhip_arena *
     auto it = s->coef_cache.find(a->identity);
     if (it != s->coef_cache.end()) {
       CacheEntry &ce = it->second;
-      if (ce.ready) {
-        // the copy that filled the entry may still be running on its stream
-        const hipError_t q = hipEventQuery(ce.ready);
-        if (q == hipSuccess) {
-          s->free_events.push_back(ce.ready);
-          ce.ready = nullptr;
-        } else if (q == hipErrorNotReady) {
-          if (ce.stream != stream) HIP_TRY(hipStreamWaitEvent(stream, ce.ready, 0), "upload dependency");
-        } else {
-          return hip_fail(q, "upload event");
-        }
-      }
+      HIP_TRY(await_upload(s, ce, stream), "upload dependency");
       ce.last_use = ++s->tick;
       s->diag.upload_hits++;
       *out = (const int32_t *)ce.buf.ptr;
@@ -1824,6 +1831,163 @@ extern "C" jxrhip_status jxrhip_last_batch_timing(jxrhip_session *s, double *ms,
   if (ms) *ms = s->last_ms;
   if (launches) *launches = s->last_launches;
   return JXRHIP_OK;
+}
+
+// ---- allocation-free batches -------------------------------------------------
+// The device descriptor of a plan whose coefficients and descriptors are in
+// the session's caches already; allocates nothing, and fails when any of
+// them is missing.
+static jxrhip_status resident_image(jxrhip_session *s, const jxrhip_plan *p, hipStream_t stream,
+                                    const DevImage **out) {
+  const void *coef[2] = {nullptr, nullptr};
+  for (uint32_t a = 0; a < p->arena_count; a++) {
+    const uint64_t id = p->arenas[a]->identity;
+    auto it = id ? s->coef_cache.find(id) : s->coef_cache.end();
+    if (it == s->coef_cache.end())
+      return fail(JXRHIP_ERR_INVALID_ARGUMENT, "coefficients are not resident (jxrhip_session_upload first)");
+    HIP_TRY(await_upload(s, it->second, stream), "upload dependency");
+    it->second.last_use = ++s->tick;
+    coef[a] = it->second.buf.ptr;
+  }
+  auto it = s->desc_cache.find(p->id);
+  if (it == s->desc_cache.end() || it->second.deps[0] != coef[0] || it->second.deps[1] != coef[1])
+    return fail(JXRHIP_ERR_INVALID_ARGUMENT, "descriptors are not resident (jxrhip_session_upload first)");
+  it->second.last_use = ++s->tick;
+  *out = (const DevImage *)it->second.buf.ptr;
+  return JXRHIP_OK;
+}
+
+// Host staging of one allocation-free call: fixed arrays, so the call
+// allocates nothing on the host either. One call at a time in the process.
+namespace {
+struct StaticStaging {
+  std::mutex m;
+  const DevImage *dimgs[kStaticMaxImages];
+  uint32_t bucket_of[kStaticMaxImages];
+  uint64_t keys[kStaticMaxLaunches];
+  Slot slots[kStaticMaxSlots];
+  uint32_t statuses[kStaticMaxImages];
+};
+StaticStaging g_static;
+}  // namespace
+
+extern "C" jxrhip_status jxrhip_decode_batch_static(jxrhip_session *s, const jxrhip_plan *const *plans,
+                                                    uint32_t count, void *const *dsts, uint32_t *stages) {
+  if (!s || (count && (!plans || !dsts))) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  if (count > kStaticMaxImages)
+    return fail(JXRHIP_ERR_BUDGET, "more than " + std::to_string(kStaticMaxImages) +
+                                       " images in one allocation-free batch");
+  for (uint32_t i = 0; i < count; i++)
+    if (!plans[i] || !dsts[i]) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null plan or destination");
+  if (count == 0) return JXRHIP_OK;
+  std::lock_guard<std::mutex> host(g_static.m);
+  std::lock_guard<std::mutex> g(s->lock);
+  make_current(s);
+  if (!s->scratch) return fail(JXRHIP_ERR_DEVICE, "no static scratch on this device");
+  hipStream_t s0 = s->streams[0];
+  StaticStaging &st = g_static;
+  // shape buckets, in order of first appearance
+  uint32_t nb = 0;
+  size_t nslots = 0;
+  for (uint32_t i = 0; i < count; i++) {
+    jxrhip_status r = resident_image(s, plans[i], s0, &st.dimgs[i]);
+    if (r) return r;
+    uint32_t b = 0;
+    while (b < nb && st.keys[b] != plans[i]->geometry_key) b++;
+    if (b == nb) {
+      if (nb == kStaticMaxLaunches)
+        return fail(JXRHIP_ERR_BUDGET, "more than " + std::to_string(kStaticMaxLaunches) +
+                                           " shape buckets in one allocation-free batch");
+      st.keys[nb++] = plans[i]->geometry_key;
+    }
+    st.bucket_of[i] = b;
+    nslots += plans[i]->groups.size();
+  }
+  if (nslots > kStaticMaxSlots)
+    return fail(JXRHIP_ERR_BUDGET, "more than " + std::to_string(kStaticMaxSlots) +
+                                       " plane groups in one allocation-free batch");
+  uint32_t *status_dev = reinterpret_cast<uint32_t *>(s->scratch + kStaticStatusOffset);
+  Slot *slot_dev = reinterpret_cast<Slot *>(s->scratch + kStaticSlotOffset);
+  uint32_t *item_dev = reinterpret_cast<uint32_t *>(s->scratch + kStaticItemOffset);
+  BucketLaunch launches[kStaticMaxLaunches];
+  size_t next_slot = 0;
+  uint64_t total_items = 0;
+  for (uint32_t b = 0; b < nb; b++) {
+    const jxrhip_plan *first = nullptr;
+    for (uint32_t i = 0; i < count && !first; i++)
+      if (st.bucket_of[i] == b) first = plans[i];
+    uint32_t tx, ty, words;
+    choose_tile(s, first, tx, ty, words);
+    for (uint32_t i = 0; i < count; i++)
+      if (st.bucket_of[i] == b) words = std::max(words, tile_words(plans[i], tx, ty));
+    if ((size_t)words * 4 > s->max_lds) return fail(JXRHIP_ERR_INVALID_PLAN, "tile does not fit in LDS");
+    BucketLaunch L{next_slot, 0, 0, tx, ty, words, first->dev.storage, 0};
+    for (uint32_t i = 0; i < count; i++) {
+      if (st.bucket_of[i] != b) continue;
+      for (size_t gi = 0; gi < plans[i]->groups.size(); gi++) {
+        Slot &sl = st.slots[next_slot++];
+        sl = Slot{};
+        sl.img = st.dimgs[i] + gi;
+        sl.out = (uint8_t *)dsts[i];
+        sl.status = status_dev + i;
+        sl.tiles_x = (plans[i]->mbx + tx - 1) / tx;
+        sl.tile_start = L.items;
+        L.items += sl.tiles_x * ((plans[i]->mby + ty - 1) / ty);
+        if (total_items + L.items > kStaticMaxItems)
+          return fail(JXRHIP_ERR_BUDGET, "more than " + std::to_string(kStaticMaxItems) +
+                                             " work items in one allocation-free batch");
+      }
+    }
+    L.n = (uint32_t)(next_slot - L.first);
+    int resident = 0;
+    if (!s->groups_per_cu) HIP_TRY(kernel_occupancy(L.storage, (size_t)L.words * 4, &resident), "occupancy");
+    const uint32_t per_cu = s->groups_per_cu ? s->groups_per_cu : (uint32_t)std::max(resident, 1);
+    L.grid = std::max<uint32_t>((std::min<uint32_t>(L.items, s->cus * per_cu) + 7u) & ~7u, 8u);
+    total_items += L.items;
+    launches[b] = L;
+  }
+  // the slot table goes up from the host staging; everything after it is
+  // kernels on the same stream, and the call waits for all of it
+  HIP_TRY(hipMemcpyAsync(slot_dev, st.slots, next_slot * sizeof(Slot), hipMemcpyHostToDevice, s0), "slot upload");
+  for (uint32_t i = 0; i < count; i++) HIP_TRY(clear_bit_rows(plans[i], dsts[i], s0), "one-bit rows");
+  HIP_TRY(hipEventRecord(s->t0, s0), "hipEventRecord");
+  uint64_t item_base = 0;
+  for (uint32_t b = 0; b < nb; b++) {
+    const BucketLaunch &L = launches[b];
+    KernelArgs args{};
+    args.slots = slot_dev + L.first;
+    args.item_slot = item_dev + item_base;
+    args.counters = reinterpret_cast<uint32_t *>(s->scratch + kStaticCounterOffset + (size_t)b * 8 * 128);
+    item_base += L.items;
+    args.slot_count = L.n;
+    args.item_count = L.items;
+    args.persistent = 1;
+    args.tile_x = L.tx;
+    args.tile_y = L.ty;
+    args.lds_words = L.words;
+    args.storage = L.storage;
+    // the item-table kernel also zeroes the share counters and the status words
+    HIP_TRY(launch_expand(args.slots, L.n, L.items, (uint32_t *)args.item_slot, args.counters, s0), "item table");
+    HIP_TRY(launch_tiles(args, L.grid, s0), "persistent launch");
+    s->diag.kernel_launches++;
+  }
+  HIP_TRY(hipEventRecord(s->t1, s0), "hipEventRecord");
+  HIP_TRY(hipStreamSynchronize(s0), "hipStreamSynchronize");
+  HIP_TRY(hipMemcpy(st.statuses, status_dev, sizeof(uint32_t) * count, hipMemcpyDeviceToHost), "status readback");
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, s->t0, s->t1) == hipSuccess) s->last_ms = ms;
+  s->last_launches = nb;
+  s->diag.submissions += count;
+  jxrhip_status result = JXRHIP_OK;
+  for (uint32_t i = 0; i < count; i++) {
+    if (stages) stages[i] = st.statuses[i];
+    if (st.statuses[i] && result == JXRHIP_OK) {
+      result = fail(JXRHIP_ERR_KERNEL_ARITHMETIC, "image " + std::to_string(i) +
+                                                      ": reconstruction arithmetic overflow (stage " +
+                                                      std::to_string(st.statuses[i]) + ")");
+    }
+  }
+  return result;
 }
 
 // ---- captured batches -----------------------------------------------------

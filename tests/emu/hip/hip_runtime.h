@@ -23,6 +23,8 @@
 #define __forceinline__ inline
 #define __shared__ static
 #define __launch_bounds__(...)
+#define __align__(n) __attribute__((aligned(n)))
+#define HIP_SYMBOL(x) (&(x))
 #define JXR_GLOBAL
 
 struct dim3 {
@@ -131,6 +133,11 @@ hipError_t hipMemcpyAsync(void *dst, const void *src, size_t bytes, hipMemcpyKin
 hipError_t hipMemsetAsync(void *dst, int value, size_t bytes, hipStream_t s);
 hipError_t hipMemset2DAsync(void *dst, size_t pitch, int value, size_t width, size_t height, hipStream_t s);
 hipError_t hipFuncSetAttribute(const void *f, hipFuncAttribute attr, int value);
+// a device symbol's address: the host array itself in the CPU build
+inline hipError_t hipGetSymbolAddress(void **p, const void *symbol) {
+  *p = const_cast<void *>(symbol);
+  return hipSuccess;
+}
 hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *blocks, const void *f, int threads, size_t lds);
 
 // Graph capture: while a capture is open, launches (on any stream, as the

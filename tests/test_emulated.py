@@ -29,3 +29,13 @@ def test_emulated_session_lifetime_and_reuploads_under_asan():
                          capture_output=True, text=True, env=env, timeout=600)
     assert res.returncode == 0, res.stdout[-3000:] + res.stderr[-3000:]
     assert "0 failures" in res.stdout
+
+
+def test_emulated_allocation_free_batches_under_asan():
+    """jxrhip_decode_batch_static on the CPU build (tests/emu_no_alloc.py)."""
+    emu_lib.build(asan=True)
+    env = dict(os.environ, LD_PRELOAD=emu_lib.asan_runtime(), ASAN_OPTIONS="detect_leaks=0")
+    res = subprocess.run([sys.executable, os.path.join(HERE, "emu_no_alloc.py"), "--asan"],
+                         capture_output=True, text=True, env=env, timeout=600)
+    assert res.returncode == 0, res.stdout[-3000:] + res.stderr[-3000:]
+    assert "0 failures" in res.stdout
