@@ -332,6 +332,70 @@ jxrhip_status jxrhip_submission_wait(jxrhip_submission *submission, jxrhip_image
 /* Frees a submission, waiting first (Drop, submission.rs:132-139). */
 void jxrhip_submission_destroy(jxrhip_submission *submission);
 
+/* Consumers of a pending submission, with no host wait
+ * (CudaSubmission::{enqueue_consumer_wait, output_layout,
+ * pending_device_buffer}, submission.rs:50-93).
+ * enqueue_wait: work queued next on `hip_stream` (a hipStream_t of the
+ *   session's device and of the HIP runtime this library is bound to; NULL
+ *   is the null stream) runs after the submission's reconstruction. A
+ *   process that loads PyTorch-ROCm before this library binds both to the
+ *   HIP runtime torch loaded (one soname), so torch's streams qualify.
+ * output_layout: the validated surface layout of the output, before
+ *   completion.
+ * pending_device_ptr: the output buffer of a jxrhip_submit submission (not
+ *   of submit_into, whose destination the caller holds). A consumer stream
+ *   may read it after enqueue_wait, and must be done with it before the
+ *   submission is waited for or destroyed (a wait hands the buffer to the
+ *   result image, which keeps it). */
+jxrhip_status jxrhip_submission_enqueue_wait(const jxrhip_submission *submission, void *hip_stream);
+jxrhip_status jxrhip_submission_output_layout(const jxrhip_submission *submission, jxrhip_surface_layout *out);
+jxrhip_status jxrhip_submission_pending_device_ptr(const jxrhip_submission *submission, void **device_ptr,
+                                                   uint64_t *bytes);
+
+/* ---- decode reports (crates/jxr-core/src/report.rs:9-70) --------------- */
+/* DecodeStage */
+#define JXRHIP_DECODE_STAGE_PARSE 0u
+#define JXRHIP_DECODE_STAGE_ENTROPY_DECODE 1u
+#define JXRHIP_DECODE_STAGE_INVERSE_SCAN 2u
+#define JXRHIP_DECODE_STAGE_COEFFICIENT_REMAP 3u
+#define JXRHIP_DECODE_STAGE_DC_LOW_PASS_PREDICTION 4u
+#define JXRHIP_DECODE_STAGE_DEQUANTIZE_AND_FIRST_INVERSE_TRANSFORM 5u
+#define JXRHIP_DECODE_STAGE_FIRST_OVERLAP 6u
+#define JXRHIP_DECODE_STAGE_HIGH_PASS_PREDICTION 7u
+#define JXRHIP_DECODE_STAGE_SECOND_INVERSE_TRANSFORM 8u
+#define JXRHIP_DECODE_STAGE_SECOND_OVERLAP 9u
+#define JXRHIP_DECODE_STAGE_CHROMA_RECONSTRUCTION 10u
+#define JXRHIP_DECODE_STAGE_COLOR_AND_ALPHA_CONVERSION 11u
+#define JXRHIP_DECODE_STAGE_CROP_CLIP_AND_PACK 12u
+#define JXRHIP_DECODE_STAGE_HOST_READBACK 13u
+/* StageExecutor; HIP takes the place of the reference's Cuda */
+#define JXRHIP_EXECUTOR_CPU_SCALAR 0u
+#define JXRHIP_EXECUTOR_CPU_SIMD 1u
+#define JXRHIP_EXECUTOR_METAL 2u
+#define JXRHIP_EXECUTOR_HIP 3u
+/* BackendRequest / BackendKind (the caller's request, passed through) */
+#define JXRHIP_BACKEND_AUTO 0u
+#define JXRHIP_BACKEND_CPU 1u
+#define JXRHIP_BACKEND_HIP 2u
+/* FallbackReason (none for a device decode) */
+#define JXRHIP_FALLBACK_NONE 0u
+typedef struct jxrhip_stage_report {
+  uint32_t stage;    /* JXRHIP_DECODE_STAGE_* */
+  uint32_t executor; /* JXRHIP_EXECUTOR_*     */
+} jxrhip_stage_report;
+typedef struct jxrhip_decode_report {
+  uint32_t requested; /* JXRHIP_BACKEND_* */
+  uint32_t selected;  /* JXRHIP_BACKEND_HIP */
+  uint32_t fallback;  /* JXRHIP_FALLBACK_NONE */
+  uint32_t stage_count;
+  jxrhip_stage_report stages[14];
+} jxrhip_decode_report;
+/* The route and per-stage ownership of a device decode of `plan`
+ * (CudaDecodePlan::decode_report, plan.rs:214-253); `host_readback`
+ * appends the readback stage, as decode_to_host reports. */
+jxrhip_status jxrhip_plan_decode_report(const jxrhip_plan *plan, uint32_t requested, int32_t host_readback,
+                                        jxrhip_decode_report *out);
+
 /* Device destinations (CudaDecoderSession::allocate_destination /
  * allocate_batch_destination, session.rs:85-101): plain device memory of the
  * session's device, usable as submit_into / submit_batch targets. A dense
@@ -385,6 +449,10 @@ jxrhip_status jxrhip_submit_batch(jxrhip_session *session, const jxrhip_plan *co
                                   uint32_t count, void *const *device_destinations,
                                   uint32_t mode, jxrhip_batch **out);
 int32_t jxrhip_batch_is_complete(const jxrhip_batch *batch);
+/* A device-side wait on `hip_stream` for every image of the batch (as
+ * jxrhip_submission_enqueue_wait); the destinations may then be read on
+ * that stream, with no host wait. */
+jxrhip_status jxrhip_batch_enqueue_wait(const jxrhip_batch *batch, void *hip_stream);
 /* Waits, frees the batch, and writes each image's stage code (0 = ok) to
  * stages[count] when non-NULL. Returns JXRHIP_ERR_KERNEL_ARITHMETIC if any
  * image failed; siblings are unaffected. */

@@ -176,6 +176,29 @@ class Diagnostics(C.Structure):
     ]
 
 
+class StageReport(C.Structure):
+    _fields_ = [("stage", C.c_uint32), ("executor", C.c_uint32)]
+
+
+class DecodeReport(C.Structure):
+    _fields_ = [
+        ("requested", C.c_uint32),
+        ("selected", C.c_uint32),
+        ("fallback", C.c_uint32),
+        ("stage_count", C.c_uint32),
+        ("stages", StageReport * 14),
+    ]
+
+
+# DecodeStage / StageExecutor / BackendKind codes (include/jxr_hip.h)
+DECODE_STAGES = ["Parse", "EntropyDecode", "InverseScan", "CoefficientRemap", "DcLowPassPrediction",
+                 "DequantizeAndFirstInverseTransform", "FirstOverlap", "HighPassPrediction",
+                 "SecondInverseTransform", "SecondOverlap", "ChromaReconstruction", "ColorAndAlphaConversion",
+                 "CropClipAndPack", "HostReadback"]
+EXECUTORS = ["CpuScalar", "CpuSimd", "Metal", "Hip"]
+BACKEND_AUTO, BACKEND_CPU, BACKEND_HIP = 0, 1, 2
+
+
 class TileChoice(C.Structure):
     _fields_ = [
         ("bucket_key", C.c_uint64),
@@ -300,6 +323,11 @@ SIGNATURES = {
     "jxrhip_submission_is_complete": (C.c_int32, [vp]),
     "jxrhip_submission_wait": (st, [vp, pp, u32p]),
     "jxrhip_submission_destroy": (None, [vp]),
+    "jxrhip_submission_enqueue_wait": (st, [vp, vp]),
+    "jxrhip_submission_output_layout": (st, [vp, C.POINTER(SurfaceLayout)]),
+    "jxrhip_submission_pending_device_ptr": (st, [vp, pp, C.POINTER(C.c_uint64)]),
+    "jxrhip_plan_decode_report": (st, [vp, C.c_uint32, C.c_int32, C.POINTER(DecodeReport)]),
+    "jxrhip_batch_enqueue_wait": (st, [vp, vp]),
     "jxrhip_allocate_destination": (st, [vp, C.c_uint64, pp]),
     "jxrhip_free_destination": (st, [vp, vp]),
     "jxrhip_fill_destination": (st, [vp, vp, C.c_uint8, C.c_uint64]),

@@ -127,6 +127,7 @@ struct jxrhip_plan {
   uint64_t descriptor_bytes = 0;
   uint64_t coefficient_bytes = 0;
   uint64_t geometry_key = 0;  // bucket key: plane geometries
+  jxrhip_surface_layout surface{};  // the validated output layout
 };
 
 static int32_t bias_of(uint32_t depth, uint32_t shift_bits, bool *bad) {
@@ -557,6 +558,7 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
       dv.row_stride[i] = sp.row_stride_bytes;
     }
     plan->output_bytes = s.byte_len;
+    plan->surface = s;
     plan->content_bytes = content;
   }
   // ---- work and footprint figures ----
@@ -775,6 +777,7 @@ struct jxrhip_image {
 
 struct jxrhip_submission {
   jxrhip_session *session = nullptr;
+  jxrhip_surface_layout layout{};  // the plan's output layout
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   uint32_t slot = UINT32_MAX;  // status word index
@@ -1200,6 +1203,7 @@ static jxrhip_status submit_impl(jxrhip_session *s, const jxrhip_plan *p, void *
   std::unique_ptr<jxrhip_submission> sub(new (std::nothrow) jxrhip_submission());
   if (!sub) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "out of host memory");
   sub->session = s;
+  sub->layout = p->surface;
   sub->stream = s->streams[s->next_stream++ % kStreams];
   if (s->free_status.empty()) return fail(JXRHIP_ERR_BUDGET, "too many submissions in flight");
   // a failure once work may be queued waits for the stream, then returns
@@ -1331,6 +1335,55 @@ extern "C" void jxrhip_submission_destroy(jxrhip_submission *sub) {
   (void)hipEventSynchronize(sub->done);
   release_submission(sub);
 }
+
+// ---- consumers of a pending submission (submission.rs:50-93) ---------------
+extern "C" jxrhip_status jxrhip_submission_enqueue_wait(const jxrhip_submission *sub, void *stream) {
+  if (!sub) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null submission");
+  make_current(sub->session);
+  HIP_TRY(hipStreamWaitEvent(static_cast<hipStream_t>(stream), sub->done, 0), "hipStreamWaitEvent");
+  return JXRHIP_OK;
+}
+
+extern "C" jxrhip_status jxrhip_submission_output_layout(const jxrhip_submission *sub, jxrhip_surface_layout *out) {
+  if (!sub || !out) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  *out = sub->layout;
+  return JXRHIP_OK;
+}
+
+extern "C" jxrhip_status jxrhip_submission_pending_device_ptr(const jxrhip_submission *sub, void **ptr,
+                                                              uint64_t *bytes) {
+  if (!sub || !ptr) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  *ptr = nullptr;
+  if (bytes) *bytes = 0;
+  if (!sub->output.ptr)
+    return fail(JXRHIP_ERR_INVALID_ARGUMENT, "the submission writes a caller-owned destination");
+  *ptr = sub->output.ptr;
+  if (bytes) *bytes = sub->output_bytes;
+  return JXRHIP_OK;
+}
+
+// DecodeReport of a device reconstruction (crates/jxr-cuda/src/plan.rs:
+// 214-253): the stages before reconstruction ran on the CPU, the eight
+// reconstruction stages on the device, then an optional host readback.
+extern "C" jxrhip_status jxrhip_plan_decode_report(const jxrhip_plan *p, uint32_t requested, int32_t host_readback,
+                                                   jxrhip_decode_report *out) {
+  if (!p || !out) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  std::memset(out, 0, sizeof *out);
+  out->requested = requested;
+  out->selected = JXRHIP_BACKEND_HIP;
+  out->fallback = JXRHIP_FALLBACK_NONE;
+  uint32_t n = 0;
+  for (uint32_t st = JXRHIP_DECODE_STAGE_PARSE; st <= JXRHIP_DECODE_STAGE_DC_LOW_PASS_PREDICTION; st++)
+    out->stages[n++] = jxrhip_stage_report{st, JXRHIP_EXECUTOR_CPU_SCALAR};
+  for (uint32_t st = JXRHIP_DECODE_STAGE_DEQUANTIZE_AND_FIRST_INVERSE_TRANSFORM;
+       st <= JXRHIP_DECODE_STAGE_CROP_CLIP_AND_PACK; st++)
+    out->stages[n++] = jxrhip_stage_report{st, JXRHIP_EXECUTOR_HIP};
+  if (host_readback) out->stages[n++] = jxrhip_stage_report{JXRHIP_DECODE_STAGE_HOST_READBACK, JXRHIP_EXECUTOR_CPU_SCALAR};
+  out->stage_count = n;
+  return JXRHIP_OK;
+}
+
+extern "C" jxrhip_status jxrhip_batch_enqueue_wait(const jxrhip_batch *b, void *stream);
 
 extern "C" jxrhip_status jxrhip_allocate_destination(jxrhip_session *s, uint64_t bytes, void **ptr) {
   if (!s || !ptr) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
@@ -1773,6 +1826,19 @@ extern "C" int32_t jxrhip_batch_is_complete(const jxrhip_batch *b) {
   for (auto *sub : b->subs)
     if (!jxrhip_submission_is_complete(sub)) return 0;
   return 1;
+}
+
+// A device-side wait on `stream` for every image of the batch.
+extern "C" jxrhip_status jxrhip_batch_enqueue_wait(const jxrhip_batch *b, void *stream) {
+  if (!b) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null batch");
+  make_current(b->session);
+  hipStream_t cs = static_cast<hipStream_t>(stream);
+  if (b->done) {
+    HIP_TRY(hipStreamWaitEvent(cs, b->done, 0), "hipStreamWaitEvent");
+    return JXRHIP_OK;
+  }
+  for (const jxrhip_submission *sub : b->subs) HIP_TRY(hipStreamWaitEvent(cs, sub->done, 0), "hipStreamWaitEvent");
+  return JXRHIP_OK;
 }
 
 extern "C" jxrhip_status jxrhip_batch_wait(jxrhip_batch *b, uint32_t *stages) {

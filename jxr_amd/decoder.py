@@ -309,6 +309,14 @@ class HipDecodePlan:
     def device_footprint(self) -> int:
         return load().jxrhip_plan_device_footprint(self._h)
 
+    def decode_report(self, requested: int = abi.BACKEND_HIP, host_readback: bool = False) -> dict:
+        """Route and per-stage ownership of a device decode
+        (CudaDecodePlan::decode_report, crates/jxr-cuda/src/plan.rs:214-253)."""
+        r = abi.DecodeReport()
+        check(load().jxrhip_plan_decode_report(self.handle, requested, 1 if host_readback else 0, C.byref(r)),
+              "decode_report")
+        return _report(r)
+
     @property
     def algorithmic_bytes(self) -> int:
         return load().jxrhip_plan_algorithmic_bytes(self._h)
@@ -330,6 +338,13 @@ def plan_hip_route(reconstructed_coefficients: int, strict: bool, available: boo
     check(load().jxrhip_route(reconstructed_coefficients, 1 if strict else 0, 1 if available else 0, C.byref(use)),
           "route")
     return bool(use.value)
+
+
+def _report(r: abi.DecodeReport) -> dict:
+    backends = {abi.BACKEND_AUTO: "Auto", abi.BACKEND_CPU: "Cpu", abi.BACKEND_HIP: "Hip"}
+    return {"requested": backends.get(r.requested, r.requested), "selected": backends.get(r.selected, r.selected),
+            "fallback": None if r.fallback == 0 else r.fallback,
+            "stages": [(abi.DECODE_STAGES[x.stage], abi.EXECUTORS[x.executor]) for x in r.stages[: r.stage_count]]}
 
 
 class ResidentHipImage:
@@ -374,6 +389,23 @@ class HipSubmission:
     def is_complete(self) -> bool:
         return bool(load().jxrhip_submission_is_complete(self._h))
 
+    def enqueue_consumer_wait(self, hip_stream: int):
+        """Work queued next on `hip_stream` runs after this reconstruction
+        (CudaSubmission::enqueue_consumer_wait)."""
+        check(load().jxrhip_submission_enqueue_wait(self._h, C.c_void_p(hip_stream)), "enqueue_wait")
+
+    def output_layout(self) -> abi.SurfaceLayout:
+        out = abi.SurfaceLayout()
+        check(load().jxrhip_submission_output_layout(self._h, C.byref(out)), "output_layout")
+        return out
+
+    def pending_device_ptr(self):
+        """(pointer, bytes) of the output a consumer stream may read after
+        enqueue_consumer_wait (CudaSubmission::pending_device_buffer)."""
+        p, n = C.c_void_p(), C.c_uint64()
+        check(load().jxrhip_submission_pending_device_ptr(self._h, C.byref(p), C.byref(n)), "pending_device_ptr")
+        return p.value or 0, n.value
+
     def wait(self) -> Optional[ResidentHipImage]:
         lib = load()
         img = C.c_void_p()
@@ -400,6 +432,10 @@ class HipBatch:
 
     def is_complete(self) -> bool:
         return bool(load().jxrhip_batch_is_complete(self._h))
+
+    def enqueue_consumer_wait(self, hip_stream: int):
+        """Work queued next on `hip_stream` runs after every image of the batch."""
+        check(load().jxrhip_batch_enqueue_wait(self._h, C.c_void_p(hip_stream)), "batch_enqueue_wait")
 
     def wait(self, raise_on_error=True) -> np.ndarray:
         """Returns each image's stage code (0 = ok)."""

@@ -55,6 +55,8 @@ STRUCTS = {
     "jxrhip_server_result": abi.ServerResult,
     "jxrhip_server_stats": abi.ServerStats,
     "jxrhip_valu_rate": abi.ValuRate,
+    "jxrhip_stage_report": abi.StageReport,
+    "jxrhip_decode_report": abi.DecodeReport,
 }
 
 
@@ -84,3 +86,21 @@ def test_session_creation_fails_loudly_without_a_device():
     with pytest.raises(JxrHipError) as e:
         HipDecoderSession(0)
     assert e.value.code == abi.ERR_UNAVAILABLE
+
+
+def test_decode_report_names_cpu_and_device_stages():
+    # CudaDecodePlan::decode_report (crates/jxr-cuda/src/plan.rs:214-253):
+    # five CPU stages, eight device stages, then the readback when asked
+    import numpy as np
+    from arena_util import ManualArena, desc_for
+    from jxr_amd import HipDecodePlan, OutputSpec
+    a = ManualArena([np.zeros((1, 1), np.int64)])
+    spec = OutputSpec(storage=abi.STORE_U8, depth=abi.DEPTH_U8, layout=abi.LAYOUT_LUMA, output_color=abi.COLOR_LUMA)
+    plan = HipDecodePlan(desc_for(a, abi.COLOR_LUMA, 1, spec, (0, 0, 16, 16)), keepalive=a)
+    for readback in (False, True):
+        r = plan.decode_report(abi.BACKEND_AUTO, host_readback=readback)
+        assert r["requested"] == "Auto" and r["selected"] == "Hip" and r["fallback"] is None
+        want = [(st, "CpuScalar") for st in abi.DECODE_STAGES[:5]] + [(st, "Hip") for st in abi.DECODE_STAGES[5:13]]
+        if readback:
+            want.append(("HostReadback", "CpuScalar"))
+        assert r["stages"] == want

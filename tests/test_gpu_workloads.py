@@ -6,8 +6,14 @@ against the CPU oracle:
   C2  4096x4096 gray u16 pink noise, 8x8 soft tiles, ov2, q75, flexbits;
   C3  all 256 mixed 4:2:0 images as one persistent batch into one dense NCHW
       allocation, replayed from a captured graph;
-  C4  two of the 1024x1024 RGBA f32 lossless images with separate alpha.
+  C4  all 128 of the 1024x1024 RGBA f32 lossless images with separate alpha
+      as one persistent batch (2 GiB of planar f32 output);
+  C5  a Poisson burst of the serving mix (RGB u8 at 256..1024 px, 4096x4096
+      gray u16, 512x512 windows of an 8192x8192 source) through the serving
+      loop at its default HBM budget, every result's bytes equal to the
+      oracle's decode of its pool entry.
 """
+import time
 from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
@@ -67,12 +73,55 @@ def test_c3_full_batch_in_one_dense_allocation(session):
         assert np.array_equal(got, want), f"image {i}"
 
 
-def test_c4_full_size_float_alpha(session):
-    items = workloads.c4(count=2)
+def test_c4_full_batch_float_alpha(session):
+    items = workloads.c4(count=128)
     plans = [HipDecodePlan.from_encoded(e, sp) for e, sp in items]
     bufs = [session.allocate_destination(p.output_bytes) for p in plans]
+    for b in bufs:
+        b.fill(0x5C)
     stages = session.submit_batch(plans, [b.ptr for b in bufs], abi.BATCH_PERSISTENT).wait()
     assert not np.any(stages)
-    for (enc, spec), b in zip(items, bufs):
-        want = O.decode(image_desc(enc, spec))
-        assert np.array_equal(b.to_host(), want)
+    for k in range(0, len(items), 16):
+        chunk = items[k:k + 16]
+        wants = O.decode_many([image_desc(e, sp) for e, sp in chunk], 16)
+        for i, want in enumerate(wants):
+            assert np.array_equal(bufs[k + i].to_host(), want), f"image {k + i}"
+    for b in bufs:
+        b.close()
+
+
+def test_c5_poisson_burst_through_the_serving_loop(session):
+    from jxr_amd import HipServer
+    from jxr_amd.serving import clock_ns
+    pool = workloads.c5_pool()
+    plans = {k: [HipDecodePlan.from_encoded(e, sp) for e, sp in pool[k]] for k in ("small", "gray", "region")}
+    sched = workloads.c5_schedule(0.5)
+    used = sorted({(kind, k) for _, kind, k in sched})
+    with ThreadPoolExecutor(16) as ex:
+        wants = dict(zip(used, ex.map(lambda u: O.decode(image_desc(*pool[u[0]][u[1]])), used)))
+    print(f"[c5] {len(sched)} requests over {len(used)} pool entries", flush=True)
+    server = HipServer(session)  # the default budget (8 GiB, or 3/4 of free HBM if less)
+    start = clock_ns() + 10_000_000
+    arrivals = [start + int(t * 1e9) for t, _, _ in sched]
+    i = done = 0
+    deadline = time.perf_counter() + 120.0
+    try:
+        while done < len(sched):
+            assert time.perf_counter() < deadline, f"{len(sched) - done} requests without a result"
+            now = clock_ns()
+            while i < len(sched) and arrivals[i] <= now:
+                _, kind, k = sched[i]
+                server.submit(plans[kind][k], tag=i, arrival_ns=arrivals[i])
+                i += 1
+            for r in server.poll(1024, timeout_us=1000):
+                _, kind, k = sched[r.tag]
+                assert r.status == 0 and r.image is not None, (r.tag, r.status, r.stage)
+                want = wants[(kind, k)]
+                got = r.image.readback()
+                r.image.close()
+                assert np.array_equal(got[: want.nbytes], want), (r.tag, kind, k)
+                done += 1
+        st = server.stats()
+        assert st["completed"] == len(sched) and st["failed"] == 0 and st["rejected"] == 0
+    finally:
+        server.close()
