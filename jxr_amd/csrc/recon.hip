@@ -1692,6 +1692,7 @@ struct TileShared {
   int tmx, tmy, nx, ny;
   int any_lp, any_sub, any_pot;
   uint32_t out_mag;  // OR of magbits of every sample the tile's output stage can read
+  Slot next_rec;     // one-thread workgroups: the next tile's slot record
 };
 
 // A stage's per-plane item starts, read once into registers before its loop.
@@ -1847,15 +1848,107 @@ __device__ __forceinline__ void plane_setup(TileShared &ts, int pl, uint32_t til
   w.mD = div_magic(w.wD);
 }
 
+// Work assignment. Workgroups are dispatched to the eight XCDs round-robin
+// (b mod 8), and the items of a launch are split into eight contiguous
+// shares, one per XCD, so tiles processed at the same time on one XCD are
+// neighbours whose halo reads meet in its L2. Workgroup b takes item b/8 of
+// its XCD's share first; after that it claims items from the share's counter
+// (the first G/8 items of each share are the static ones), and once its share
+// is drained it moves on to the other shares, so XCDs that run slower or
+// finish later still end together. Per-image launches run one item per
+// workgroup (the grid is rounded up to a multiple of 8). Item i belongs to
+// image item_slot[i].
+//
+// One call site keeps process_tile inlined once, so its LDS pointers stay
+// LDS-addressed. One instantiation per storage kind keeps each kernel's
+// registers and code to what its output format needs; every image of a
+// launch shares the kind.
+struct Shares {
+  uint32_t n, count, per;  // items, shares, static items per share
+  __device__ __forceinline__ uint32_t lo(uint32_t r) const {
+    return count == 1 ? 0u : (uint32_t)(((uint64_t)n * r) >> 3);
+  }
+  __device__ __forceinline__ uint32_t hi(uint32_t r) const {
+    return count == 1 ? n : (uint32_t)(((uint64_t)n * (r + 1)) >> 3);
+  }
+};
+constexpr uint32_t kNoItem = 0xFFFFFFFFu;
+constexpr uint32_t kCounterStride = 32;  // words between share counters (own cache lines)
+
+// Thread 0: the next item from share r onwards, given a claimed index of r.
+__device__ __forceinline__ uint32_t next_item(const Shares &sh, uint32_t *counters, uint32_t &r, uint32_t &tries,
+                                              uint32_t claimed) {
+  for (;;) {
+    const uint32_t it = sh.lo(r) + sh.per + claimed;
+    if (claimed < sh.hi(r) - sh.lo(r) && it < sh.hi(r)) return it;
+    if (++tries >= sh.count) return kNoItem;
+    r = r + 1 == sh.count ? 0u : r + 1;
+    claimed = atomicAdd(counters + r * kCounterStride, 1u);
+  }
+}
+
+// The next tile of a workgroup, fetched while the current one runs, so that
+// no load of the work queue sits between two tiles: thread 0 claims at stage
+// A, turns the claim into the next item and loads its slot index at stage C,
+// and lanes 0-7 of wave 0 load the slot's eight words at stage D, one word a
+// lane (the slot record would otherwise hold eight registers of every lane
+// through the output stage).
+struct Queue {
+  Shares sh;
+  uint32_t r, tries;     // thread 0: share being drained, shares given up
+  uint32_t *claim_from;  // thread 0: counter the stage-A claim came from, or null
+  uint32_t claim;        // thread 0: its result
+  uint32_t next;         // thread 0: the next item (kNoItem: none)
+  uint32_t next_slot;    // thread 0: its slot
+  uint32_t word;         // lanes 0-7: word `lane` of the next slot's record
+};
+
+__device__ __forceinline__ void queue_next_item(Queue &q, const KernelArgs &args) {
+  if (threadIdx.x != 0) return;
+  q.next = q.claim_from ? next_item(q.sh, args.counters, q.r, q.tries, q.claim) : kNoItem;
+  q.next_slot = q.next != kNoItem && args.persistent ? gptr(args.item_slot)[q.next] : 0u;
+}
+
+// Persistent launches only: a per-image launch runs one item per workgroup,
+// whose record is the launch's inline slot. A workgroup of one thread (the
+// CPU test build) copies the whole record to `staged` instead.
+__device__ __forceinline__ void queue_next_slot(Queue &q, const KernelArgs &args, Slot &staged) {
+  if (threadIdx.x >= 8 || !args.persistent) return;
+  const uint32_t next = __builtin_amdgcn_readfirstlane(q.next);
+  const uint32_t ns = __builtin_amdgcn_readfirstlane(q.next_slot);
+  static_assert(sizeof(Slot) == 32, "a slot record is eight words");
+  if (blockDim.x == 1) {
+    if (next != kNoItem) staged = args.slots[ns];
+    return;
+  }
+  q.word = next != kNoItem ? gptr(reinterpret_cast<const uint32_t *>(args.slots + ns))[threadIdx.x] : 0u;
+}
+
+// The fetched record becomes the workgroup's current one (between barriers).
+__device__ __forceinline__ void queue_publish(const Queue &q, const KernelArgs &args, const Slot &staged, Slot &rec,
+                                              uint32_t &item, uint32_t &slot) {
+  if (args.persistent && threadIdx.x < 8) {
+    if (blockDim.x == 1)
+      rec = staged;
+    else
+      reinterpret_cast<uint32_t *>(&rec)[threadIdx.x] = q.word;
+  }
+  if (threadIdx.x == 0) {
+    if (!args.persistent) rec = args.inline_slot;
+    item = q.next;
+    slot = q.next_slot;
+  }
+}
+
 // One tile of one image. `reload` is false when the previous tile of this
 // workgroup belonged to the same image, whose descriptor is still in LDS.
-// `claim_from` (thread 0 only; null for none) is a work counter to claim the
-// workgroup's next item from: the atomic is issued when stage A starts, so
-// its latency hides behind the stage's loads, and its result lands in
-// `claim`.
+// `q` fetches the workgroup's next tile meanwhile: the claim atomic is
+// issued when stage A starts, so its latency hides behind the stage's loads,
+// and the loads that turn it into an item and a slot record follow in
+// stages C and D.
 template <uint32_t STORE>
 __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, const KernelArgs &args, int *lds,
-                                             TileShared &ts, bool reload, uint32_t *claim_from, uint32_t &claim) {
+                                             TileShared &ts, bool reload, Queue &q) {
   __syncthreads();  // the previous tile's output stage has read the tile state
   if (reload) {
     const JXR_GLOBAL v4u *src = (const JXR_GLOBAL v4u *)gptr(slot.img);
@@ -1871,7 +1964,7 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
   const int tmx = ts.tmx, tmy = ts.tmy, nx = ts.nx, ny = ts.ny;
   uint32_t *status = slot.status;
 
-  if (threadIdx.x == 0 && claim_from) claim = atomicAdd(claim_from, 1u);
+  if (threadIdx.x == 0 && q.claim_from) q.claim = atomicAdd(q.claim_from, 1u);
 
   // stage A: first level of the tile's macroblocks and their halo
   const Starts stA = starts_of(ts.sA, np);
@@ -1923,6 +2016,8 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
     __syncthreads();
   }
 
+  queue_next_item(q, args);
+
   // stage C: high-pass prediction and second level, tile blocks + halo
   uint32_t omag = 0;  // magnitude of every sample this thread writes in stages C and D
   const Starts stC = starts_of(ts.sC, np);
@@ -1935,6 +2030,8 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
   }
   if (!ts.any_pot) atomicOr(&ts.out_mag, omag);
   __syncthreads();
+
+  queue_next_slot(q, args, ts.next_rec);
 
   // stage D: sample-level overlap (modes 1, 2)
   if (ts.any_pot) {
@@ -2002,77 +2099,42 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
   }
 }
 
-// Work assignment. Workgroups are dispatched to the eight XCDs round-robin
-// (b mod 8), and the items of a launch are split into eight contiguous
-// shares, one per XCD, so tiles processed at the same time on one XCD are
-// neighbours whose halo reads meet in its L2. Workgroup b takes item b/8 of
-// its XCD's share first; after that it claims items from the share's counter
-// (the first G/8 items of each share are the static ones), and once its share
-// is drained it moves on to the other shares, so XCDs that run slower or
-// finish later still end together. Per-image launches run one item per
-// workgroup (the grid is rounded up to a multiple of 8). Item i belongs to
-// image item_slot[i].
-//
-// One call site keeps process_tile inlined once, so its LDS pointers stay
-// LDS-addressed. One instantiation per storage kind keeps each kernel's
-// registers and code to what its output format needs; every image of a
-// launch shares the kind.
-struct Shares {
-  uint32_t n, count, per;  // items, shares, static items per share
-  __device__ __forceinline__ uint32_t lo(uint32_t r) const {
-    return count == 1 ? 0u : (uint32_t)(((uint64_t)n * r) >> 3);
-  }
-  __device__ __forceinline__ uint32_t hi(uint32_t r) const {
-    return count == 1 ? n : (uint32_t)(((uint64_t)n * (r + 1)) >> 3);
-  }
-};
-constexpr uint32_t kNoItem = 0xFFFFFFFFu;
-constexpr uint32_t kCounterStride = 32;  // words between share counters (own cache lines)
-
-// Thread 0: the next item from share r onwards, given a claimed index of r.
-__device__ __forceinline__ uint32_t next_item(const Shares &sh, uint32_t *counters, uint32_t &r, uint32_t &tries,
-                                              uint32_t claimed) {
-  for (;;) {
-    const uint32_t it = sh.lo(r) + sh.per + claimed;
-    if (claimed < sh.hi(r) - sh.lo(r) && it < sh.hi(r)) return it;
-    if (++tries >= sh.count) return kNoItem;
-    r = r + 1 == sh.count ? 0u : r + 1;
-    claimed = atomicAdd(counters + r * kCounterStride, 1u);
-  }
-}
-
 template <uint32_t STORE>
 __global__ void __launch_bounds__(kThreads, kMinWavesPerSimd) jxr_reconstruct_tiles(KernelArgs args) {
   HIP_DYNAMIC_SHARED(v4i, lds_quads)  // 16-byte aligned: regions are read and written in quads
   int *lds = reinterpret_cast<int *>(lds_quads);
   __shared__ TileShared ts;
-  __shared__ uint32_t shared_item;
+  __shared__ uint32_t shared_item, shared_slot;  // the workgroup's current item and its slot
+  __shared__ Slot shared_rec;                    // that slot's record
   const uint32_t G = gridDim.x, b = blockIdx.x;
-  const Shares sh{args.item_count, (G & 7u) == 0 ? 8u : 1u, (G & 7u) == 0 ? G >> 3 : G};
-  const uint32_t x = sh.count == 8 ? (b & 7u) : 0u;
-  uint32_t r = x, tries = 0;  // thread 0: share being drained, shares given up
+  Queue q{};
+  q.sh = Shares{args.item_count, (G & 7u) == 0 ? 8u : 1u, (G & 7u) == 0 ? G >> 3 : G};
+  const uint32_t x = q.sh.count == 8 ? (b & 7u) : 0u;
+  q.r = x;
   if (threadIdx.x == 0) {
-    uint32_t first = sh.lo(x) + (sh.count == 8 ? b >> 3 : b);
-    if (first >= sh.hi(x))
-      first = args.persistent ? next_item(sh, args.counters, r, tries, atomicAdd(args.counters + r * kCounterStride, 1u))
-                              : kNoItem;
-    shared_item = first;
+    uint32_t first = q.sh.lo(x) + (q.sh.count == 8 ? b >> 3 : b);
+    if (first >= q.sh.hi(x))
+      first = args.persistent
+                  ? next_item(q.sh, args.counters, q.r, q.tries, atomicAdd(args.counters + q.r * kCounterStride, 1u))
+                  : kNoItem;
+    q.next = first;
+    q.next_slot = first != kNoItem && args.persistent ? gptr(args.item_slot)[first] : 0u;
   }
+  queue_next_slot(q, args, ts.next_rec);
+  queue_publish(q, args, ts.next_rec, shared_rec, shared_item, shared_slot);
   __syncthreads();
-  uint32_t item = shared_item;
-  const JXR_GLOBAL uint32_t *item_slot = gptr(args.item_slot);
   uint32_t loaded = kNoItem;
-  while (item != kNoItem) {
-    const uint32_t slot_index = args.persistent ? item_slot[item] : 0u;
-    const Slot &slot = args.persistent ? args.slots[slot_index] : args.inline_slot;
-    uint32_t claim = 0;
-    uint32_t *claim_from = args.persistent && tries < sh.count ? args.counters + r * kCounterStride : nullptr;
-    process_tile<STORE>(slot, item - slot.tile_start, args, lds, ts, slot_index != loaded, claim_from, claim);
+  for (;;) {
+    const uint32_t item = shared_item;
+    if (item == kNoItem) break;
+    const uint32_t slot_index = shared_slot;
+    if (threadIdx.x == 0)
+      q.claim_from = args.persistent && q.tries < q.sh.count ? args.counters + q.r * kCounterStride : nullptr;
+    process_tile<STORE>(shared_rec, item - shared_rec.tile_start, args, lds, ts, slot_index != loaded, q);
     loaded = slot_index;
-    __syncthreads();  // every lane has finished with shared_item
-    if (threadIdx.x == 0) shared_item = claim_from ? next_item(sh, args.counters, r, tries, claim) : kNoItem;
+    __syncthreads();  // every lane has finished with the current item, slot and record
+    queue_publish(q, args, ts.next_rec, shared_rec, shared_item, shared_slot);
     __syncthreads();
-    item = shared_item;
   }
 }
 

@@ -70,6 +70,8 @@ inline unsigned atomicAdd(unsigned *p, unsigned v) {
   *p = old + v;
   return old;
 }
+// one thread runs a workgroup: it is its own first lane
+inline unsigned __builtin_amdgcn_readfirstlane(unsigned v) { return v; }
 inline float __uint_as_float(unsigned u) {
   float f;
   memcpy(&f, &u, 4);
