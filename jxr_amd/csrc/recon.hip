@@ -1891,16 +1891,16 @@ __device__ __forceinline__ uint32_t next_item(const Shares &sh, uint32_t *counte
 // no load of the work queue sits between two tiles: thread 0 claims at stage
 // A, turns the claim into the next item and loads its slot index at stage C,
 // and lanes 0-7 of wave 0 load the slot's eight words at stage D, one word a
-// lane (the slot record would otherwise hold eight registers of every lane
-// through the output stage).
+// lane (`word`, the one register this holds through the output stage). The
+// queue state is thread 0's alone and lives in LDS, so that it holds no
+// register of every lane through the tile.
 struct Queue {
   Shares sh;
-  uint32_t r, tries;     // thread 0: share being drained, shares given up
-  uint32_t *claim_from;  // thread 0: counter the stage-A claim came from, or null
-  uint32_t claim;        // thread 0: its result
-  uint32_t next;         // thread 0: the next item (kNoItem: none)
-  uint32_t next_slot;    // thread 0: its slot
-  uint32_t word;         // lanes 0-7: word `lane` of the next slot's record
+  uint32_t r, tries;     // share being drained, shares given up
+  uint32_t *claim_from;  // counter the stage-A claim came from, or null
+  uint32_t claim;        // its result
+  uint32_t next;         // the next item (kNoItem: none)
+  uint32_t next_slot;    // its slot
 };
 
 __device__ __forceinline__ void queue_next_item(Queue &q, const KernelArgs &args) {
@@ -1912,26 +1912,37 @@ __device__ __forceinline__ void queue_next_item(Queue &q, const KernelArgs &args
 // Persistent launches only: a per-image launch runs one item per workgroup,
 // whose record is the launch's inline slot. A workgroup of one thread (the
 // CPU test build) copies the whole record to `staged` instead.
-__device__ __forceinline__ void queue_next_slot(Queue &q, const KernelArgs &args, Slot &staged) {
+__device__ __forceinline__ void queue_next_slot(const Queue &q, const KernelArgs &args, Slot &staged,
+                                                uint32_t &word) {
   if (threadIdx.x >= 8 || !args.persistent) return;
-  const uint32_t next = __builtin_amdgcn_readfirstlane(q.next);
-  const uint32_t ns = __builtin_amdgcn_readfirstlane(q.next_slot);
+  // thread 0 reads its own queue state and hands it to lanes 1-7 through a
+  // cross-lane read: lanes 1-7 reading the LDS copy themselves would race
+  // with thread 0's writes (no barrier orders them), and the compiler may
+  // move such a read ahead of the write
+  uint32_t next = kNoItem, ns = 0;
+  if (threadIdx.x == 0) {
+    next = q.next;
+    ns = q.next_slot;
+  }
+  next = __builtin_amdgcn_readfirstlane(next);
+  ns = __builtin_amdgcn_readfirstlane(ns);
   static_assert(sizeof(Slot) == 32, "a slot record is eight words");
   if (blockDim.x == 1) {
     if (next != kNoItem) staged = args.slots[ns];
     return;
   }
-  q.word = next != kNoItem ? gptr(reinterpret_cast<const uint32_t *>(args.slots + ns))[threadIdx.x] : 0u;
+  word = next != kNoItem ? gptr(reinterpret_cast<const uint32_t *>(args.slots + ns))[threadIdx.x] : 0u;
 }
 
 // The fetched record becomes the workgroup's current one (between barriers).
-__device__ __forceinline__ void queue_publish(const Queue &q, const KernelArgs &args, const Slot &staged, Slot &rec,
+__device__ __forceinline__ void queue_publish(const Queue &q, uint32_t word, const KernelArgs &args, const Slot &staged,
+                                              Slot &rec,
                                               uint32_t &item, uint32_t &slot) {
   if (args.persistent && threadIdx.x < 8) {
     if (blockDim.x == 1)
       rec = staged;
     else
-      reinterpret_cast<uint32_t *>(&rec)[threadIdx.x] = q.word;
+      reinterpret_cast<uint32_t *>(&rec)[threadIdx.x] = word;
   }
   if (threadIdx.x == 0) {
     if (!args.persistent) rec = args.inline_slot;
@@ -1948,7 +1959,7 @@ __device__ __forceinline__ void queue_publish(const Queue &q, const KernelArgs &
 // stages C and D.
 template <uint32_t STORE>
 __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, const KernelArgs &args, int *lds,
-                                             TileShared &ts, bool reload, Queue &q) {
+                                             TileShared &ts, bool reload, Queue &q, uint32_t &word) {
   __syncthreads();  // the previous tile's output stage has read the tile state
   if (reload) {
     const JXR_GLOBAL v4u *src = (const JXR_GLOBAL v4u *)gptr(slot.img);
@@ -2031,7 +2042,7 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
   if (!ts.any_pot) atomicOr(&ts.out_mag, omag);
   __syncthreads();
 
-  queue_next_slot(q, args, ts.next_rec);
+  queue_next_slot(q, args, ts.next_rec, word);
 
   // stage D: sample-level overlap (modes 1, 2)
   if (ts.any_pot) {
@@ -2106,12 +2117,14 @@ __global__ void __launch_bounds__(kThreads, kMinWavesPerSimd) jxr_reconstruct_ti
   __shared__ TileShared ts;
   __shared__ uint32_t shared_item, shared_slot;  // the workgroup's current item and its slot
   __shared__ Slot shared_rec;                    // that slot's record
+  __shared__ Queue q;
   const uint32_t G = gridDim.x, b = blockIdx.x;
-  Queue q{};
-  q.sh = Shares{args.item_count, (G & 7u) == 0 ? 8u : 1u, (G & 7u) == 0 ? G >> 3 : G};
-  const uint32_t x = q.sh.count == 8 ? (b & 7u) : 0u;
-  q.r = x;
+  uint32_t word = 0;  // lanes 0-7: word `lane` of the next tile's slot record
   if (threadIdx.x == 0) {
+    q.sh = Shares{args.item_count, (G & 7u) == 0 ? 8u : 1u, (G & 7u) == 0 ? G >> 3 : G};
+    const uint32_t x = q.sh.count == 8 ? (b & 7u) : 0u;
+    q.r = x;
+    q.tries = 0;
     uint32_t first = q.sh.lo(x) + (q.sh.count == 8 ? b >> 3 : b);
     if (first >= q.sh.hi(x))
       first = args.persistent
@@ -2120,8 +2133,8 @@ __global__ void __launch_bounds__(kThreads, kMinWavesPerSimd) jxr_reconstruct_ti
     q.next = first;
     q.next_slot = first != kNoItem && args.persistent ? gptr(args.item_slot)[first] : 0u;
   }
-  queue_next_slot(q, args, ts.next_rec);
-  queue_publish(q, args, ts.next_rec, shared_rec, shared_item, shared_slot);
+  queue_next_slot(q, args, ts.next_rec, word);
+  queue_publish(q, word, args, ts.next_rec, shared_rec, shared_item, shared_slot);
   __syncthreads();
   uint32_t loaded = kNoItem;
   for (;;) {
@@ -2130,10 +2143,10 @@ __global__ void __launch_bounds__(kThreads, kMinWavesPerSimd) jxr_reconstruct_ti
     const uint32_t slot_index = shared_slot;
     if (threadIdx.x == 0)
       q.claim_from = args.persistent && q.tries < q.sh.count ? args.counters + q.r * kCounterStride : nullptr;
-    process_tile<STORE>(shared_rec, item - shared_rec.tile_start, args, lds, ts, slot_index != loaded, q);
+    process_tile<STORE>(shared_rec, item - shared_rec.tile_start, args, lds, ts, slot_index != loaded, q, word);
     loaded = slot_index;
     __syncthreads();  // every lane has finished with the current item, slot and record
-    queue_publish(q, args, ts.next_rec, shared_rec, shared_item, shared_slot);
+    queue_publish(q, word, args, ts.next_rec, shared_rec, shared_item, shared_slot);
     __syncthreads();
   }
 }
