@@ -881,23 +881,29 @@ __device__ __forceinline__ uint32_t f16_bits(int v, bool scaled, A &a) {
 }
 
 // scaling.rs:254-296, with the CPU's clip to f32::MAX on exponent overflow;
-// `ok` is cleared when the mantissa does not fit the f32 field.
+// `ok` is cleared when the mantissa does not fit the f32 field. A coded
+// subnormal is normalised in one step (a leading-zero count) rather than by
+// the reference's one-bit-a-round loop, which a wave would run as long as
+// its slowest lane: the same shift, capped where the exponent reaches 1.
+// Magnitudes and mantissas fit 32 bits (lengths up to 23 are validated).
 template <class A>
 __device__ __forceinline__ uint32_t f32_bits(int v, bool scaled, uint32_t length, int bias, A &a, bool &ok) {
   const int q = scaled ? a.add(v, 3) >> 3 : v;
   const uint32_t sign = q < 0 ? 0x80000000u : 0u;
-  const unsigned long long mag = q < 0 ? (unsigned long long)(-(long long)q) : (unsigned long long)q;
-  const unsigned long long implicit = 1ull << length;
+  const uint32_t mag = q < 0 ? 0u - (uint32_t)q : (uint32_t)q;
+  const uint32_t implicit = 1u << length;
   long long exponent = (long long)(mag >> length);
-  unsigned long long mantissa = (mag & (implicit - 1)) | implicit;
+  uint32_t mantissa = (mag & (implicit - 1u)) | implicit;
   if (exponent == 0) {
     mantissa ^= implicit;
     exponent = 1;
   }
   exponent = exponent - bias + 127;
-  while (mantissa < implicit && exponent > 1 && mantissa > 0) {
-    exponent -= 1;
-    mantissa <<= 1;
+  if (mantissa < implicit && mantissa > 0 && exponent > 1) {
+    const int need = (int)length - (31 - __builtin_clz(mantissa));
+    const int steps = (long long)need < exponent - 1 ? need : (int)(exponent - 1);
+    mantissa <<= steps;
+    exponent -= steps;
   }
   if (mantissa < implicit)
     exponent = 0;
@@ -905,8 +911,8 @@ __device__ __forceinline__ uint32_t f32_bits(int v, bool scaled, uint32_t length
     mantissa ^= implicit;
   mantissa <<= 23 - length;
   if (exponent < 0 || exponent > 0xff) return sign | 0x7f7fffffu;
-  if (mantissa > 0x7fffffull) ok = false;
-  return sign | ((uint32_t)exponent << 23) | (uint32_t)mantissa;
+  if (mantissa > 0x7fffffu) ok = false;
+  return sign | ((uint32_t)exponent << 23) | mantissa;
 }
 
 // round(v * a / M) with both clipped to M (premultiply.rs); for M <= 65535
