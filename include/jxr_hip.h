@@ -63,6 +63,7 @@ typedef int32_t jxrhip_status;
 #define JXRHIP_STAGE_OVERLAP 2u
 #define JXRHIP_STAGE_PREDICTION 3u
 #define JXRHIP_STAGE_SECOND_TRANSFORM 4u
+#define JXRHIP_STAGE_SAMPLE_SCALE 5u /* reduced decode: the flat-probe sample scale */
 #define JXRHIP_STAGE_OUTPUT 16u
 
 /* ---- coefficient arena (crates/jxr-core/src/plan.rs:11-162) ------------ */
@@ -226,7 +227,24 @@ typedef struct jxrhip_surface_layout {
   uint64_t byte_len;
 } jxrhip_surface_layout;
 
-/* Everything CudaDecodePlan::from_prepared takes (plan.rs:91-148). */
+/* DecodeScale (crates/jxr-core/src/request.rs:10-40). A reduced decode
+ * stops after the first-level transform (quarter: one sample per block, from
+ * the DC and low-pass bands) or at the macroblock DC (sixteenth: one sample
+ * per macroblock) and scales those coefficients to the samples they stand
+ * for (crates/jxr-native/src/reconstruct/pipeline.rs:175-260,
+ * sample_scale.rs). The reference runs it on the CPU only; its CUDA plan
+ * rejects it (crates/jxr-cuda/src/plan.rs:634). With a reduced scale the
+ * output crop is in the coded plane at that scale:
+ * (coded margin + region origin) / 4 or / 16, as output_crop does
+ * (crates/jxr-native/src/decode/output.rs:484-501). The caller's prepared
+ * plan has already required frequency-mode packets and, for a quarter,
+ * the low-pass band (crates/jxr-native/src/prepare.rs:154-194). */
+#define JXRHIP_SCALE_FULL 0u
+#define JXRHIP_SCALE_QUARTER 1u
+#define JXRHIP_SCALE_SIXTEENTH 2u
+
+/* Everything CudaDecodePlan::from_prepared takes (plan.rs:91-148), plus the
+ * decode scale. */
 typedef struct jxrhip_image_desc {
   uint32_t abi_version;     /* JXRHIP_ABI_VERSION                              */
   uint32_t color_format;    /* primary ColorFormat, JXRHIP_COLOR_*             */
@@ -234,7 +252,7 @@ typedef struct jxrhip_image_desc {
   uint32_t alpha_mode;      /* JXRHIP_ALPHA_*                                  */
   uint32_t scaled;          /* primary SCALED_FLAG: chroma doubled after the
                                first-level transform (jxr-core image.rs:83-85) */
-  uint32_t reserved0;
+  uint32_t scale;           /* JXRHIP_SCALE_*: native decode resolution       */
   const jxrhip_arena *primary;
   const jxrhip_arena *separate_alpha; /* one luma plane, or NULL             */
   jxrhip_output_format output;

@@ -26,6 +26,7 @@ STAGE_FIRST_TRANSFORM = 1
 STAGE_OVERLAP = 2
 STAGE_PREDICTION = 3
 STAGE_SECOND_TRANSFORM = 4
+STAGE_SAMPLE_SCALE = 5
 STAGE_OUTPUT = 16
 
 BANDS_DC_ONLY, BANDS_NO_HIGH_PASS, BANDS_NO_FLEXBITS, BANDS_ALL = 0, 1, 2, 3
@@ -34,6 +35,7 @@ PRED_NONE, PRED_FROM_LEFT, PRED_FROM_TOP, PRED_FROM_TOP_LEFT = 0, 1, 2, 3
 COLOR_LUMA, COLOR_YUV420, COLOR_YUV422, COLOR_YUV444, COLOR_RGB = 0, 1, 2, 3, 4
 COLOR_CMYK, COLOR_CMYK_DIRECT, COLOR_YUVK, COLOR_RGBE, COLOR_NCOMPONENT = 5, 6, 7, 8, 9
 ALPHA_NONE, ALPHA_INTEGRATED, ALPHA_SEPARATE = 0, 1, 2
+SCALE_FULL, SCALE_QUARTER, SCALE_SIXTEENTH = 0, 1, 2
 
 DEPTH_BIT1_WHITE, DEPTH_BIT1_BLACK, DEPTH_U8, DEPTH_U10, DEPTH_U16 = 0, 1, 2, 3, 4
 DEPTH_I16, DEPTH_F16, DEPTH_I32, DEPTH_F32 = 5, 6, 7, 8
@@ -150,7 +152,7 @@ class ImageDesc(C.Structure):
         ("component_count", C.c_uint32),
         ("alpha_mode", C.c_uint32),
         ("scaled", C.c_uint32),
-        ("reserved0", C.c_uint32),
+        ("scale", C.c_uint32),
         ("primary", C.POINTER(Arena)),
         ("separate_alpha", C.POINTER(Arena)),
         ("output", OutputFormat),

@@ -694,6 +694,8 @@ typedef struct {
   jxrhip_coefficient_plane cp;
   uint32_t bc, br;
   uint32_t mb_ox, mb_oy, mbx, mby;
+  uint32_t resolution; /* JXRHIP_SCALE_* */
+  uint32_t ux, uy;     /* samples across and down one macroblock at that scale */
   uint32_t *raster; /* [mbx*mby] -> arena macroblock index */
   int scale_after_first;
   int alpha;
@@ -747,8 +749,9 @@ static int clipped_axis(const uint32_t *sizes, uint32_t n, uint32_t origin, uint
 /* Window of one coefficient plane and its raster map
  * (crates/jxr-cuda/src/plan.rs:495-574). */
 static int plane_setup(plane_t *p, const jxrhip_arena *a, uint32_t index, int scale, int alpha,
-                       char *msg, size_t msglen) {
+                       uint32_t resolution, char *msg, size_t msglen) {
   memset(p, 0, sizeof *p);
+  p->resolution = resolution;
   if (index >= a->plane_count) FAIL(JXRHIP_ERR_INVALID_PLAN, "coefficient plane index is absent");
   p->arena = a;
   p->cp = a->planes[index];
@@ -830,8 +833,12 @@ static int plane_setup(plane_t *p, const jxrhip_arena *a, uint32_t index, int sc
   }
   p->lw = p->mbx * p->bc;
   p->lh = p->mby * p->br;
-  p->sw = p->lw * 4;
-  p->sh = p->lh * 4;
+  /* a quarter keeps one sample per block, a sixteenth one per macroblock
+   * (pipeline.rs:198-260) */
+  p->ux = resolution == JXRHIP_SCALE_QUARTER ? p->bc : resolution == JXRHIP_SCALE_SIXTEENTH ? 1 : p->bc * 4;
+  p->uy = resolution == JXRHIP_SCALE_QUARTER ? p->br : resolution == JXRHIP_SCALE_SIXTEENTH ? 1 : p->br * 4;
+  p->sw = p->mbx * p->ux;
+  p->sh = p->mby * p->uy;
   p->low = (int32_t *)calloc((size_t)p->lw * p->lh, sizeof(int32_t));
   p->samples = (int32_t *)calloc((size_t)p->sw * p->sh, sizeof(int32_t));
   if (!p->low || !p->samples) {
@@ -846,8 +853,10 @@ static int dequant(int32_t v, uint32_t step, int *e) {
 }
 
 /* Dequantize DC/LP and run the first-level inverse transform of every
- * macroblock (pipeline.rs:605-680; kernels/reconstruction.rs:6-64). */
-static int first_level(plane_t *p) {
+ * macroblock (pipeline.rs:605-680; kernels/reconstruction.rs:6-64). With
+ * dc_only the low-pass band is taken as zero, as a sixteenth-size decode
+ * reads it (LowBands::Dc, pipeline.rs:363-372, 488-506). */
+static int first_level(plane_t *p, int dc_only) {
   const jxrhip_arena *a = p->arena;
   uint32_t blocks = p->bc * p->br;
   for (uint32_t my = 0; my < p->mby; my++)
@@ -858,7 +867,7 @@ static int first_level(plane_t *p) {
       uint32_t bands = a->bands[i];
       int32_t low[16] = {0};
       low[0] = dequant(c[0], a->quantizers[3 * i], e);
-      if (bands != JXRHIP_BANDS_DC_ONLY)
+      if (bands != JXRHIP_BANDS_DC_ONLY && !dc_only)
         for (uint32_t k = 1; k < blocks; k++)
           low[k] = dequant(c[bands >= JXRHIP_BANDS_NO_FLEXBITS ? 16 * k : k], a->quantizers[3 * i + 1], e);
       if (p->bc == 4)
@@ -1047,8 +1056,454 @@ static int second_level(plane_t *p, uint32_t *stage) {
   return 0;
 }
 
+/* ---------------------------------------------------------------------- */
+/* reduced resolution (reconstruct/pipeline.rs:175-260,363-486;            */
+/* reconstruct/sample_scale.rs)                                            */
+/* ---------------------------------------------------------------------- */
+/* Regions of one to five macroblocks have every arrangement an overlap
+ * operator can meet (sample_scale.rs:43-52), and the flat probe levels are
+ * 0x10001..0x10004 at both signs (sample_scale.rs:33-41). */
+#define PROBE_EXTENT 5u
+static const int32_t PROBE_LEVELS[4] = {0x10001, 0x10002, 0x10003, 0x10004};
+
+/* probe_offset (sample_scale.rs:406-417) */
+static uint32_t probe_offset(uint32_t offset, uint32_t extent) {
+  const uint32_t ends = PROBE_EXTENT / 2;
+  if (extent <= PROBE_EXTENT || offset < ends) return offset;
+  if (extent - offset <= ends) return PROBE_EXTENT - (extent - offset);
+  return ends;
+}
+
+typedef struct {
+  uint32_t bounds[PROBE_EXTENT + 2]; /* probe regions, incl. 0 and the extent */
+  uint32_t nb;
+  uint32_t mbs;                      /* macroblocks along the probe            */
+  uint32_t *pos;                     /* probe macroblock of each macroblock    */
+  uint32_t npos;
+} probe_axis_t;
+
+/* ProbeAxis::new (sample_scale.rs:336-381): one probe region for each
+ * extent a region of the component has, every extent of five or more
+ * counting as five; each macroblock maps to the probe macroblock placed in
+ * its region as it is placed in its own. */
+static int probe_axis(const uint32_t *regions, uint32_t n, probe_axis_t *a) {
+  memset(a, 0, sizeof *a);
+  if (n < 2 || regions[n - 1] <= regions[0]) return 1;
+  a->pos = (uint32_t *)malloc(sizeof(uint32_t) * (regions[n - 1] - regions[0]));
+  if (!a->pos) return 1;
+  int64_t starts[PROBE_EXTENT + 1];
+  for (uint32_t k = 0; k <= PROBE_EXTENT; k++) starts[k] = -1;
+  a->bounds[a->nb++] = 0;
+  for (uint32_t i = 0; i + 1 < n; i++) {
+    if (regions[i + 1] <= regions[i]) {
+      free(a->pos);
+      a->pos = NULL;
+      return 1;
+    }
+    uint32_t extent = regions[i + 1] - regions[i];
+    uint32_t pe = extent < PROBE_EXTENT ? extent : PROBE_EXTENT;
+    if (starts[pe] < 0) {
+      starts[pe] = a->mbs;
+      a->mbs += pe;
+      a->bounds[a->nb++] = a->mbs;
+    }
+    for (uint32_t off = 0; off < extent; off++) a->pos[a->npos++] = (uint32_t)starts[pe] + probe_offset(off, extent);
+  }
+  return 0;
+}
+
+int jxro_probe_axis(const uint32_t *regions, uint32_t count, uint32_t *bounds, uint32_t *bound_count,
+                    uint32_t *positions, uint32_t capacity) {
+  probe_axis_t a;
+  if (probe_axis(regions, count, &a)) return 1;
+  int r = a.npos > capacity;
+  if (!r) {
+    memcpy(bounds, a.bounds, sizeof(uint32_t) * a.nb);
+    *bound_count = a.nb;
+    memcpy(positions, a.pos, sizeof(uint32_t) * a.npos);
+  }
+  free(a.pos);
+  return r;
+}
+
+/* Forward windows of the full-resolution schedule: the postfilter's
+ * windows are disjoint, so the prefilter runs them in any order. */
+static int forward_indices4(int32_t *s, uint64_t i0, uint64_t i1, uint64_t i2, uint64_t i3) {
+  int32_t v[4] = {s[i0], s[i1], s[i2], s[i3]};
+  int err = jxro_forward_overlap_4(v);
+  s[i0] = v[0];
+  s[i1] = v[1];
+  s[i2] = v[2];
+  s[i3] = v[3];
+  return err;
+}
+static int forward_full_region(int32_t *s, uint64_t stride, uint32_t L, uint32_t R, uint32_t T, uint32_t B) {
+  int err = 0;
+  for (uint32_t y = T + 2; y + 2 < B; y += 4)
+    for (uint32_t x = L + 2; x + 2 < R; x += 4) {
+      int32_t v[16];
+      for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) v[r * 4 + c] = s[(y + r) * stride + x + c];
+      err |= jxro_forward_overlap_4x4(v);
+      for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) s[(y + r) * stride + x + c] = v[r * 4 + c];
+    }
+  for (uint32_t y = T + 2; y + 2 < B; y += 4) {
+    uint32_t xs[4] = {L, L + 1, R - 2, R - 1};
+    for (int k = 0; k < 4; k++)
+      err |= forward_indices4(s, y * stride + xs[k], (y + 1) * stride + xs[k], (y + 2) * stride + xs[k],
+                              (y + 3) * stride + xs[k]);
+  }
+  for (uint32_t x = L + 2; x + 2 < R; x += 4) {
+    uint32_t ys[4] = {T, T + 1, B - 2, B - 1};
+    for (int k = 0; k < 4; k++) {
+      uint64_t b = ys[k] * stride + x;
+      err |= forward_indices4(s, b, b + 1, b + 2, b + 3);
+    }
+  }
+  uint32_t cx[4] = {L, R - 2, L, R - 2}, cy[4] = {T, T, B - 2, B - 2};
+  for (int k = 0; k < 4; k++) {
+    uint64_t b = cy[k] * stride + cx[k];
+    err |= forward_indices4(s, b, b + 1, b + stride, b + stride + 1);
+  }
+  return err;
+}
+
+static int forward_pair2(int32_t *s, uint64_t i0, uint64_t i1) {
+  int32_t v[2] = {s[i0], s[i1]};
+  int err = jxro_forward_overlap_2(v);
+  s[i0] = v[0];
+  s[i1] = v[1];
+  return err;
+}
+/* The ordered subsampled steps of one boundary row undone last to first. */
+static int forward_boundary_row(int32_t *s, uint64_t stride, uint32_t L, uint32_t R, uint32_t row) {
+  int err = 0, *e = &err;
+  uint64_t c0 = row * stride + L, n0 = row * stride + L + 1;
+  uint64_t c1 = row * stride + R - 1, n1 = row * stride + R - 2;
+  s[c1] = SUB(s[c1], s[n1]);
+  s[c0] = SUB(s[c0], s[n0]);
+  for (uint32_t x = L + 2; x < R; x += 2) err |= forward_pair2(s, row * stride + x - 1, row * stride + x);
+  s[c1] = ADD(s[c1], s[n1]);
+  s[c0] = ADD(s[c0], s[n0]);
+  return err;
+}
+/* subsampled_region run backwards: its boundary rows are the first and last
+ * steps, and the steps between touch disjoint samples. */
+static int forward_subsampled_region(int32_t *s, uint64_t stride, uint32_t L, uint32_t R, uint32_t T,
+                                     uint32_t B) {
+  int err = forward_boundary_row(s, stride, L, R, B - 1);
+  for (uint32_t y = T + 2; y < B; y += 2) {
+    for (uint32_t x = L + 2; x < R; x += 2) {
+      int32_t v[4] = {s[(y - 1) * stride + x - 1], s[(y - 1) * stride + x], s[y * stride + x - 1],
+                      s[y * stride + x]};
+      err |= jxro_forward_overlap_2x2(v);
+      s[(y - 1) * stride + x - 1] = v[0];
+      s[(y - 1) * stride + x] = v[1];
+      s[y * stride + x - 1] = v[2];
+      s[y * stride + x] = v[3];
+    }
+    err |= forward_pair2(s, (y - 1) * stride + L, y * stride + L);
+    err |= forward_pair2(s, (y - 1) * stride + R - 1, y * stride + R - 1);
+  }
+  err |= forward_boundary_row(s, stride, L, R, T);
+  return err;
+}
+
+/* Forward analysis at unit steps of a component of mbx x mby macroblocks
+ * whose samples are `samples` (row-major, mbx*bc*4 across), with hard
+ * regions at the macroblock boundaries cb x rb: the stages reconstruction
+ * inverts, run backwards (analyze.rs:56-86): sample prefilter, block
+ * transforms, low-pass prefilter, first-level transforms. A component whose
+ * low bands are doubled after the first level divides them by two,
+ * rounding the magnitude half up (analyze.rs:500-530 at unit steps,
+ * quantization.rs:27-48). Writes 256 coefficients per macroblock, raster
+ * order, block k's at 16k with its DC/low-pass coefficient at 16k. */
+int jxro_analyze_plane(const int32_t *samples, uint32_t mbx, uint32_t mby, uint32_t bc, uint32_t br,
+                       int32_t doubled, uint32_t overlap, const uint32_t *cb, uint32_t nc, const uint32_t *rb,
+                       uint32_t nr, int32_t *out) {
+  if (!((bc == 4 && br == 4) || (bc == 2 && br == 4) || (bc == 2 && br == 2)) || nc < 2 || nr < 2) return 1;
+  const uint32_t sw = mbx * bc * 4, sh = mby * br * 4, lw = mbx * bc, lh = mby * br;
+  int32_t *s = (int32_t *)malloc(sizeof(int32_t) * sw * sh);
+  int32_t *low = (int32_t *)malloc(sizeof(int32_t) * lw * lh);
+  int32_t *high = (int32_t *)malloc(sizeof(int32_t) * 16 * lw * lh);
+  int err = !s || !low || !high;
+  if (!err) {
+    memcpy(s, samples, sizeof(int32_t) * sw * sh);
+    for (uint32_t j = 0; j + 1 < nr && overlap >= 1; j++)
+      for (uint32_t i = 0; i + 1 < nc; i++)
+        err |= forward_full_region(s, sw, cb[i] * bc * 4, cb[i + 1] * bc * 4, rb[j] * br * 4, rb[j + 1] * br * 4);
+    for (uint32_t by = 0; by < lh; by++)
+      for (uint32_t bx = 0; bx < lw; bx++) {
+        int32_t v[16];
+        for (int r = 0; r < 4; r++)
+          for (int c = 0; c < 4; c++) v[r * 4 + c] = s[(uint64_t)(by * 4 + r) * sw + bx * 4 + c];
+        err |= jxro_forward_core_transform(v);
+        low[(uint64_t)by * lw + bx] = v[0];
+        memcpy(high + 16 * ((uint64_t)by * lw + bx), v, sizeof v);
+      }
+    for (uint32_t j = 0; j + 1 < nr && overlap == 2; j++)
+      for (uint32_t i = 0; i + 1 < nc; i++) {
+        uint32_t L = cb[i] * bc, R = cb[i + 1] * bc, T = rb[j] * br, B = rb[j + 1] * br;
+        err |= bc == 4 ? forward_full_region(low, lw, L, R, T, B) : forward_subsampled_region(low, lw, L, R, T, B);
+      }
+    for (uint32_t my = 0; my < mby; my++)
+      for (uint32_t mx = 0; mx < mbx; mx++) {
+        int32_t v[16] = {0};
+        for (uint32_t r = 0; r < br; r++)
+          for (uint32_t c = 0; c < bc; c++) v[r * bc + c] = low[(uint64_t)(my * br + r) * lw + mx * bc + c];
+        if (bc == 4)
+          err |= jxro_forward_core_transform(v);
+        else if (br == 4)
+          err |= jxro_forward_chroma_422(v);
+        else
+          err |= jxro_forward_chroma_420(v);
+        int32_t *m = out + ((uint64_t)my * mbx + mx) * 256;
+        memset(m, 0, 256 * sizeof(int32_t));
+        for (uint32_t k = 0; k < bc * br; k++) {
+          uint32_t bx = mx * bc + k % bc, by = my * br + k / bc;
+          memcpy(m + 16 * k, high + 16 * ((uint64_t)by * lw + bx), 16 * sizeof(int32_t));
+          int32_t q = v[k];
+          if (doubled) {
+            uint32_t mag = q < 0 ? 0u - (uint32_t)q : (uint32_t)q;
+            uint32_t h = mag / 2 + (mag & 1u);
+            q = q < 0 ? -(int32_t)h : (int32_t)h;
+          }
+          m[16 * k] = q;
+        }
+      }
+  }
+  free(s);
+  free(low);
+  free(high);
+  return err;
+}
+
+/* The DC and low-pass coefficients of the flat probe component of `level`
+ * (16 a macroblock: block k's at k). */
+static int analyze_flat(int32_t level, uint32_t bc, uint32_t br, int doubled, uint32_t overlap,
+                        const probe_axis_t *cols, const probe_axis_t *rows, int32_t *out) {
+  const uint32_t mbx = cols->mbs, mby = rows->mbs, n = mbx * bc * 4 * mby * br * 4;
+  int32_t *flat = (int32_t *)malloc(sizeof(int32_t) * n);
+  int32_t *full = (int32_t *)malloc(sizeof(int32_t) * 256 * mbx * mby);
+  int err = !flat || !full;
+  if (!err) {
+    for (uint32_t i = 0; i < n; i++) flat[i] = level;
+    err = jxro_analyze_plane(flat, mbx, mby, bc, br, doubled, overlap, cols->bounds, cols->nb, rows->bounds,
+                             rows->nb, full);
+    for (uint32_t m = 0; m < mbx * mby && !err; m++)
+      for (uint32_t k = 0; k < 16; k++) out[m * 16 + k] = full[m * 256 + 16 * k];
+  }
+  free(flat);
+  free(full);
+  return err;
+}
+
+/* The unscaled reduced picture of a plane: the block DCs after the
+ * first-level transform and its overlap filter (a quarter, block_dc_plane);
+ * at a sixteenth the dequantized macroblock DC (doubled where the plane's
+ * low bands are), or under two overlap levels the quarter picture of the DC
+ * band summed over each macroblock (macroblock_dc_plane, sum_macroblocks). */
+static int reduced_picture(plane_t *p, uint32_t *stage) {
+  const jxrhip_arena *a = p->arena;
+  const uint32_t overlap = a->overlap;
+  if (p->resolution == JXRHIP_SCALE_SIXTEENTH && overlap != 2) {
+    for (uint32_t m = 0; m < p->mbx * p->mby; m++) {
+      uint32_t i = p->raster[m];
+      int err = 0, *e = &err;
+      int32_t dc = dequant(a->coefficients[a->coefficient_offsets[i]], a->quantizers[3 * i], e);
+      if (p->scale_after_first) dc = MUL(dc, 2);
+      if (err) {
+        *stage = JXRHIP_STAGE_FIRST_TRANSFORM;
+        return 1;
+      }
+      p->samples[m] = dc;
+    }
+    return 0;
+  }
+  const int dc_only = p->resolution == JXRHIP_SCALE_SIXTEENTH;
+  if (first_level(p, dc_only)) {
+    *stage = JXRHIP_STAGE_FIRST_TRANSFORM;
+    return 1;
+  }
+  if (overlap == 2 && plane_overlap(p, 1)) {
+    *stage = JXRHIP_STAGE_OVERLAP;
+    return 1;
+  }
+  if (!dc_only) {
+    memcpy(p->samples, p->low, sizeof(int32_t) * p->lw * p->lh);
+    return 0;
+  }
+  for (uint32_t my = 0; my < p->mby; my++)
+    for (uint32_t mx = 0; mx < p->mbx; mx++) {
+      int err = 0, *e = &err;
+      int32_t sum = 0;
+      for (uint32_t r = 0; r < p->br; r++)
+        for (uint32_t c = 0; c < p->bc; c++) sum = ADD(sum, p->low[(uint64_t)(my * p->br + r) * p->lw + mx * p->bc + c]);
+      if (err) {
+        *stage = JXRHIP_STAGE_FIRST_TRANSFORM;
+        return 1;
+      }
+      p->samples[(uint64_t)my * p->mbx + mx] = sum;
+    }
+  return 0;
+}
+
+/* The unscaled reduced picture of the flat probe component of `level`,
+ * geometry `cols` x `rows` (hard regions at the probe's boundaries), with
+ * plane p's block shape, doubling, overlap and scale. */
+static int probe_picture(const plane_t *p, const probe_axis_t *cols, const probe_axis_t *rows, int32_t level,
+                         int32_t *out) {
+  const uint32_t mbs = cols->mbs * rows->mbs, blocks = p->bc * p->br;
+  int32_t *flat = (int32_t *)malloc(sizeof(int32_t) * 16 * mbs);
+  int32_t *coef = (int32_t *)malloc(sizeof(int32_t) * blocks * mbs);
+  uint32_t *offsets = (uint32_t *)malloc(sizeof(uint32_t) * mbs);
+  uint32_t *quant = (uint32_t *)malloc(sizeof(uint32_t) * 3 * mbs);
+  uint8_t *bands = (uint8_t *)malloc(mbs), *preds = (uint8_t *)malloc(mbs);
+  uint32_t *cx = (uint32_t *)malloc(sizeof(uint32_t) * mbs), *cy = (uint32_t *)malloc(sizeof(uint32_t) * mbs);
+  uint32_t tw[PROBE_EXTENT + 1], th[PROBE_EXTENT + 1];
+  int r = !flat || !coef || !offsets || !quant || !bands || !preds || !cx || !cy;
+  if (!r) r = analyze_flat(level, p->bc, p->br, p->scale_after_first, p->arena->overlap, cols, rows, flat);
+  if (!r) {
+    for (uint32_t m = 0; m < mbs; m++) {
+      for (uint32_t k = 0; k < blocks; k++) coef[m * blocks + k] = flat[m * 16 + k];
+      offsets[m] = m * blocks;
+      quant[3 * m] = quant[3 * m + 1] = quant[3 * m + 2] = 1;
+      bands[m] = JXRHIP_BANDS_NO_HIGH_PASS;
+      preds[m] = JXRHIP_PRED_NONE;
+      cx[m] = m % cols->mbs;
+      cy[m] = m / cols->mbs;
+    }
+    for (uint32_t i = 0; i + 1 < cols->nb; i++) tw[i] = cols->bounds[i + 1] - cols->bounds[i];
+    for (uint32_t j = 0; j + 1 < rows->nb; j++) th[j] = rows->bounds[j + 1] - rows->bounds[j];
+    jxrhip_coefficient_plane cp = {0, (uint64_t)blocks * mbs, 0, mbs, p->bc, p->br};
+    jxrhip_arena a;
+    memset(&a, 0, sizeof a);
+    a.coefficients = coef;
+    a.coefficient_count = (uint64_t)blocks * mbs;
+    a.macroblock_count = mbs;
+    a.coefficient_offsets = offsets;
+    a.quantizers = quant;
+    a.bands = bands;
+    a.hp_predictions = preds;
+    a.coded_x = cx;
+    a.coded_y = cy;
+    a.planes = &cp;
+    a.plane_count = 1;
+    a.overlap = p->arena->overlap;
+    a.hard_tiles = 1;
+    a.tile_column_count = cols->nb - 1;
+    a.tile_column_widths = tw;
+    a.tile_row_count = rows->nb - 1;
+    a.tile_row_heights = th;
+    plane_t q;
+    r = plane_setup(&q, &a, 0, p->scale_after_first, p->alpha, p->resolution, NULL, 0);
+    if (!r) {
+      uint32_t st = 0;
+      r = reduced_picture(&q, &st);
+      if (!r) memcpy(out, q.samples, sizeof(int32_t) * q.sw * q.sh);
+      plane_free(&q);
+    }
+  }
+  free(flat);
+  free(coef);
+  free(offsets);
+  free(quant);
+  free(bands);
+  free(preds);
+  free(cx);
+  free(cy);
+  return r;
+}
+
+/* floor(n / d) for d > 0 (i64::div_euclid) */
+static int64_t floor_div(int64_t n, int64_t d) {
+  int64_t q = n / d;
+  return (n % d < 0) ? q - 1 : q;
+}
+
+/* SampleScale (sample_scale.rs:157-285): without overlap the gain of the
+ * transform stages left out, as a shift; with it, the line through the
+ * reduced pictures of flat probes at opposite levels, per probe sample,
+ * each sample of the plane taking the probe sample placed like it. */
+static int scale_reduced(plane_t *p, uint32_t *stage) {
+  const uint32_t n = p->sw * p->sh;
+  const uint32_t mb_gain = (p->bc == 4 && p->br == 4) ? 2 : 1;
+  const uint32_t bits = p->resolution == JXRHIP_SCALE_QUARTER ? 2 : mb_gain + 2;
+  if (p->arena->overlap == 0) {
+    for (uint32_t i = 0; i < n; i++) p->samples[i] >>= bits;
+    return 0;
+  }
+  probe_axis_t cols, rows;
+  if (probe_axis(p->col_bounds, p->ncol, &cols)) {
+    *stage = JXRHIP_STAGE_SAMPLE_SCALE;
+    return 1;
+  }
+  if (probe_axis(p->row_bounds, p->nrow, &rows)) {
+    free(cols.pos);
+    *stage = JXRHIP_STAGE_SAMPLE_SCALE;
+    return 1;
+  }
+  const uint32_t ux = p->ux, uy = p->uy;
+  const uint32_t stride = cols.mbs * ux, length = stride * rows.mbs * uy;
+  int64_t *rise = (int64_t *)calloc(length, sizeof(int64_t));
+  int64_t *offset = (int64_t *)calloc(length, sizeof(int64_t));
+  int32_t *up = (int32_t *)malloc(sizeof(int32_t) * length), *down = (int32_t *)malloc(sizeof(int32_t) * length);
+  int r = !rise || !offset || !up || !down;
+  int64_t span = 0, count = 0;
+  for (int k = 0; k < 4 && !r; k++) {
+    r = probe_picture(p, &cols, &rows, PROBE_LEVELS[k], up) || probe_picture(p, &cols, &rows, -PROBE_LEVELS[k], down);
+    for (uint32_t i = 0; i < length && !r; i++) {
+      rise[i] += (int64_t)up[i] - down[i];
+      offset[i] += (int64_t)up[i] + down[i];
+    }
+    span += 2 * (int64_t)PROBE_LEVELS[k];
+    count += 2;
+  }
+  const int64_t gain = count * span;
+  for (uint32_t y = 0; y < p->sh && !r; y++) {
+    const uint32_t py = rows.pos[y / uy] * uy + y % uy;
+    for (uint32_t x = 0; x < p->sw && !r; x++) {
+      const uint32_t px = cols.pos[x / ux] * ux + x % ux;
+      const int64_t rs = rise[(uint64_t)py * stride + px], os = offset[(uint64_t)py * stride + px];
+      /* FlatTerms (sample_scale.rs:228-245): divisor = count * rise > 0,
+       * bias = offset * span - divisor / 2; level = floor((c * gain - bias)
+       * / divisor), zero for zero (sample_scale.rs:300-315) */
+      const int64_t divisor = count * rs;
+      if (divisor <= 0) {
+        r = 1;
+        break;
+      }
+      const int64_t bias = os * span - divisor / 2;
+      int32_t *v = &p->samples[(uint64_t)y * p->sw + x];
+      if (*v == 0) continue;
+      const int64_t level = floor_div((int64_t)*v * gain - bias, divisor);
+      if (level > INT32_MAX || level < INT32_MIN) {
+        r = 1;
+        break;
+      }
+      *v = (int32_t)level;
+    }
+  }
+  free(rise);
+  free(offset);
+  free(up);
+  free(down);
+  free(cols.pos);
+  free(rows.pos);
+  if (r) *stage = JXRHIP_STAGE_SAMPLE_SCALE;
+  return r;
+}
+
+static int reconstruct_reduced(plane_t *p, uint32_t *stage) {
+  if (reduced_picture(p, stage)) return JXRHIP_ERR_KERNEL_ARITHMETIC;
+  if (scale_reduced(p, stage)) return JXRHIP_ERR_KERNEL_ARITHMETIC;
+  return 0;
+}
+
 static int reconstruct(plane_t *p, uint32_t *stage) {
-  if (first_level(p)) {
+  if (p->resolution != JXRHIP_SCALE_FULL) return reconstruct_reduced(p, stage);
+  if (first_level(p, 0)) {
     *stage = JXRHIP_STAGE_FIRST_TRANSFORM;
     return JXRHIP_ERR_KERNEL_ARITHMETIC;
   }
@@ -1088,6 +1543,7 @@ static int image_setup(image_t *im, const jxrhip_image_desc *d, char *msg, size_
   im->alpha_index = -1;
   if (d->abi_version != JXRHIP_ABI_VERSION) FAIL(JXRHIP_ERR_INVALID_ARGUMENT, "ABI version");
   if (!d->primary) FAIL(JXRHIP_ERR_INVALID_PLAN, "no primary arena");
+  if (d->scale > JXRHIP_SCALE_SIXTEENTH) FAIL(JXRHIP_ERR_INVALID_PLAN, "decode scale");
   uint32_t colors = d->component_count;
   uint32_t cf = d->color_format;
   uint32_t expect = cf == JXRHIP_COLOR_LUMA ? 1
@@ -1105,7 +1561,7 @@ static int image_setup(image_t *im, const jxrhip_image_desc *d, char *msg, size_
   for (uint32_t i = 0; i < colors + integrated; i++) {
     int alpha = integrated && i == colors;
     int scale = !alpha && d->scaled && i != 0;
-    int r = plane_setup(&im->p[i], d->primary, i, scale, alpha, msg, msglen);
+    int r = plane_setup(&im->p[i], d->primary, i, scale, alpha, d->scale, msg, msglen);
     if (r) {
       image_free(im);
       return r;
@@ -1118,7 +1574,7 @@ static int image_setup(image_t *im, const jxrhip_image_desc *d, char *msg, size_
       image_free(im);
       FAIL(JXRHIP_ERR_INVALID_PLAN, "separate alpha arena is not one independent luma plane");
     }
-    int r = plane_setup(&im->p[im->count], d->separate_alpha, 0, 0, 1, msg, msglen);
+    int r = plane_setup(&im->p[im->count], d->separate_alpha, 0, 0, 1, d->scale, msg, msglen);
     if (r) {
       image_free(im);
       return r;
@@ -1153,6 +1609,7 @@ typedef struct {
   channel_scale alpha_scale;
   int32_t u16_max;
   int upsample; /* 0 none, 1 = 4:2:0, 2 = 4:2:2 */
+  uint32_t luma_mb; /* luma samples across a macroblock at the decode scale */
   int kind;     /* OUT_* */
   uint32_t depth;
 } out_ctx;
@@ -1168,7 +1625,7 @@ static int layout_has_alpha(uint32_t l) {
 
 /* sample of a full-resolution plane at coded position (x, y) */
 static int32_t plane_at(const plane_t *p, uint32_t x, uint32_t y) {
-  uint32_t lx = x - p->mb_ox * p->bc * 4, ly = y - p->mb_oy * p->br * 4;
+  uint32_t lx = x - p->mb_ox * p->ux, ly = y - p->mb_oy * p->uy;
   return p->samples[(uint64_t)ly * p->sw + lx];
 }
 
@@ -1185,8 +1642,8 @@ static int32_t chroma_clamped(const plane_t *p, int64_t cx, int64_t cy) {
  * 174-325). The plane's end samples stand in beyond its edges. */
 static int32_t chroma_at(const out_ctx *o, const plane_t *p, uint32_t x, uint32_t y) {
   const jxrhip_output_format *f = &o->d->output;
-  int64_t lx = (int64_t)x - (int64_t)p->mb_ox * 16;
-  int64_t ly = (int64_t)y - (int64_t)p->mb_oy * 16;
+  int64_t lx = (int64_t)x - (int64_t)p->mb_ox * o->luma_mb;
+  int64_t ly = (int64_t)y - (int64_t)p->mb_oy * o->luma_mb;
   int64_t cx = lx >> 1;
   int32_t pair[2];
   if (o->upsample == 2) {
@@ -1349,6 +1806,9 @@ static int output_setup(out_ctx *o, const jxrhip_image_desc *d, image_t *im, cha
       FAIL(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "planar YUV output keeps the coded 4:2:0 or 4:2:2 sampling");
     if (f->premultiply_alpha) FAIL(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "premultiplied YUV output");
     if (!d->surface.planar) FAIL(JXRHIP_ERR_INVALID_PLAN, "YUV output is planar");
+    /* prepare.rs:185-192: a sixteenth has one chroma sample per macroblock */
+    if (d->scale == JXRHIP_SCALE_SIXTEENTH)
+      FAIL(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "sixteenth-resolution subsampled planar YUV output");
     o->kind = OUT_YUV;
   } else if (dep == JXRHIP_DEPTH_BIT1_WHITE || dep == JXRHIP_DEPTH_BIT1_BLACK || dep == JXRHIP_DEPTH_RGB555 ||
              dep == JXRHIP_DEPTH_RGB565 || dep == JXRHIP_DEPTH_RGB101010) {
@@ -1372,7 +1832,14 @@ static int output_setup(out_ctx *o, const jxrhip_image_desc *d, image_t *im, cha
     FAIL(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "T.832 output colour and bit depth matrix");
   o->u16_max = dep == JXRHIP_DEPTH_U10 ? 1023 : 65535;
   uint32_t cf = d->color_format;
-  o->upsample = o->kind == OUT_YUV ? 0 : cf == JXRHIP_COLOR_YUV420 ? 1 : cf == JXRHIP_COLOR_YUV422 ? 2 : 0;
+  /* interpolates_chroma (metadata.rs:253-263): not for native planar
+   * output, nor at a sixteenth, where chroma has a sample per macroblock as
+   * luma does (reconstructed_color, decode/output.rs:464-482) */
+  o->upsample = o->kind == OUT_YUV || d->scale == JXRHIP_SCALE_SIXTEENTH ? 0
+                : cf == JXRHIP_COLOR_YUV420                            ? 1
+                : cf == JXRHIP_COLOR_YUV422                            ? 2
+                                                                       : 0;
+  o->luma_mb = d->scale == JXRHIP_SCALE_QUARTER ? 4 : d->scale == JXRHIP_SCALE_SIXTEENTH ? 1 : 16;
   if (o->upsample && (f->chroma_centering_x > 4 || (o->upsample == 1 && f->chroma_centering_y > 4)))
     FAIL(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "unknown chroma centering has no exact reconstruction filter");
   for (uint32_t c = 0; c < MAXP; c++) {
@@ -1415,13 +1882,12 @@ static int output_setup(out_ctx *o, const jxrhip_image_desc *d, image_t *im, cha
   /* crop inside every plane's window (plan.rs:282-354) */
   for (uint32_t i = 0; i < im->count; i++) {
     const plane_t *p = &im->p[i];
-    int sub = cf == JXRHIP_COLOR_YUV420 ? 1 : cf == JXRHIP_COLOR_YUV422 ? 2 : 0;
-    int chroma = (int)i < (int)im->colors && (i == 1 || i == 2) && sub;
-    uint32_t dx = chroma ? 2 : 1, dy = (chroma && sub == 1) ? 2 : 1;
+    /* a plane's samples per luma sample: its macroblock over luma's */
+    uint32_t dx = o->luma_mb / p->ux, dy = o->luma_mb / p->uy;
     uint64_t left = f->crop_x / dx, top = f->crop_y / dy;
     uint64_t right = ((uint64_t)f->crop_x + f->crop_width - 1) / dx + 1;
     uint64_t bottom = ((uint64_t)f->crop_y + f->crop_height - 1) / dy + 1;
-    uint64_t px = (uint64_t)p->mb_ox * p->bc * 4, py = (uint64_t)p->mb_oy * p->br * 4;
+    uint64_t px = (uint64_t)p->mb_ox * p->ux, py = (uint64_t)p->mb_oy * p->uy;
     if (left < px || top < py || right > px + p->sw || bottom > py + p->sh)
       FAIL(JXRHIP_ERR_INVALID_PLAN, "output crop is outside a sample plane");
   }
@@ -1725,8 +2191,8 @@ int jxro_reconstruct_plane(const jxrhip_image_desc *d, uint32_t index, int32_t *
   if (!r) {
     *width = p->sw;
     *height = p->sh;
-    *origin_x = p->mb_ox * p->bc * 4;
-    *origin_y = p->mb_oy * p->br * 4;
+    *origin_x = p->mb_ox * p->ux;
+    *origin_y = p->mb_oy * p->uy;
     if ((uint64_t)p->sw * p->sh > capacity)
       r = JXRHIP_ERR_BUFFER_TOO_SMALL;
     else

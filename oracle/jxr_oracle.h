@@ -73,6 +73,25 @@ int jxro_float16_bits(int32_t sample, uint32_t scaled, uint16_t *bits);
 int jxro_convert(uint32_t internal, uint32_t output, const int32_t in[4], int32_t packed_rgb,
                  int32_t red_blue_not_swapped, int32_t out[4]);
 
+/* ProbeAxis (sample_scale.rs:336-381) of an axis whose regions have the
+ * macroblock boundaries `regions` (count entries, from 0): the probe's
+ * region boundaries and, for each macroblock, its probe macroblock.
+ * 0 ok, 1 a partition that does not advance or more positions than
+ * `capacity`. */
+int jxro_probe_axis(const uint32_t *regions, uint32_t count, uint32_t *bounds, uint32_t *bound_count,
+                    uint32_t *positions, uint32_t capacity);
+
+/* Forward analysis at unit steps of one component (mbx x mby macroblocks
+ * of bc x br blocks; samples row-major, mbx*bc*4 across) with hard regions
+ * at the macroblock boundaries column_bounds x row_bounds (from 0 to the
+ * extent): the exact inverse of its full reconstruction, as
+ * analyze_component (analyze.rs:56-86) runs it for the sample-scale probe.
+ * `doubled` halves the DC and low-pass coefficients, rounding the magnitude
+ * half up. Writes 256 coefficients per macroblock (block k at 16k). */
+int jxro_analyze_plane(const int32_t *samples, uint32_t mbx, uint32_t mby, uint32_t bc, uint32_t br,
+                       int32_t doubled, uint32_t overlap, const uint32_t *column_bounds, uint32_t column_count,
+                       const uint32_t *row_bounds, uint32_t row_count, int32_t *coefficients);
+
 /* Postfilter over a whole plane with the given tile boundaries (sample
  * units, including 0 and the extent), as reconstruct/overlap.rs:124-161. */
 int jxro_overlap_plane(int32_t *samples, uint32_t width, uint32_t height,

@@ -133,3 +133,81 @@ def row_of_pixels(values_per_plane, **kw):
     n = len(values_per_plane[0])
     planes = [np.array(vals, np.int64).reshape(1, n) for vals in values_per_plane]
     return ManualArena(planes, **kw), n
+
+
+class BlockArena:
+    """Arena of explicit macroblocks, one list per plane. A macroblock is
+    (coefficients, bands, (dc, lp, hp) steps) with its 256 coefficients
+    block by block (block k's DC/low-pass value at 16k); the arena stores
+    what its bands code (1, blocks or blocks*16 values). Macroblocks are in
+    raster order over mbx x mby, placed at `origin` in the coded image, whose
+    tile grid `tiles` = (column widths, row heights) must cover the
+    window."""
+
+    def __init__(self, planes, geometry, mbx, mby, overlap=0, tiles=None, hard_tiles=False, origin=(0, 0)):
+        n = len(planes)
+        coef, offsets, quant, bands, cx, cy = [], [], [], [], [], []
+        self.planes = (abi.CoefficientPlane * n)()
+        for i, mbs in enumerate(planes):
+            bc, br = geometry[i]
+            blocks = bc * br
+            cp = self.planes[i]
+            cp.coefficient_offset = len(coef)
+            cp.macroblock_offset = len(offsets)
+            cp.block_columns, cp.block_rows = bc, br
+            assert len(mbs) == mbx * mby
+            for m, (c, b, q) in enumerate(mbs):
+                c = np.asarray(c, np.int64).reshape(-1)
+                offsets.append(len(coef))
+                if b == abi.BANDS_DC_ONLY:
+                    coef.append(int(c[0]))
+                elif b == abi.BANDS_NO_HIGH_PASS:
+                    coef.extend(int(c[16 * k]) for k in range(blocks))
+                else:
+                    coef.extend(int(v) for v in c[: blocks * 16])
+                quant.extend(q)
+                bands.append(b)
+                cx.append(origin[0] + m % mbx)
+                cy.append(origin[1] + m // mbx)
+            cp.coefficient_count = len(coef) - cp.coefficient_offset
+            cp.macroblock_count = len(offsets) - cp.macroblock_offset
+        self.coef = np.array(coef if coef else [0], np.int32)
+        self.offsets = np.array(offsets, np.uint32)
+        self.quant = np.array(quant, np.uint32)
+        self.bands = np.array(bands, np.uint8)
+        self.preds = np.zeros(len(bands), np.uint8)
+        self.cx = np.array(cx, np.uint32)
+        self.cy = np.array(cy, np.uint32)
+        cols, rows = tiles if tiles else ([origin[0] + mbx], [origin[1] + mby])
+        self.tw = np.array(cols, np.uint32)
+        self.th = np.array(rows, np.uint32)
+        a = abi.Arena()
+        u32 = lambda arr: arr.ctypes.data_as(abi.u32p)
+        a.coefficients = self.coef.ctypes.data_as(C.POINTER(C.c_int32))
+        a.coefficient_count = len(coef)
+        a.macroblock_count = len(offsets)
+        a.coefficient_offsets = u32(self.offsets)
+        a.quantizers = u32(self.quant)
+        a.bands = self.bands.ctypes.data_as(C.POINTER(C.c_uint8))
+        a.hp_predictions = self.preds.ctypes.data_as(C.POINTER(C.c_uint8))
+        a.coded_x = u32(self.cx)
+        a.coded_y = u32(self.cy)
+        a.planes = self.planes
+        a.plane_count = n
+        a.overlap = overlap
+        a.hard_tiles = 1 if hard_tiles else 0
+        a.tile_column_count = len(self.tw)
+        a.tile_column_widths = u32(self.tw)
+        a.tile_row_count = len(self.th)
+        a.tile_row_heights = u32(self.th)
+        a.identity = next(_ids)
+        self.arena = a
+        self.mbx, self.mby = mbx, mby
+
+
+def dc_macroblock(value, step=1, bands=abi.BANDS_DC_ONLY):
+    """A macroblock whose DC is `value` and every other coefficient zero
+    (the reference tests' dc_macroblock)."""
+    c = np.zeros(256, np.int64)
+    c[0] = value
+    return (c, bands, (step, 1, 1))

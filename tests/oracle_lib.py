@@ -55,6 +55,11 @@ def lib():
     L.jxro_reconstruct_plane.restype = C.c_int
     L.jxro_reconstruct_plane.argtypes = [C.POINTER(abi.ImageDesc), C.c_uint32, i32p, C.c_uint64, u32p, u32p,
                                          u32p, u32p, u32p]
+    L.jxro_probe_axis.restype = C.c_int
+    L.jxro_probe_axis.argtypes = [u32p, C.c_uint32, u32p, u32p, u32p, C.c_uint32]
+    L.jxro_analyze_plane.restype = C.c_int
+    L.jxro_analyze_plane.argtypes = [i32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32,
+                                     u32p, C.c_uint32, u32p, C.c_uint32, i32p]
     L.jxro_decode_many.restype = C.c_int
     L.jxro_decode_many.argtypes = [C.POINTER(C.POINTER(abi.ImageDesc)), C.POINTER(C.c_void_p),
                                    C.POINTER(C.c_uint64), C.c_uint32, C.c_uint32]
@@ -127,3 +132,34 @@ def convert(internal, output, values, packed_rgb=False, red_blue_not_swapped=Tru
     code = lib().jxro_convert(internal, output, inp.ctypes.data_as(C.POINTER(C.c_int32)), 1 if packed_rgb else 0,
                               1 if red_blue_not_swapped else 0, out.ctypes.data_as(C.POINTER(C.c_int32)))
     return out.tolist(), code
+
+
+def probe_axis(regions):
+    """(bounds, positions) of the sample-scale probe axis of regions with
+    these macroblock boundaries, or None when the partition is refused."""
+    r = np.ascontiguousarray(np.array(regions, np.uint32))
+    bounds = np.zeros(16, np.uint32)
+    nb = C.c_uint32()
+    cap = max(int(r[-1]) if r.size else 0, 1)
+    pos = np.zeros(cap, np.uint32)
+    u32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
+    if lib().jxro_probe_axis(u32(r), r.size, u32(bounds), C.byref(nb), u32(pos), cap):
+        return None
+    total = int(r[-1] - r[0])
+    return bounds[: nb.value].tolist(), pos[:total].tolist()
+
+
+def analyze_plane(samples, geometry, overlap, column_bounds, row_bounds, doubled=False):
+    """Forward analysis at unit steps: (mby, mbx, 256) coefficients."""
+    bc, br = geometry
+    s = np.ascontiguousarray(np.asarray(samples, np.int32))
+    mby, mbx = s.shape[0] // (br * 4), s.shape[1] // (bc * 4)
+    out = np.zeros((mby, mbx, 256), np.int32)
+    cb = np.ascontiguousarray(np.array(column_bounds, np.uint32))
+    rb = np.ascontiguousarray(np.array(row_bounds, np.uint32))
+    u32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
+    i32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    if lib().jxro_analyze_plane(i32(s), mbx, mby, bc, br, 1 if doubled else 0, overlap, u32(cb), cb.size, u32(rb),
+                                rb.size, i32(out)):
+        raise OracleError(1, 0, "analyze_plane")
+    return out
