@@ -18,6 +18,7 @@ from .decoder import (
     image_desc,
     plan_hip_route,
     quality_parameters,
+    reduced_crop,
     surface_layout,
 )
 from .serving import HipServer, ServerResult
@@ -25,5 +26,5 @@ from .serving import HipServer, ServerResult
 __all__ = [
     "abi", "JxrHipError", "load", "EncodedImage", "HipBatch", "HipDecodePlan", "HipDecoderSession",
     "HipSubmission", "OutputSpec", "ResidentHipImage", "image_desc", "plan_hip_route", "quality_parameters",
-    "surface_layout", "HipServer", "ServerResult",
+    "surface_layout", "reduced_crop", "HipServer", "ServerResult",
 ]

@@ -5,8 +5,10 @@ forward analysis are linked into the same shared library. Packed-FP32
 instructions are disabled for every HIP compilation.
 """
 import os
+import shutil
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -17,23 +19,33 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 
 def build(verbose: bool = False) -> str:
-    """Compiles every source into LIB (always, even when it exists)."""
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [
-        HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    """Compiles every source into LIB (always, even when it exists): the
+    sources in parallel, then one link."""
+    tmpdir = tempfile.mkdtemp(prefix="jxr_hip_build_")
+    flags = [
+        HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-Wall", "-Wno-unused-function",
         "-I", os.path.join(ROOT, "include"),
-        *srcs, "-o", tmp,
     ]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    out = res.stdout + res.stderr
-    noise = [l for l in out.splitlines() if "packed-fp32-ops" not in l]
+    objs, procs = [], []
+    for src in SOURCES:
+        obj = os.path.join(tmpdir, src + ".o")
+        objs.append(obj)
+        procs.append(subprocess.Popen([*flags, "-c", os.path.join(CSRC, src), "-o", obj],
+                                      stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
+    outs = [(p.communicate()[0], p.returncode) for p in procs]
+    tmp = LIB + ".tmp"
+    link = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp],
+                          capture_output=True, text=True)
+    outs.append((link.stdout + link.stderr, link.returncode))
+    noise = [l for out, _ in outs for l in out.splitlines() if "packed-fp32-ops" not in l]
     if verbose and noise:
         print("\n".join(noise), file=sys.stderr)
-    if res.returncode != 0:
+    if any(rc != 0 for _, rc in outs):
+        shutil.rmtree(tmpdir, ignore_errors=True)
         raise RuntimeError("hipcc failed:\n" + "\n".join(noise))
+    shutil.rmtree(tmpdir, ignore_errors=True)
     os.replace(tmp, LIB)
     return LIB
 

@@ -540,6 +540,74 @@ void finish_arena(ArenaOut &o, const jxrhip_encode_params &prm) {
 
 }  // namespace
 
+namespace jxr {
+
+// DC and low-pass coefficients of a flat component of `level` at unit
+// steps, of mbx x mby macroblocks of bc x br blocks whose regions end at the
+// macroblock boundaries `cols` x `rows` (from 0 to the extent): the forward
+// stages full reconstruction inverts (sample prefilter, block transforms,
+// low-pass prefilter, first-level transforms; analyze.rs:56-86), which the
+// sample scale of a reduced decode probes (sample_scale.rs:157-213). A
+// component whose low bands reconstruction doubles has them divided by two,
+// its magnitude rounded half up (analyze.rs:500-530 at unit steps).
+// `out` receives bc * br values per macroblock, raster order.
+void flat_probe_coefficients(int32_t level, uint32_t bc, uint32_t br, bool doubled, uint32_t overlap,
+                             const std::vector<uint32_t> &cols, const std::vector<uint32_t> &rows,
+                             std::vector<int32_t> &out) {
+  const uint32_t mbx = cols.back(), mby = rows.back(), blocks = bc * br;
+  const uint32_t sw = mbx * bc * 4, lw = mbx * bc, lh = mby * br;
+  std::vector<i32> s((size_t)sw * mby * br * 4, level);
+  Regions sreg, lreg;
+  for (uint32_t c : cols) {
+    sreg.cols.push_back(c * bc * 4);
+    lreg.cols.push_back(c * bc);
+  }
+  for (uint32_t r : rows) {
+    sreg.rows.push_back(r * br * 4);
+    lreg.rows.push_back(r * br);
+  }
+  if (overlap >= 1) prefilter_full(s, sw, sreg);
+  std::vector<i32> low((size_t)lw * lh);
+  for (uint32_t by = 0; by < lh; by++)
+    for (uint32_t bx = 0; bx < lw; bx++) {
+      i32 v[16];
+      for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) v[r * 4 + c] = s[(size_t)(by * 4 + r) * sw + bx * 4 + c];
+      forward_core(v);
+      low[(size_t)by * lw + bx] = v[0];
+    }
+  if (overlap == 2) {
+    if (bc == 4)
+      prefilter_full(low, lw, lreg);
+    else
+      prefilter_subsampled(low, lw, lreg);
+  }
+  out.assign((size_t)mbx * mby * blocks, 0);
+  for (uint32_t my = 0; my < mby; my++)
+    for (uint32_t mx = 0; mx < mbx; mx++) {
+      i32 v[16] = {0};
+      for (uint32_t r = 0; r < br; r++)
+        for (uint32_t c = 0; c < bc; c++) v[r * bc + c] = low[(size_t)(my * br + r) * lw + mx * bc + c];
+      if (bc == 4)
+        forward_core(v);
+      else if (br == 4)
+        forward_chroma422(v);
+      else
+        forward_chroma420(v);
+      for (uint32_t k = 0; k < blocks; k++) {
+        i32 q = v[k];
+        if (doubled) {
+          const uint32_t mag = q < 0 ? 0u - (uint32_t)q : (uint32_t)q;
+          const uint32_t h = mag / 2 + (mag & 1u);
+          q = q < 0 ? -(i32)h : (i32)h;
+        }
+        out[((size_t)my * mbx + mx) * blocks + k] = q;
+      }
+    }
+}
+
+}  // namespace jxr
+
 struct jxrhip_encoded {
   ArenaOut primary;
   ArenaOut alpha;

@@ -436,10 +436,11 @@ __device__ __forceinline__ uint32_t boundary_bits(const uint32_t *mask, uint32_t
 
 // ---- stage bodies, one work item each ---------------------------------------
 // Dequantize DC/LP and run the first-level inverse transform of macroblock
-// (mx, my), written to the plane's low region.
+// (mx, my), written to the plane's low region. `dc_only` takes the low-pass
+// band as zero, as a sixteenth-size decode reads it (pipeline.rs:488-506).
 __device__ __forceinline__ void first_level_mb(const DevImage &im, const DevPlane &p, const PlaneGeo &g, int *low,
                                uint32_t *tab, int tx, int ty, int tabw, int tabh, int mx, int my,
-                               uint32_t *status) {
+                               bool dc_only, uint32_t *status) {
   const MbRec r = load_mb(p.mbs, (uint32_t)my * im.mbx + (uint32_t)mx);
   // what the second level needs of the record, for macroblocks it reaches
   if ((unsigned)tx < (unsigned)tabw && (unsigned)ty < (unsigned)tabh)
@@ -449,7 +450,7 @@ __device__ __forceinline__ void first_level_mb(const DevImage &im, const DevPlan
   const uint32_t blocks = p.bc * p.br;
   int v[16];
   bool ovf = false;
-  const bool lp = r.bands != JXRHIP_BANDS_DC_ONLY;
+  const bool lp = r.bands != JXRHIP_BANDS_DC_ONLY && !dc_only;
   const uint32_t stride = r.bands >= JXRHIP_BANDS_NO_FLEXBITS ? 16u : 1u;
   // every load is issued before any is used (an absent coefficient reads the
   // DC again), so the sixteen loads overlap instead of waiting one by one
@@ -1731,6 +1732,7 @@ struct PlaneShare {
   int lp, sub, pot;                 // LP overlap, subsampled LP overlap, sample overlap
 };
 
+template <bool R>
 __device__ __forceinline__ PlaneShare plane_share(const DevImage &im, int q, int tmx, int tmy, int nx, int ny,
                                                   int TX, int TY) {
   const DevPlane &p = im.planes[q];
@@ -1739,6 +1741,29 @@ __device__ __forceinline__ PlaneShare plane_share(const DevImage &im, int q, int
   const int lw = (int)im.mbx * bc, lh = (int)im.mby * br;
   const bool lp = p.overlap == 2, sub = lp && bc == 2;
   PlaneShare c;
+  if constexpr (R) {
+    // a reduced decode runs stage A (none at a sixteenth without low-pass
+    // overlap, which reads the macroblock DC alone), stage B, and stage R
+    // over the whole reduced sample region in place of stages C and D
+    const uint32_t ux = reduced_units(p.bc, im.scale), uy = reduced_units(p.br, im.scale);
+    const bool no_first = im.scale == JXRHIP_SCALE_SIXTEENTH && !lp;
+    c.sw = (int)reduced_sample_words(ux, uy, (uint32_t)TX, (uint32_t)TY);
+    c.lw = (int)low_region_words(p.bc, p.br, p.overlap, (uint32_t)TX, (uint32_t)TY);
+    c.tw = (int)mb_table_words((uint32_t)TX, (uint32_t)TY);
+    const int cmx0 = max(tmx - h, 0), cmx1 = min(tmx + nx + h, (int)im.mbx);
+    const int cmy0 = max(tmy - h, 0), cmy1 = min(tmy + ny + h, (int)im.mby);
+    const int corners = (cmx1 - cmx0) * (cmy1 - cmy0) * 2;
+    c.cA = no_first ? 0 : (nx + 2 * h) * (ny + 2 * h);
+    c.cB1 = !lp ? 0 : bc == 4 ? (nx + 1) * (ny + 1) : corners;
+    c.cB2 = sub ? cell_columns(nx) * cell_rows(p, ny) : 0;
+    c.cB3 = sub ? corners : 0;
+    c.cC = c.sw;
+    c.cD = 0;
+    c.lp = lp;
+    c.sub = sub;
+    c.pot = 0;
+    return c;
+  }
   c.sw = (int)sample_region_words(p.bc, p.br, (uint32_t)TX, (uint32_t)TY);
   c.lw = (int)low_region_words(p.bc, p.br, p.overlap, (uint32_t)TX, (uint32_t)TY);
   c.tw = (int)mb_table_words((uint32_t)TX, (uint32_t)TY);
@@ -1761,6 +1786,7 @@ __device__ __forceinline__ PlaneShare plane_share(const DevImage &im, int q, int
 // widths and counts, its regions' place in LDS and its entries of the
 // stages' prefix sums over planes (recomputing the earlier planes' shares
 // in registers rather than waiting for their threads).
+template <bool R>
 __device__ __forceinline__ void plane_setup(TileShared &ts, int pl, uint32_t tile, uint32_t tiles_x, int TX,
                                             int TY) {
   const DevImage &im = ts.im;
@@ -1772,7 +1798,7 @@ __device__ __forceinline__ void plane_setup(TileShared &ts, int pl, uint32_t til
   int ps = 0, pl_ = 0, pt = 0, pA = 0, pB1 = 0, pB2 = 0, pB3 = 0, pC = 0, pD = 0;
   int ts_ = 0, tl = 0, any_lp = 0, any_sub = 0, any_pot = 0;
   for (int q = 0; q < np; q++) {
-    const PlaneShare c = plane_share(im, q, tmx, tmy, nx, ny, TX, TY);
+    const PlaneShare c = plane_share<R>(im, q, tmx, tmy, nx, ny, TX, TY);
     if (q < pl) {
       ps += c.sw;
       pl_ += c.lw;
@@ -1850,6 +1876,25 @@ __device__ __forceinline__ void plane_setup(TileShared &ts, int pl, uint32_t til
   w.mA = div_magic(w.wA);
   w.mB = div_magic(w.wB);
   w.mB2 = div_magic(w.wB2);
+  if constexpr (R) {
+    // reduced samples: the region in reduced units (see reduced_sample_words),
+    // and in ts.br the samples stage R computes: the tile's and, at a
+    // quarter, a one-sample halo (the blocks the low-pass overlap has
+    // finished, all chroma upsampling reads)
+    const uint32_t ux = reduced_units(p.bc, im.scale), uy = reduced_units(p.br, im.scale);
+    const int halo = im.scale == JXRHIP_SCALE_QUARTER ? 1 : 0;
+    g.mw = ux;
+    g.mh = uy;
+    g.pw = im.mbx * ux;
+    g.ph = im.mby * uy;
+    g.sox = ((tmx * (int)ux) & ~3) - 4;
+    g.soy = tmy * (int)uy - kSampleHaloRows;
+    g.srw = reduced_region_width(ux, (uint32_t)TX);
+    g.srh = (uint32_t)TY * uy + 2 * kSampleHaloRows;
+    ts.br[pl] = BlockRange{max(tmx * (int)ux - halo, 0), min((tmx + nx) * (int)ux + halo, (int)g.pw),
+                           max(tmy * (int)uy - halo, 0), min((tmy + ny) * (int)uy + halo, (int)g.ph)};
+    w.wC = g.srw;
+  }
   w.mC = div_magic(w.wC);
   w.mD = div_magic(w.wD);
 }
@@ -1963,7 +2008,57 @@ __device__ __forceinline__ void queue_publish(const Queue &q, uint32_t word, con
 // issued when stage A starts, so its latency hides behind the stage's loads,
 // and the loads that turn it into an item and a slot record follow in
 // stages C and D.
-template <uint32_t STORE>
+// ---- reduced resolution: one sample of the reduced picture ----------------
+// floor(n / d) for d > 0 (i64::div_euclid)
+__device__ __forceinline__ long long floor_div(long long n, long long d) {
+  const long long q = n / d;
+  return (n - q * d) < 0 ? q - 1 : q;
+}
+
+// Reduced sample (x, y) of a plane (reduced window coordinates), scaled, into
+// the plane's sample region (pipeline.rs:175-260,383-486): at a quarter the
+// block DC after the first level and its overlap; at a sixteenth the
+// macroblock's dequantized DC (doubled where the plane's low bands are) or,
+// under two overlap levels, the sum of its blocks' DCs from the DC band.
+// Then the plane's sample scale (sample_scale.rs:255-316). Returns the
+// magnitude bits of the stored sample.
+__device__ __forceinline__ uint32_t reduced_sample(const DevImage &im, const DevPlane &p, const PlaneGeo &g,
+                                                   const JXR_GLOBAL DevReduce *rd, const int *low, int *smp, int x,
+                                                   int y, uint32_t *status) {
+  int v;
+  bool ovf = false;
+  if (im.scale == JXRHIP_SCALE_QUARTER) {
+    v = low[(y - g.loy) * (int)g.lrw + (x - g.lox)];
+  } else if (p.overlap == 2) {
+    const int *b = low + ((y << g.brs) - g.loy) * (int)g.lrw + ((x << g.bcs) - g.lox);
+    v = 0;
+    for (int r = 0; r < (int)p.br; r++)
+      for (int c = 0; c < (int)p.bc; c++) ovf |= __builtin_add_overflow(v, b[r * (int)g.lrw + c], &v);
+  } else {
+    const MbRec r = load_mb(p.mbs, (uint32_t)y * im.mbx + (uint32_t)x);
+    v = dequant_checked(gptr(p.coef)[r.coef_off], r.qdc, ovf);
+    if (p.scale_first) ovf |= __builtin_mul_overflow(v, 2, &v);
+  }
+  if (ovf) fail(status, JXRHIP_STAGE_FIRST_TRANSFORM);
+  const uint32_t mode = rd->mode;
+  if (mode == kReduceShift) {
+    v >>= rd->shift;
+  } else if (mode == kReduceFlat && v != 0) {
+    const JXR_GLOBAL int64_t *t =
+        gptr(rd->terms) + 2 * ((uint64_t)gptr(rd->rows)[y] * rd->stride + gptr(rd->cols)[x]);
+    const long long level = floor_div((long long)v * rd->gain - t[1], t[0]);
+    if (level > INT32_MAX || level < INT32_MIN) {
+      fail(status, JXRHIP_STAGE_SAMPLE_SCALE);
+      v = 0;
+    } else {
+      v = (int)level;
+    }
+  }
+  smp[(y - g.soy) * (int)g.srw + (x - g.sox)] = v;
+  return magbits(v);
+}
+
+template <uint32_t STORE, bool R>
 __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, const KernelArgs &args, int *lds,
                                              TileShared &ts, bool reload, Queue &q, uint32_t &word) {
   __syncthreads();  // the previous tile's output stage has read the tile state
@@ -1974,7 +2069,7 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
     __syncthreads();
   }
   for (int pl = threadIdx.x; pl < (int)ts.im.plane_count; pl += blockDim.x)
-    plane_setup(ts, pl, tile, slot.tiles_x, (int)args.tile_x, (int)args.tile_y);
+    plane_setup<R>(ts, pl, tile, slot.tiles_x, (int)args.tile_x, (int)args.tile_y);
   __syncthreads();
   const DevImage &im = ts.im;
   const int np = (int)im.plane_count;
@@ -1993,7 +2088,8 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
     const int mx = tmx - h + u.r, my = tmy - h + u.q;
     if (mx < 0 || my < 0 || mx >= (int)im.mbx || my >= (int)im.mby) continue;
     first_level_mb(im, p, ts.g[pl], lds + ts.loff[pl], reinterpret_cast<uint32_t *>(lds + ts.toff[pl]),
-                   mx - tmx + 1, my - tmy + 1, ts.tabw, ts.tabh, mx, my, status);
+                   mx - tmx + 1, my - tmy + 1, ts.tabw, ts.tabh, mx, my,
+                   R && im.scale == JXRHIP_SCALE_SIXTEENTH, status);
   }
   __syncthreads();
 
@@ -2035,6 +2131,45 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
 
   queue_next_item(q, args);
 
+  if constexpr (R) {
+    // stage R: every sample of each plane's reduced region; those outside
+    // the samples the tile computes (ts.br) are zero
+    uint32_t rmag = 0;
+    const JXR_GLOBAL DevReduce *red = gptr(im.reduce);
+    const Starts stR = starts_of(ts.sC, np);
+    for (int t = threadIdx.x; t < ts.sC[np]; t += blockDim.x) {
+      const int pl = plane_of(stR, t);
+      const PlaneGeo &g = ts.g[pl];
+      const BlockRange &rr = ts.br[pl];
+      const DivMod u = divmod(t - ts.sC[pl], ts.w[pl].wC, ts.w[pl].mC);
+      const int x = g.sox + u.r, y = g.soy + u.q;
+      if (x >= rr.bx0 && x < rr.bx1 && y >= rr.by0 && y < rr.by1)
+        rmag |= reduced_sample(im, im.planes[pl], g, red + pl, lds + ts.loff[pl], lds + ts.soff[pl], x, y, status);
+      else
+        lds[ts.soff[pl] + u.q * (int)g.srw + u.r] = 0;
+    }
+    atomicOr(&ts.out_mag, rmag);
+    __syncthreads();
+    queue_next_slot(q, args, ts.next_rec, word);
+    if (im.raw) {
+      // the probe pictures of a sample scale: each plane's tile samples as
+      // they are, i32 rows of the plane's surface plane
+      for (int pl = 0; pl < np; pl++) {
+        const PlaneGeo &g = ts.g[pl];
+        const int w = nx * (int)g.mw, n = w * ny * (int)g.mh;
+        const uint32_t m = div_magic((uint32_t)w);
+        JXR_GLOBAL uint8_t *base = gptr_mut(slot.out) + im.plane_offset[pl];
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+          const DivMod u = divmod(t, (uint32_t)w, m);
+          const int x = tmx * (int)g.mw + u.r, y = tmy * (int)g.mh + u.q;
+          *reinterpret_cast<JXR_GLOBAL int *>(base + (uint64_t)y * im.row_stride[pl] + (uint64_t)x * 4) =
+              lds[ts.soff[pl] + (y - g.soy) * (int)g.srw + (x - g.sox)];
+        }
+      }
+      return;
+    }
+  } else {
+
   // stage C: high-pass prediction and second level, tile blocks + halo
   uint32_t omag = 0;  // magnitude of every sample this thread writes in stages C and D
   const Starts stC = starts_of(ts.sC, np);
@@ -2055,7 +2190,6 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
     const Starts stD = starts_of(ts.sD, np);
     for (int t = threadIdx.x; t < ts.sD[np]; t += blockDim.x) {
       const int pl = plane_of(stD, t);
-      const DevPlane &p = im.planes[pl];
       const PlaneGeo &g = ts.g[pl];
       const DivMod u = divmod(t - ts.sD[pl], ts.w[pl].wD, ts.w[pl].mD);
       const int kx = (tmx << g.bcs) - 1 + u.r, ky = (tmy << g.brs) - 1 + u.q;
@@ -2069,10 +2203,13 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
     atomicOr(&ts.out_mag, omag);
     __syncthreads();
   }
+  }  // full resolution
 
   // stage E: output groups of 4x2 pixels over the tile, clipped to the crop
-  const int x0 = max(tmx * 16, (int)im.crop_x), x1 = min((tmx + nx) * 16, (int)(im.crop_x + im.crop_w));
-  const int y0 = max(tmy * 16, (int)im.crop_y), y1 = min((tmy + ny) * 16, (int)(im.crop_y + im.crop_h));
+  // (luma samples across a macroblock: 16, or 4 and 1 at the reduced scales)
+  const int MB = R ? (int)im.mb_px : 16;
+  const int x0 = max(tmx * MB, (int)im.crop_x), x1 = min((tmx + nx) * MB, (int)(im.crop_x + im.crop_w));
+  const int y0 = max(tmy * MB, (int)im.crop_y), y1 = min((tmy + ny) * MB, (int)(im.crop_y + im.crop_h));
   if (x1 <= x0 || y1 <= y0) return;
   const int gx0 = x0 >> 2, gx1 = ((x1 - 1) >> 2) + 1;
   const int gy0 = y0 >> 1, gy1 = ((y1 - 1) >> 1) + 1;
@@ -2116,7 +2253,7 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
   }
 }
 
-template <uint32_t STORE>
+template <uint32_t STORE, bool R>
 __global__ void __launch_bounds__(kThreads, kMinWavesPerSimd) jxr_reconstruct_tiles(KernelArgs args) {
   HIP_DYNAMIC_SHARED(v4i, lds_quads)  // 16-byte aligned: regions are read and written in quads
   int *lds = reinterpret_cast<int *>(lds_quads);
@@ -2149,7 +2286,7 @@ __global__ void __launch_bounds__(kThreads, kMinWavesPerSimd) jxr_reconstruct_ti
     const uint32_t slot_index = shared_slot;
     if (threadIdx.x == 0)
       q.claim_from = args.persistent && q.tries < q.sh.count ? args.counters + q.r * kCounterStride : nullptr;
-    process_tile<STORE>(shared_rec, item - shared_rec.tile_start, args, lds, ts, slot_index != loaded, q, word);
+    process_tile<STORE, R>(shared_rec, item - shared_rec.tile_start, args, lds, ts, slot_index != loaded, q, word);
     loaded = slot_index;
     __syncthreads();  // every lane has finished with the current item, slot and record
     queue_publish(q, word, args, ts.next_rec, shared_rec, shared_item, shared_slot);
@@ -2184,81 +2321,105 @@ hipError_t static_scratch_address(int device, uint8_t **out) {
 }
 
 // ---- host-side launch wrappers ----------------------------------------------
-hipError_t launch_tiles(const KernelArgs &args, uint32_t grid, hipStream_t stream) {
-  const size_t lds = (size_t)args.lds_words * sizeof(int);
-  switch (args.storage) {
-    case JXRHIP_STORE_U8:
-      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_U8>, dim3(grid), dim3(kThreads), lds, stream, args);
-      break;
-    case JXRHIP_STORE_U16:
-      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_U16>, dim3(grid), dim3(kThreads), lds, stream, args);
-      break;
-    case JXRHIP_STORE_I16:
-      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_I16>, dim3(grid), dim3(kThreads), lds, stream, args);
-      break;
-    case JXRHIP_STORE_I32:
-      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_I32>, dim3(grid), dim3(kThreads), lds, stream, args);
-      break;
-    case JXRHIP_STORE_F16:
-      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_F16>, dim3(grid), dim3(kThreads), lds, stream, args);
-      break;
-    case JXRHIP_STORE_F32:
-      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_F32>, dim3(grid), dim3(kThreads), lds, stream, args);
-      break;
-    case JXRHIP_STORE_BITS:
-      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_BITS>, dim3(grid), dim3(kThreads), lds, stream, args);
-      break;
+// One kernel per storage kind and resolution (full, or the reduced scales).
+template <bool R>
+static const void *tiles_kernel(uint32_t storage) {
+  switch (storage) {
+    case JXRHIP_STORE_U8: return reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_U8, R>);
+    case JXRHIP_STORE_U16: return reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_U16, R>);
+    case JXRHIP_STORE_I16: return reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_I16, R>);
+    case JXRHIP_STORE_I32: return reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_I32, R>);
+    case JXRHIP_STORE_F16: return reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_F16, R>);
+    case JXRHIP_STORE_F32: return reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_F32, R>);
+    case JXRHIP_STORE_BITS: return reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_BITS, R>);
     case JXRHIP_STORE_PACKED16:
-      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_PACKED16>, dim3(grid), dim3(kThreads), lds, stream,
-                         args);
-      break;
+      return reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_PACKED16, R>);
     case JXRHIP_STORE_PACKED32:
-      hipLaunchKernelGGL(jxr_reconstruct_tiles<JXRHIP_STORE_PACKED32>, dim3(grid), dim3(kThreads), lds, stream,
-                         args);
-      break;
-    default:
-      return hipErrorInvalidValue;
+      return reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_PACKED32, R>);
+    default: return nullptr;
   }
+}
+
+template <uint32_t S, bool R>
+static hipError_t launch_one(const KernelArgs &args, uint32_t grid, size_t lds, hipStream_t stream) {
+  hipLaunchKernelGGL((jxr_reconstruct_tiles<S, R>), dim3(grid), dim3(kThreads), lds, stream, args);
   return hipGetLastError();
 }
 
-hipError_t kernel_occupancy(uint32_t storage, size_t dynamic_lds, int *groups_per_cu) {
-  const void *k;
-  switch (storage) {
-    case JXRHIP_STORE_U8: k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_U8>); break;
-    case JXRHIP_STORE_U16: k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_U16>); break;
-    case JXRHIP_STORE_I16: k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_I16>); break;
-    case JXRHIP_STORE_I32: k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_I32>); break;
-    case JXRHIP_STORE_F16: k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_F16>); break;
-    case JXRHIP_STORE_F32: k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_F32>); break;
-    case JXRHIP_STORE_BITS: k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_BITS>); break;
-    case JXRHIP_STORE_PACKED16:
-      k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_PACKED16>);
-      break;
-    case JXRHIP_STORE_PACKED32:
-      k = reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_PACKED32>);
-      break;
+template <bool R>
+static hipError_t launch_store(const KernelArgs &args, uint32_t grid, size_t lds, hipStream_t stream) {
+  switch (args.storage) {
+    case JXRHIP_STORE_U8: return launch_one<JXRHIP_STORE_U8, R>(args, grid, lds, stream);
+    case JXRHIP_STORE_U16: return launch_one<JXRHIP_STORE_U16, R>(args, grid, lds, stream);
+    case JXRHIP_STORE_I16: return launch_one<JXRHIP_STORE_I16, R>(args, grid, lds, stream);
+    case JXRHIP_STORE_I32: return launch_one<JXRHIP_STORE_I32, R>(args, grid, lds, stream);
+    case JXRHIP_STORE_F16: return launch_one<JXRHIP_STORE_F16, R>(args, grid, lds, stream);
+    case JXRHIP_STORE_F32: return launch_one<JXRHIP_STORE_F32, R>(args, grid, lds, stream);
+    case JXRHIP_STORE_BITS: return launch_one<JXRHIP_STORE_BITS, R>(args, grid, lds, stream);
+    case JXRHIP_STORE_PACKED16: return launch_one<JXRHIP_STORE_PACKED16, R>(args, grid, lds, stream);
+    case JXRHIP_STORE_PACKED32: return launch_one<JXRHIP_STORE_PACKED32, R>(args, grid, lds, stream);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_tiles(const KernelArgs &args, uint32_t grid, hipStream_t stream) {
+  const size_t lds = (size_t)args.lds_words * sizeof(int);
+  return args.reduced ? launch_store<true>(args, grid, lds, stream) : launch_store<false>(args, grid, lds, stream);
+}
+
+hipError_t kernel_occupancy(uint32_t storage, uint32_t reduced, size_t dynamic_lds, int *groups_per_cu) {
+  const void *k = reduced ? tiles_kernel<true>(storage) : tiles_kernel<false>(storage);
+  if (!k) return hipErrorInvalidValue;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu, k, (int)kThreads, dynamic_lds);
 }
 
 hipError_t set_kernel_lds_limit(size_t bytes) {
-  const void *kernels[] = {
-      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_U8>),
-      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_U16>),
-      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_I16>),
-      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_I32>),
-      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_F16>),
-      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_F32>),
-      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_BITS>),
-      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_PACKED16>),
-      reinterpret_cast<const void *>(&jxr_reconstruct_tiles<JXRHIP_STORE_PACKED32>)};
-  for (const void *k : kernels) {
-    const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) return e;
-  }
+  const uint32_t kinds[] = {JXRHIP_STORE_U8,  JXRHIP_STORE_U16,  JXRHIP_STORE_I16,      JXRHIP_STORE_I32,
+                            JXRHIP_STORE_F16, JXRHIP_STORE_F32, JXRHIP_STORE_BITS,     JXRHIP_STORE_PACKED16,
+                            JXRHIP_STORE_PACKED32};
+  for (uint32_t kind : kinds)
+    for (const void *k : {tiles_kernel<false>(kind), tiles_kernel<true>(kind)}) {
+      const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+      if (e != hipSuccess) return e;
+    }
   return hipSuccess;
+}
+
+// The sample scale of a reduced decode under overlap filtering
+// (sample_scale.rs:157-246): for each probe sample, the terms of the line
+// through the unscaled reduced pictures of the flat probes, `pictures`
+// holding `levels` pairs (the picture of +level, then of -level) of
+// `length` samples each, and `span` twice the sum of the levels. A term
+// whose divisor is not positive fails the status word (its divisor is kept
+// at one).
+__global__ void __launch_bounds__(kThreads) jxr_scale_terms(const int32_t *pictures, uint32_t length,
+                                                            uint32_t levels, int64_t span, int64_t *terms,
+                                                            uint32_t *status) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < length; i += gridDim.x * blockDim.x) {
+    long long rise = 0, offset = 0;
+    for (uint32_t k = 0; k < levels; k++) {
+      const long long up = pictures[(uint64_t)(2 * k) * length + i];
+      const long long down = pictures[(uint64_t)(2 * k + 1) * length + i];
+      rise += up - down;
+      offset += up + down;
+    }
+    const long long count = 2 * (long long)levels;
+    long long divisor = count * rise;
+    if (divisor <= 0) {
+      fail(status, JXRHIP_STAGE_SAMPLE_SCALE);
+      divisor = 1;
+    }
+    terms[2 * (uint64_t)i] = divisor;
+    terms[2 * (uint64_t)i + 1] = offset * span - divisor / 2;
+  }
+}
+
+hipError_t launch_scale_terms(const int32_t *pictures, uint32_t length, uint32_t levels, int64_t span,
+                              int64_t *terms, uint32_t *status, hipStream_t stream) {
+  if (!length) return hipSuccess;
+  hipLaunchKernelGGL(jxr_scale_terms, dim3((length + kThreads - 1) / kThreads), dim3(kThreads), 0, stream,
+                     pictures, length, levels, span, terms, status);
+  return hipGetLastError();
 }
 
 // item_slot[i] = index of the image that owns work item i: workgroup b

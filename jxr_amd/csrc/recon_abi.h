@@ -38,6 +38,27 @@ struct DevPlane {
   uint32_t pad[3];
 };
 
+// How a reduced decode turns the coefficients it stops at into samples
+// (crates/jxr-native/src/reconstruct/sample_scale.rs:57-70, 255-316), one per
+// plane: a shift by the gain of the transform stages left out, or, under
+// overlap filtering, the line through the reduced pictures of flat probe
+// images, looked up for each sample at the probe sample placed like it.
+enum ReduceMode : uint32_t {
+  kReduceShift = 0,  // sample = coefficient >> shift
+  kReduceFlat = 1,   // sample = floor((c * gain - bias) / divisor), 0 for 0
+  kReduceNone = 2,   // probe pictures: the coefficient itself
+};
+struct DevReduce {
+  const int64_t *terms;   // flat: {divisor, bias} of each probe sample
+  const uint32_t *cols;   // flat: probe sample column of each reduced column
+  const uint32_t *rows;   // flat: probe sample row of each reduced row
+  int64_t gain;           // flat: probe pictures times the level span
+  uint32_t mode;          // ReduceMode
+  uint32_t shift;         // shift: gain bits of the stages left out
+  uint32_t stride;        // flat: probe samples across
+  uint32_t pad[3];        // 64 bytes
+};
+
 // Bias, rounding, shift, postscale of one channel (scaling.rs:42-124).
 struct ChanScale {
   int32_t shifted_bias;
@@ -113,6 +134,15 @@ struct alignas(64) DevImage {
   uint32_t pixel_channels;
   uint32_t channel_base;
   uint32_t store_channels;
+  // reduced decode (JXRHIP_SCALE_*): luma samples across a macroblock
+  // (16, 4 or 1), whether the tile stores raw reduced pictures (the probe
+  // images of the sample scale) instead of output pixels, and each plane's
+  // sample scale
+  uint32_t scale;
+  uint32_t mb_px;
+  uint32_t raw;
+  uint32_t pad_reduce;
+  const DevReduce *reduce;
   uint64_t plane_offset[kMaxPlanes];       // output surface planes
   uint64_t row_stride[kMaxPlanes];
   ChanScale color_scale[8];
@@ -139,6 +169,7 @@ struct KernelArgs {
   uint32_t tile_x, tile_y;  // macroblocks per tile
   uint32_t lds_words;    // dynamic LDS in 32-bit words
   uint32_t storage;      // JXRHIP_STORE_* of every image of the launch
+  uint32_t reduced;      // every image of the launch is a reduced decode
 };
 
 // Scratch of the allocation-free batch entry point, a static device array of
@@ -177,6 +208,19 @@ __host__ __device__ inline uint32_t low_region_words(uint32_t bc, uint32_t br, u
                                                      uint32_t ty) {
   uint32_t h = low_halo(bc, overlap);
   return (tx + 2 * h) * bc * (ty + 2 * h) * br;
+}
+// A reduced decode keeps a sample per block (a quarter) or per macroblock (a
+// sixteenth): samples across one macroblock of a plane of `blocks` blocks.
+__host__ __device__ inline uint32_t reduced_units(uint32_t blocks, uint32_t scale) {
+  return scale == JXRHIP_SCALE_QUARTER ? blocks : 1u;
+}
+// Reduced sample region of a tile: rows start on a 16-byte boundary four
+// samples left of the tile's first, and hold the tile, a one-sample halo and
+// the overhang of the output stage's four-sample groups; two rows of halo
+// above and below.
+__host__ __device__ inline uint32_t reduced_region_width(uint32_t ux, uint32_t tx) { return (tx * ux + 15u) & ~3u; }
+__host__ __device__ inline uint32_t reduced_sample_words(uint32_t ux, uint32_t uy, uint32_t tx, uint32_t ty) {
+  return reduced_region_width(ux, tx) * (ty * uy + 2 * kSampleHaloRows);
 }
 // Per plane, the second level's view of the macroblock records of the tile
 // and its one-macroblock halo (coefficient offset, HP step, bands and HP

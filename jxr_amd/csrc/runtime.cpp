@@ -40,7 +40,12 @@ hipError_t launch_tiles(const KernelArgs &args, uint32_t grid, hipStream_t strea
 hipError_t launch_clear_rows(uint8_t *base, uint64_t pitch, uint32_t row_bytes, uint32_t rows, hipStream_t stream);
 hipError_t launch_expand(const Slot *slots, uint32_t slot_count, uint32_t item_count, uint32_t *item_slot,
                          uint32_t *counters, hipStream_t stream);
-hipError_t kernel_occupancy(uint32_t storage, size_t dynamic_lds, int *groups_per_cu);
+hipError_t kernel_occupancy(uint32_t storage, uint32_t reduced, size_t dynamic_lds, int *groups_per_cu);
+hipError_t launch_scale_terms(const int32_t *pictures, uint32_t length, uint32_t levels, int64_t span,
+                              int64_t *terms, uint32_t *status, hipStream_t stream);
+void flat_probe_coefficients(int32_t level, uint32_t bc, uint32_t br, bool doubled, uint32_t overlap,
+                             const std::vector<uint32_t> &cols, const std::vector<uint32_t> &rows,
+                             std::vector<int32_t> &out);
 hipError_t set_kernel_lds_limit(size_t bytes);
 hipError_t launch_valu_chains(uint32_t *out, uint32_t groups, uint32_t iterations, hipStream_t stream);
 uint32_t valu_per_iteration();
@@ -106,6 +111,32 @@ struct PlanPlane {
   uint32_t overlap = 0;
   std::vector<MbRec> recs;                 // raster order
   std::vector<uint32_t> colmask, rowmask;  // empty for soft tiles
+  std::vector<uint32_t> regions_x, regions_y;  // region boundaries (macroblocks), 0 .. extent
+  // reduced decode: samples across / down a macroblock, the sample scale
+  // (ReduceMode, shift) and, under overlap, the probe set whose terms it
+  // reads and, for each reduced column and row, the probe sample placed
+  // like it (sample_scale.rs:405-417)
+  uint32_t ux = 0, uy = 0;
+  uint32_t reduce_mode = kReduceShift, reduce_shift = 0;
+  int32_t probe = -1;
+  std::vector<uint32_t> probe_cols, probe_rows;
+};
+
+// The flat probe images of one plane configuration's sample scale
+// (sample_scale.rs:157-246): the levels +-0x10001 .. +-0x10004 analysed at
+// unit steps over a probe of one region for each region extent of the plane
+// (five or more folding onto five), hard tiles at the probe's regions.
+constexpr uint32_t kProbeLevels = 4;
+constexpr uint32_t kProbeExtent = 5;
+struct ProbeSet {
+  uint32_t arena, bc, br, overlap;
+  bool scale_first, alpha;
+  uint32_t mbx, mby;                     // probe macroblocks
+  std::vector<uint32_t> colmask, rowmask;
+  std::vector<MbRec> recs;               // shared by every probe image
+  std::vector<int32_t> coef[2 * kProbeLevels];  // +L1, -L1, +L2, ...: DC and low-pass
+  uint32_t stride, length;               // probe picture: samples across, samples
+  int64_t span, gain;
 };
 
 static std::atomic<uint64_t> g_plan_ids{1};
@@ -128,6 +159,8 @@ struct jxrhip_plan {
   uint64_t coefficient_bytes = 0;
   uint64_t geometry_key = 0;  // bucket key: plane geometries
   jxrhip_surface_layout surface{};  // the validated output layout
+  uint32_t scale = JXRHIP_SCALE_FULL;
+  std::vector<ProbeSet> probes;     // reduced decode under overlap
 };
 
 static int32_t bias_of(uint32_t depth, uint32_t shift_bits, bool *bad) {
@@ -275,8 +308,120 @@ static jxrhip_status build_plane(jxrhip_plan *plan, const jxrhip_arena *a, uint3
         !tile_mask(a->tile_row_heights, a->tile_row_count, y0, mby, pp.rowmask))
       return fail(JXRHIP_ERR_INVALID_PLAN, "tile partition does not cover reconstruction window");
   }
+  // regions no overlap operator crosses: the window, split at hard tile
+  // edges (device_plan/overlap.rs:102-123)
+  pp.regions_x.push_back(0);
+  for (uint32_t m = 1; m < mbx && !pp.colmask.empty(); m++)
+    if ((pp.colmask[m >> 5] >> (m & 31)) & 1u) pp.regions_x.push_back(m);
+  pp.regions_x.push_back(mbx);
+  pp.regions_y.push_back(0);
+  for (uint32_t m = 1; m < mby && !pp.rowmask.empty(); m++)
+    if ((pp.rowmask[m >> 5] >> (m & 31)) & 1u) pp.regions_y.push_back(m);
+  pp.regions_y.push_back(mby);
   if (a->overlap > 2) return fail(JXRHIP_ERR_INVALID_PLAN, "overlap mode");
   plan->planes.push_back(std::move(pp));
+  return JXRHIP_OK;
+}
+
+// ProbeAxis (sample_scale.rs:336-381): the probe's region boundaries along
+// one axis, and the probe macroblock placed in its region as each
+// macroblock of the plane is placed in its own (probe_offset,
+// sample_scale.rs:405-417).
+static void probe_axis(const std::vector<uint32_t> &regions, std::vector<uint32_t> &bounds,
+                       std::vector<uint32_t> &pos) {
+  bounds.assign(1, 0);
+  pos.clear();
+  int64_t starts[kProbeExtent + 1];
+  std::fill(starts, starts + kProbeExtent + 1, int64_t(-1));
+  uint32_t mbs = 0;
+  const uint32_t ends = kProbeExtent / 2;
+  for (size_t i = 0; i + 1 < regions.size(); i++) {
+    const uint32_t extent = regions[i + 1] - regions[i];  // positive: regions advance
+    const uint32_t pe = std::min(extent, kProbeExtent);
+    if (starts[pe] < 0) {
+      starts[pe] = mbs;
+      mbs += pe;
+      bounds.push_back(mbs);
+    }
+    for (uint32_t off = 0; off < extent; off++) {
+      const uint32_t o = (extent <= kProbeExtent || off < ends) ? off
+                         : extent - off <= ends                 ? kProbeExtent - (extent - off)
+                                                                : ends;
+      pos.push_back((uint32_t)starts[pe] + o);
+    }
+  }
+}
+
+static void probe_mask(const std::vector<uint32_t> &bounds, std::vector<uint32_t> &mask) {
+  mask.assign(bounds.back() / 32 + 2, 0u);
+  for (size_t i = 1; i + 1 < bounds.size(); i++) mask[bounds[i] >> 5] |= 1u << (bounds[i] & 31);
+}
+
+// Sample scale of every plane of a reduced decode (pipeline.rs:84-112,
+// sample_scale.rs:157-246): without overlap the gain of the transform
+// stages left out (two bits for the block transform, and at a sixteenth two
+// more for a full macroblock's first level, one for subsampled chroma); with
+// it, the flat probe images of the plane's configuration, whose reduced
+// pictures the device turns into the scale's terms.
+static jxrhip_status build_reduction(jxrhip_plan *plan) {
+  const uint32_t scale = plan->scale;
+  for (PlanPlane &pp : plan->planes) {
+    pp.ux = reduced_units(pp.bc, scale);
+    pp.uy = reduced_units(pp.br, scale);
+    const uint32_t mb_gain = pp.bc == 4 && pp.br == 4 ? 2 : 1;
+    pp.reduce_shift = scale == JXRHIP_SCALE_QUARTER ? 2 : mb_gain + 2;
+    if (pp.overlap == 0) {
+      pp.reduce_mode = kReduceShift;
+      continue;
+    }
+    pp.reduce_mode = kReduceFlat;
+    std::vector<uint32_t> bx, px, by, py;
+    probe_axis(pp.regions_x, bx, px);
+    probe_axis(pp.regions_y, by, py);
+    // planes of one arena share its regions: one probe set serves every
+    // plane of the same configuration
+    int32_t found = -1;
+    for (size_t k = 0; k < plan->probes.size(); k++) {
+      const ProbeSet &q = plan->probes[k];
+      if (q.arena == pp.arena && q.bc == pp.bc && q.br == pp.br && q.overlap == pp.overlap &&
+          q.scale_first == pp.scale_first)
+        found = (int32_t)k;
+    }
+    if (found < 0) {
+      ProbeSet q{};
+      q.arena = pp.arena;
+      q.bc = pp.bc;
+      q.br = pp.br;
+      q.overlap = pp.overlap;
+      q.scale_first = pp.scale_first;
+      q.alpha = pp.alpha;
+      q.mbx = bx.back();
+      q.mby = by.back();
+      probe_mask(bx, q.colmask);
+      probe_mask(by, q.rowmask);
+      const uint32_t blocks = pp.bc * pp.br;
+      q.recs.resize((size_t)q.mbx * q.mby);
+      for (uint32_t m = 0; m < q.mbx * q.mby; m++)
+        q.recs[m] = MbRec{m * blocks, 1, 1, 1, JXRHIP_BANDS_NO_HIGH_PASS, JXRHIP_PRED_NONE, 0, 0};
+      q.span = 0;
+      for (uint32_t k = 0; k < kProbeLevels; k++) {
+        const int32_t level = 0x10001 + (int32_t)k;
+        flat_probe_coefficients(level, pp.bc, pp.br, pp.scale_first, pp.overlap, bx, by, q.coef[2 * k]);
+        flat_probe_coefficients(-level, pp.bc, pp.br, pp.scale_first, pp.overlap, bx, by, q.coef[2 * k + 1]);
+        q.span += 2 * (int64_t)level;
+      }
+      q.gain = 2 * (int64_t)kProbeLevels * q.span;
+      q.stride = q.mbx * pp.ux;
+      q.length = q.stride * q.mby * pp.uy;
+      plan->probes.push_back(std::move(q));
+      found = (int32_t)plan->probes.size() - 1;
+    }
+    pp.probe = found;
+    pp.probe_cols.resize((size_t)plan->mbx * pp.ux);
+    for (uint32_t x = 0; x < plan->mbx * pp.ux; x++) pp.probe_cols[x] = px[x / pp.ux] * pp.ux + x % pp.ux;
+    pp.probe_rows.resize((size_t)plan->mby * pp.uy);
+    for (uint32_t y = 0; y < plan->mby * pp.uy; y++) pp.probe_rows[y] = py[y / pp.uy] * pp.uy + y % pp.uy;
+  }
   return JXRHIP_OK;
 }
 
@@ -344,6 +489,14 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
                                                            : (p.bc == 4 && p.br == 4);
     if (!ok) return fail(JXRHIP_ERR_INVALID_PLAN, "component macroblock geometry does not match the colour format");
   }
+  // ---- decode scale ----
+  if (d->scale > JXRHIP_SCALE_SIXTEENTH) return fail(JXRHIP_ERR_INVALID_PLAN, "decode scale");
+  plan->scale = d->scale;
+  if (plan->scale != JXRHIP_SCALE_FULL) {
+    st = build_reduction(plan.get());
+    if (st) return st;
+  }
+  const uint32_t mb_px = plan->scale == JXRHIP_SCALE_QUARTER ? 4u : plan->scale == JXRHIP_SCALE_SIXTEENTH ? 1u : 16u;
 
   // ---- output policy (device_plan/output.rs:102-295) ----
   const jxrhip_output_format &f = d->output;
@@ -354,7 +507,15 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
   dv.alpha_plane = alpha_plane;
   dv.mbx = plan->mbx;
   dv.mby = plan->mby;
-  dv.upsample = cf == JXRHIP_COLOR_YUV420 ? 1 : cf == JXRHIP_COLOR_YUV422 ? 2 : 0;
+  // a sixteenth has one chroma sample per macroblock, as luma does: nothing
+  // to interpolate (metadata.rs:253-263, decode/output.rs:464-482)
+  dv.upsample = plan->scale == JXRHIP_SCALE_SIXTEENTH ? 0
+                : cf == JXRHIP_COLOR_YUV420           ? 1
+                : cf == JXRHIP_COLOR_YUV422           ? 2
+                                                      : 0;
+  dv.scale = plan->scale;
+  dv.mb_px = mb_px;
+  dv.raw = 0;
   const uint32_t l = f.channel_layout;
   uint32_t primary = 0;
   switch (l) {
@@ -410,6 +571,8 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
       return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "planar YUV output keeps the coded 4:2:0 or 4:2:2 sampling");
     if (f.premultiply_alpha) return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "premultiplied YUV output");
     if (!d->surface.planar) return fail(JXRHIP_ERR_INVALID_PLAN, "YUV output is planar");
+    if (plan->scale == JXRHIP_SCALE_SIXTEENTH)  // prepare.rs:185-192
+      return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "sixteenth-resolution subsampled planar YUV output");
     if (f.storage != JXRHIP_STORE_U8 && f.storage != JXRHIP_STORE_U16 && f.storage != JXRHIP_STORE_I16)
       return fail(JXRHIP_ERR_UNSUPPORTED_OUTPUT, "T.832 output colour and bit depth matrix");
     dv.out_kind = kOutYuv;
@@ -514,13 +677,14 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
   dv.f_alpha_mant = f.has_alpha_format ? f.alpha_mantissa_length : f.mantissa_length;
   dv.f_alpha_bias = f.has_alpha_format ? f.alpha_exponent_bias : f.exponent_bias;
   {
-    // crop in the coded plane -> relative to the window's sample origin
-    const uint64_t wx = (uint64_t)ox * 16, wy = (uint64_t)oy * 16;
+    // crop in the coded plane (at the decode scale) -> relative to the
+    // window's sample origin
+    const uint64_t wx = (uint64_t)ox * mb_px, wy = (uint64_t)oy * mb_px;
     const jxrhip_surface_layout &s = d->surface;
     if (f.crop_width == 0 || f.crop_height == 0 || s.width != f.crop_width || s.height != f.crop_height)
       return fail(JXRHIP_ERR_INVALID_PLAN, "output layout and prepared request differ");
-    if (f.crop_x < wx || f.crop_y < wy || (uint64_t)f.crop_x + f.crop_width > wx + (uint64_t)plan->mbx * 16 ||
-        (uint64_t)f.crop_y + f.crop_height > wy + (uint64_t)plan->mby * 16)
+    if (f.crop_x < wx || f.crop_y < wy || (uint64_t)f.crop_x + f.crop_width > wx + (uint64_t)plan->mbx * mb_px ||
+        (uint64_t)f.crop_y + f.crop_height > wy + (uint64_t)plan->mby * mb_px)
       return fail(JXRHIP_ERR_INVALID_PLAN, "output crop is outside a sample plane");
     dv.crop_x = (uint32_t)(f.crop_x - wx);
     dv.crop_y = (uint32_t)(f.crop_y - wy);
@@ -564,18 +728,30 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
   // ---- work and footprint figures ----
   plan->pixels = (uint64_t)f.crop_width * f.crop_height;
   for (const PlanPlane &p : plan->planes) {
-    plan->recon_coefs += (uint64_t)p.recs.size() * p.bc * p.br * 16;
-    plan->descriptor_bytes += p.recs.size() * sizeof(MbRec) + (p.colmask.size() + p.rowmask.size()) * 4;
+    // coefficients a decode at the plan's scale reconstructs: all of them,
+    // the DC and low-pass of each block, or each macroblock's DC
+    const uint64_t per_mb = plan->scale == JXRHIP_SCALE_QUARTER     ? p.bc * p.br
+                            : plan->scale == JXRHIP_SCALE_SIXTEENTH ? 1
+                                                                    : (uint64_t)p.bc * p.br * 16;
+    plan->recon_coefs += (uint64_t)p.recs.size() * per_mb;
+    plan->descriptor_bytes += p.recs.size() * sizeof(MbRec) + (p.colmask.size() + p.rowmask.size()) * 4 +
+                              (p.probe_cols.size() + p.probe_rows.size()) * 4;
   }
+  for (const ProbeSet &q : plan->probes) plan->descriptor_bytes += (uint64_t)q.length * 16;
   for (uint32_t a = 0; a < plan->arena_count; a++) plan->coefficient_bytes += plan->arenas[a]->coefficient_count * 4;
   {
     // algorithmic bytes: 4 B per stored coefficient read, 32 B per
     // macroblock record per plane, output bytes written
+    // (a reduced decode reads the stored DC and low-pass values of each
+    // macroblock, or its DC alone)
     uint64_t stored = 0, mbs = 0;
     for (const PlanPlane &p : plan->planes) {
       mbs += p.recs.size();
-      for (const MbRec &r : p.recs)
-        stored += r.bands == 0 ? 1 : r.bands == 1 ? p.bc * p.br : (uint64_t)p.bc * p.br * 16;
+      for (const MbRec &r : p.recs) {
+        const uint64_t coded = r.bands == 0 ? 1 : r.bands == 1 ? p.bc * p.br : (uint64_t)p.bc * p.br * 16;
+        const uint64_t low = r.bands == 0 ? 1 : p.bc * p.br;
+        stored += plan->scale == JXRHIP_SCALE_QUARTER ? low : plan->scale == JXRHIP_SCALE_SIXTEENTH ? 1 : coded;
+      }
     }
     const uint64_t out_px_bytes = plan->content_bytes;
     plan->alg_bytes = stored * 4 + mbs * sizeof(MbRec) + out_px_bytes;
@@ -635,6 +811,9 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
   uint64_t key = plan->group_planes[0].size();
   for (uint32_t i : plan->group_planes[0]) key = key * 16 + plan->planes[i].bc * 4 + plan->planes[i].br;
   plan->geometry_key = key * 16 + dv.storage;
+  // reduced decodes run the reduced kernels: buckets of their own (the
+  // plane key stays far below bit 62)
+  plan->geometry_key |= (uint64_t)plan->scale << 62;
   *out = plan.release();
   return JXRHIP_OK;
 }
@@ -1008,6 +1187,127 @@ static jxrhip_status device_coefficients(jxrhip_session *s, const jxrhip_arena *
   return JXRHIP_OK;
 }
 
+// The terms of each probe set's sample scale (sample_scale.rs:157-246),
+// computed on the device into `terms`: the reduced kernel decodes the set's
+// flat probe images (+-level, four levels) into raw reduced pictures, and
+// jxr_scale_terms fits the line through them for every probe sample. Runs
+// once per descriptor upload and waits for the device; an overflow in a
+// probe, or a term whose divisor is not positive, fails the upload.
+static jxrhip_status run_probes(jxrhip_session *s, const jxrhip_plan *p, const std::vector<int64_t *> &terms,
+                                hipStream_t stream) {
+  // layout: status word | per set: records, masks, sample scale, pictures,
+  // then each image's coefficients and descriptor
+  constexpr uint32_t kImages = 2 * kProbeLevels;
+  size_t bytes = 256;
+  for (const ProbeSet &q : p->probes) {
+    bytes += q.recs.size() * sizeof(MbRec) + (q.colmask.size() + q.rowmask.size()) * 4 + sizeof(DevReduce) +
+             (size_t)kImages * q.length * 4 + 5 * 64;
+    for (const auto &c : q.coef) bytes += c.size() * 4 + sizeof(DevImage) + 2 * 64;
+  }
+  std::vector<uint8_t> host(bytes, 0);
+  DeviceBuffer b;
+  HIP_TRY(s->pool.get(bytes, b, s->diag.scratch_hits, s->diag.scratch_misses), "probe buffer");
+  uint8_t *dev = (uint8_t *)b.ptr;
+  size_t off = 256;  // the status word is at 0
+  auto place = [&](const void *src, size_t n) -> uint8_t * {
+    off = (off + 63) & ~size_t(63);
+    if (src && n) std::memcpy(host.data() + off, src, n);
+    uint8_t *d = dev + off;
+    off += n;
+    return d;
+  };
+  struct ProbeLaunch {
+    const DevImage *img;
+    int32_t *pictures;
+    const ProbeSet *set;
+  };
+  std::vector<ProbeLaunch> launches;
+  std::vector<int32_t *> pictures;
+  for (const ProbeSet &q : p->probes) {
+    const MbRec *recs = (const MbRec *)place(q.recs.data(), q.recs.size() * sizeof(MbRec));
+    const uint32_t *cm = (const uint32_t *)place(q.colmask.data(), q.colmask.size() * 4);
+    const uint32_t *rm = (const uint32_t *)place(q.rowmask.data(), q.rowmask.size() * 4);
+    DevReduce r;
+    std::memset(&r, 0, sizeof r);
+    r.mode = kReduceNone;
+    const DevReduce *rd = (const DevReduce *)place(&r, sizeof r);
+    int32_t *pic = (int32_t *)place(nullptr, (size_t)kImages * q.length * 4);
+    pictures.push_back(pic);
+    for (uint32_t j = 0; j < kImages; j++) {
+      const int32_t *coef = (const int32_t *)place(q.coef[j].data(), q.coef[j].size() * 4);
+      DevImage im;
+      std::memset(&im, 0, sizeof im);
+      DevPlane &pl = im.planes[0];
+      pl.coef = coef;
+      pl.mbs = recs;
+      pl.colmask = cm;
+      pl.rowmask = rm;
+      pl.bc = q.bc;
+      pl.br = q.br;
+      pl.scale_first = q.scale_first ? 1 : 0;
+      pl.overlap = q.overlap;
+      pl.alpha = q.alpha ? 1 : 0;
+      im.plane_count = 1;
+      im.colors = 1;
+      im.alpha_plane = -1;
+      im.mbx = q.mbx;
+      im.mby = q.mby;
+      im.scale = p->scale;
+      im.mb_px = p->groups[0].mb_px;
+      im.raw = 1;
+      im.reduce = rd;
+      im.storage = JXRHIP_STORE_I32;
+      im.bytes = 4;
+      im.plane_offset[0] = (uint64_t)j * q.length * 4;
+      im.row_stride[0] = (uint64_t)q.stride * 4;
+      launches.push_back({(const DevImage *)place(&im, sizeof im), pic, &q});
+    }
+  }
+  auto undo = [&](jxrhip_status st) {
+    (void)hipStreamSynchronize(stream);
+    s->pool.put(b);
+    return st;
+  };
+  hipError_t e = hipMemcpyAsync(dev, host.data(), off, hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) return undo(hip_fail(e, "probe upload"));
+  for (const ProbeLaunch &L : launches) {
+    const ProbeSet &q = *L.set;
+    const uint32_t ux = reduced_units(q.bc, p->scale), uy = reduced_units(q.br, p->scale);
+    const uint32_t tx = 4, ty = 4;
+    KernelArgs args{};
+    args.inline_slot.img = L.img;
+    args.inline_slot.out = (uint8_t *)L.pictures;
+    args.inline_slot.status = (uint32_t *)dev;
+    args.inline_slot.tiles_x = (q.mbx + tx - 1) / tx;
+    args.persistent = 0;
+    args.tile_x = tx;
+    args.tile_y = ty;
+    args.lds_words = reduced_sample_words(ux, uy, tx, ty) + low_region_words(q.bc, q.br, q.overlap, tx, ty) +
+                     mb_table_words(tx, ty);
+    args.storage = JXRHIP_STORE_I32;
+    args.reduced = 1;
+    const uint32_t tiles = args.inline_slot.tiles_x * ((q.mby + ty - 1) / ty);
+    args.item_count = tiles;
+    if ((e = launch_tiles(args, (tiles + 7u) & ~7u, stream)) != hipSuccess) return undo(hip_fail(e, "probe launch"));
+    s->diag.kernel_launches++;
+  }
+  for (size_t k = 0; k < p->probes.size(); k++) {
+    const ProbeSet &q = p->probes[k];
+    if ((e = launch_scale_terms(pictures[k], q.length, kProbeLevels, q.span, terms[k], (uint32_t *)dev, stream)) !=
+        hipSuccess)
+      return undo(hip_fail(e, "sample scale terms"));
+    s->diag.kernel_launches++;
+  }
+  uint32_t status = 0;
+  e = hipMemcpyAsync(&status, dev, 4, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) return undo(hip_fail(e, "probe status"));
+  s->pool.put(b);
+  if (status) return fail(JXRHIP_ERR_KERNEL_ARITHMETIC, "reduced-resolution sample scale");
+  return JXRHIP_OK;
+}
+
 // Device DevImage for a plan (descriptors cached per plan id). The
 // coefficient pointers inside depend on the cached uploads, so the
 // descriptor is rebuilt whenever an upload changes.
@@ -1028,11 +1328,18 @@ static jxrhip_status device_image(jxrhip_session *s, const jxrhip_plan *p, hipSt
       return JXRHIP_OK;
     }
   }
-  // layout: DevImage of each plane group | recs of each plane | masks
+  // layout: DevImage of each plane group | recs of each plane | masks, and
+  // for a reduced decode: the sample scale of each group plane | each
+  // plane's probe maps | the terms of each probe set (written on the device)
   const size_t ngroups = p->groups.size();
   size_t bytes = sizeof(DevImage) * ngroups;
   for (const PlanPlane &pp : p->planes)
-    bytes += pp.recs.size() * sizeof(MbRec) + (pp.colmask.size() + pp.rowmask.size()) * 4 + 64;
+    bytes += pp.recs.size() * sizeof(MbRec) + (pp.colmask.size() + pp.rowmask.size()) * 4 + 64 +
+             (pp.probe_cols.size() + pp.probe_rows.size()) * 4;
+  if (p->scale != JXRHIP_SCALE_FULL) {
+    for (const auto &gp : p->group_planes) bytes += gp.size() * sizeof(DevReduce) + 64;
+    for (const ProbeSet &q : p->probes) bytes += (size_t)q.length * 16 + 64;
+  }
   std::vector<uint8_t> host(bytes, 0);
   DeviceBuffer b;
   if (cacheable) evict_until(s, bytes);
@@ -1063,14 +1370,58 @@ static jxrhip_status device_image(jxrhip_session *s, const jxrhip_plan *p, hipSt
     d.overlap = pp.overlap;
     d.alpha = pp.alpha ? 1 : 0;
   }
+  // reduced decode: probe terms, then each plane's sample scale
+  std::vector<DevReduce> dr(p->planes.size());
+  std::vector<int64_t *> terms(p->probes.size(), nullptr);
+  if (p->scale != JXRHIP_SCALE_FULL) {
+    for (size_t k = 0; k < p->probes.size(); k++) {
+      off = (off + 63) & ~size_t(63);
+      terms[k] = (int64_t *)((uint8_t *)b.ptr + off);
+      off += (size_t)p->probes[k].length * 16;
+    }
+    for (size_t i = 0; i < p->planes.size(); i++) {
+      const PlanPlane &pp = p->planes[i];
+      DevReduce &r = dr[i];
+      std::memset(&r, 0, sizeof r);
+      r.mode = pp.reduce_mode;
+      r.shift = pp.reduce_shift;
+      if (pp.reduce_mode == kReduceFlat) {
+        const ProbeSet &q = p->probes[pp.probe];
+        r.terms = terms[pp.probe];
+        r.gain = q.gain;
+        r.stride = q.stride;
+        std::memcpy(host.data() + off, pp.probe_cols.data(), pp.probe_cols.size() * 4);
+        r.cols = (const uint32_t *)((uint8_t *)b.ptr + off);
+        off += pp.probe_cols.size() * 4;
+        std::memcpy(host.data() + off, pp.probe_rows.data(), pp.probe_rows.size() * 4);
+        r.rows = (const uint32_t *)((uint8_t *)b.ptr + off);
+        off += pp.probe_rows.size() * 4;
+      }
+    }
+  }
   for (size_t g = 0; g < ngroups; g++) {
     DevImage dv = p->groups[g];
     for (size_t k = 0; k < p->group_planes[g].size(); k++) dv.planes[k] = dp[p->group_planes[g][k]];
+    if (p->scale != JXRHIP_SCALE_FULL) {
+      off = (off + 63) & ~size_t(63);
+      dv.reduce = (const DevReduce *)((uint8_t *)b.ptr + off);
+      for (size_t k = 0; k < p->group_planes[g].size(); k++) {
+        std::memcpy(host.data() + off, &dr[p->group_planes[g][k]], sizeof(DevReduce));
+        off += sizeof(DevReduce);
+      }
+    }
     std::memcpy(host.data() + g * sizeof(DevImage), &dv, sizeof dv);
   }
   HIP_TRY(hipMemcpyAsync(b.ptr, host.data(), bytes, hipMemcpyHostToDevice, stream), "descriptor upload");
   // pageable copies are staged before the call returns, so `host` may go
   HIP_TRY(hipStreamSynchronize(stream), "descriptor upload");
+  if (!p->probes.empty()) {
+    const jxrhip_status pst = run_probes(s, p, terms, stream);
+    if (pst) {
+      s->pool.put(b);
+      return pst;
+    }
+  }
   if (cacheable) {
     // the eviction above may have removed the stale entry already
     it = s->desc_cache.find(p->id);
@@ -1130,8 +1481,9 @@ static uint32_t tile_words(const jxrhip_plan *p, uint32_t tx, uint32_t ty) {
     uint32_t w = 0;
     for (uint32_t i : gp) {
       const PlanPlane &pp = p->planes[i];
-      w += sample_region_words(pp.bc, pp.br, tx, ty) + low_region_words(pp.bc, pp.br, pp.overlap, tx, ty) +
-           mb_table_words(tx, ty);
+      const uint32_t samples = p->scale != JXRHIP_SCALE_FULL ? reduced_sample_words(pp.ux, pp.uy, tx, ty)
+                                                             : sample_region_words(pp.bc, pp.br, tx, ty);
+      w += samples + low_region_words(pp.bc, pp.br, pp.overlap, tx, ty) + mb_table_words(tx, ty);
     }
     most = std::max(most, w);
   }
@@ -1245,6 +1597,7 @@ static jxrhip_status submit_impl(jxrhip_session *s, const jxrhip_plan *p, void *
   args.tile_y = ty;
   args.lds_words = words;
   args.storage = p->dev.storage;
+  args.reduced = p->scale != JXRHIP_SCALE_FULL ? 1 : 0;
   const uint32_t tiles = args.inline_slot.tiles_x * ((p->mby + ty - 1) / ty);
   args.item_count = tiles;
   if ((e = hipMemsetAsync(status_word, 0, 4, sub->stream)) != hipSuccess) return undo(hip_fail(e, "status reset"));
@@ -1526,6 +1879,7 @@ extern "C" jxrhip_status jxrhip_decode_to_host(jxrhip_session *s, const jxrhip_p
 struct BucketLaunch {
   size_t first;  // first slot of the bucket
   uint32_t n, items, tx, ty, words, storage, grid;
+  uint32_t reduced = 0;  // the bucket's images are reduced decodes
 };
 
 struct PreparedBatch {
@@ -1607,6 +1961,7 @@ static jxrhip_status prepare_persistent(jxrhip_session *s, const jxrhip_plan *co
     for (uint32_t i : kv.second) words = std::max(words, tile_words(plans[i], tx, ty));
     if ((size_t)words * 4 > s->max_lds) return fail(JXRHIP_ERR_INVALID_PLAN, "tile does not fit in LDS");
     BucketLaunch L{slots.size(), 0, 0, tx, ty, words, plans[kv.second[0]]->dev.storage, 0};
+    L.reduced = plans[kv.second[0]]->scale != JXRHIP_SCALE_FULL ? 1 : 0;
     for (uint32_t i : kv.second) {
       // one slot per plane group; the groups of an image share its status word
       for (size_t g = 0; g < plans[i]->groups.size(); g++) {
@@ -1626,7 +1981,8 @@ static jxrhip_status prepare_persistent(jxrhip_session *s, const jxrhip_plan *co
     // workgroups a compute unit holds at once (LDS, registers), unless the
     // session fixes it; a multiple of 8 workgroups for the XCD split
     int resident = 0;
-    if (!s->groups_per_cu) HIP_TRY(kernel_occupancy(L.storage, (size_t)L.words * 4, &resident), "occupancy");
+    if (!s->groups_per_cu)
+      HIP_TRY(kernel_occupancy(L.storage, L.reduced, (size_t)L.words * 4, &resident), "occupancy");
     const uint32_t per_cu = s->groups_per_cu ? s->groups_per_cu : (uint32_t)std::max(resident, 1);
     uint32_t grid = std::min<uint32_t>(L.items, s->cus * per_cu);
     L.grid = std::max<uint32_t>((grid + 7u) & ~7u, 8u);
@@ -1692,6 +2048,7 @@ static jxrhip_status enqueue_persistent(jxrhip_session *s, const PreparedBatch &
     args.tile_y = L.ty;
     args.lds_words = L.words;
     args.storage = L.storage;
+    args.reduced = L.reduced;
     HIP_TRY(launch_expand(args.slots, L.n, L.items, (uint32_t *)args.item_slot, args.counters, ls), "item table");
     HIP_TRY(launch_tiles(args, L.grid, ls), "persistent launch");
     s->diag.kernel_launches++;
@@ -1988,6 +2345,7 @@ extern "C" jxrhip_status jxrhip_decode_batch_static(jxrhip_session *s, const jxr
       if (st.bucket_of[i] == b) words = std::max(words, tile_words(plans[i], tx, ty));
     if ((size_t)words * 4 > s->max_lds) return fail(JXRHIP_ERR_INVALID_PLAN, "tile does not fit in LDS");
     BucketLaunch L{next_slot, 0, 0, tx, ty, words, first->dev.storage, 0};
+    L.reduced = first->scale != JXRHIP_SCALE_FULL ? 1 : 0;
     for (uint32_t i = 0; i < count; i++) {
       if (st.bucket_of[i] != b) continue;
       for (size_t gi = 0; gi < plans[i]->groups.size(); gi++) {
@@ -2006,7 +2364,8 @@ extern "C" jxrhip_status jxrhip_decode_batch_static(jxrhip_session *s, const jxr
     }
     L.n = (uint32_t)(next_slot - L.first);
     int resident = 0;
-    if (!s->groups_per_cu) HIP_TRY(kernel_occupancy(L.storage, (size_t)L.words * 4, &resident), "occupancy");
+    if (!s->groups_per_cu)
+      HIP_TRY(kernel_occupancy(L.storage, L.reduced, (size_t)L.words * 4, &resident), "occupancy");
     const uint32_t per_cu = s->groups_per_cu ? s->groups_per_cu : (uint32_t)std::max(resident, 1);
     L.grid = std::max<uint32_t>((std::min<uint32_t>(L.items, s->cus * per_cu) + 7u) & ~7u, 8u);
     total_items += L.items;
@@ -2032,6 +2391,7 @@ extern "C" jxrhip_status jxrhip_decode_batch_static(jxrhip_session *s, const jxr
     args.tile_y = L.ty;
     args.lds_words = L.words;
     args.storage = L.storage;
+    args.reduced = L.reduced;
     // the item-table kernel also zeroes the share counters and the status words
     HIP_TRY(launch_expand(args.slots, L.n, L.items, (uint32_t *)args.item_slot, args.counters, s0), "item table");
     HIP_TRY(launch_tiles(args, L.grid, s0), "persistent launch");

@@ -152,6 +152,7 @@ class OutputSpec:
     crop: Optional[tuple] = None  # (x, y, w, h) in the coded plane
     row_padding: int = 0          # extra bytes per row (strided surfaces)
     components: int = 3           # colour channels of an N-component layout
+    scale: int = abi.SCALE_FULL   # DecodeScale: full, quarter or sixteenth
 
     @property
     def bytes_per_sample(self):
@@ -256,10 +257,22 @@ def image_desc(enc: EncodedImage, spec: OutputSpec) -> abi.ImageDesc:
     o.red_blue_not_swapped = 1 if spec.red_blue_not_swapped else 0
     o.premultiply_alpha = 1 if spec.premultiply else 0
     o.chroma_centering_x, o.chroma_centering_y = spec.centering
-    cx, cy, cw, ch = spec.crop if spec.crop else (0, 0, enc.width, enc.height)
+    cx, cy, cw, ch = reduced_crop(spec.crop if spec.crop else (0, 0, enc.width, enc.height), spec.scale)
     o.crop_x, o.crop_y, o.crop_width, o.crop_height = cx, cy, cw, ch
+    d.scale = spec.scale
     d.surface = surface_layout(cw, ch, spec)
     return d
+
+
+def reduced_crop(region, scale: int):
+    """The output crop of a full-resolution region (x, y, w, h) at a decode
+    scale: its origin divided and its far edges rounded up by 4 or 16
+    (native_output_region, crates/jxr-native/src/prepare.rs:196-215; output_crop,
+    crates/jxr-native/src/decode/output.rs:484-501)."""
+    den = {abi.SCALE_FULL: 1, abi.SCALE_QUARTER: 4, abi.SCALE_SIXTEENTH: 16}[scale]
+    x, y, w, h = region
+    x0, y0 = x // den, y // den
+    return x0, y0, -(-(x + w) // den) - x0, -(-(y + h) // den) - y0
 
 
 # ---------------------------------------------------------------------------

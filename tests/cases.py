@@ -69,6 +69,7 @@ def make(case):
     if "crop" in case:
         spec.crop = case["crop"]
     spec.centering = case.get("centering", (0, 0))
+    spec.scale = case.get("scale", abi.SCALE_FULL)
     return enc, spec, px
 
 

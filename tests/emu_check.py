@@ -61,13 +61,40 @@ def main():
                 print(f"MISMATCH {name} tile {tile}", flush=True)
                 failures += 1
     session.set_tile(0, 0)
-    # persistent batch over every case at once
+    # reduced resolution: every case at a quarter and a sixteenth (the one
+    # combination the oracle refuses, the plan refuses too)
+    for tile in tiles[:2]:
+        session.set_tile(*tile)
+        for name in names:
+            for scale in (abi.SCALE_QUARTER, abi.SCALE_SIXTEENTH):
+                enc, spec, _ = make(dict(CASES[name], scale=scale))
+                code, _, want = O.decode_status(image_desc(enc, spec))
+                try:
+                    got = session.decode_to_host(HipDecodePlan.from_encoded(enc, spec))
+                except Exception as e:  # noqa: BLE001
+                    if not code:
+                        print(f"ERROR {name} scale {scale} tile {tile}: {e}", flush=True)
+                        failures += 1
+                    continue
+                if code or not np.array_equal(got, want):
+                    print(f"MISMATCH {name} scale {scale} tile {tile}", flush=True)
+                    failures += 1
+    session.set_tile(0, 0)
+    # persistent batch over every case at once, and every third case at a
+    # reduced scale besides
     encs, plans, wants = [], [], []
-    for name in names:
-        enc, spec, _ = make(CASES[name])
-        encs.append(enc)
-        plans.append(HipDecodePlan.from_encoded(enc, spec))
-        wants.append(O.decode(image_desc(enc, spec)))
+    batch_names = []
+    for k, name in enumerate(names):
+        scales = [abi.SCALE_FULL] + ([abi.SCALE_QUARTER + k // 3 % 2] if k % 3 == 0 else [])
+        for scale in scales:
+            enc, spec, _ = make(dict(CASES[name], scale=scale))
+            if O.decode_status(image_desc(enc, spec))[0]:
+                continue
+            encs.append(enc)
+            plans.append(HipDecodePlan.from_encoded(enc, spec))
+            wants.append(O.decode(image_desc(enc, spec)))
+            batch_names.append(name if scale == abi.SCALE_FULL else f"{name}@{scale}")
+    names = batch_names
     for mode in (abi.BATCH_PERSISTENT, abi.BATCH_PER_IMAGE):
         bufs = [session.allocate_destination(p.output_bytes) for p in plans]
         stages = session.submit_batch(plans, [b.ptr for b in bufs], mode).wait()

@@ -173,3 +173,41 @@ def test_n_component_images_up_to_the_reference_limit():
         else:
             refused(d)
             assert O.decode_status(d)[0] == abi.ERR_INVALID_PLAN
+
+
+def test_reduced_scales_validate_their_crop_and_policy():
+    """A reduced decode's crop is in the coded plane at its scale (3 x 2
+    macroblocks: 12 x 8 at a quarter, 3 x 2 at a sixteenth); a sixteenth of
+    subsampled planar YUV output and an unknown scale are refused by both."""
+    from jxr_amd import EncodedImage, image_desc, reduced_crop, synth
+    a = luma_arena()
+    for scale, (w, h) in [(abi.SCALE_QUARTER, (12, 8)), (abi.SCALE_SIXTEENTH, (3, 2))]:
+        d = luma_desc(a, crop=(0, 0, w, h))
+        d.scale = scale
+        d.surface = surface_for(w, h)
+        assert HipDecodePlan(d).output_bytes == w * h
+        assert O.decode_status(d)[0] == 0
+        d = luma_desc(a, crop=(0, 0, w + 1, h))
+        d.scale = scale
+        d.surface = surface_for(w + 1, h)
+        refused(d)
+        assert O.decode_status(d)[0] == abi.ERR_INVALID_PLAN
+    d = luma_desc(a)
+    d.scale = 3
+    refused(d)
+    assert O.decode_status(d)[0] == abi.ERR_INVALID_PLAN
+    enc = EncodedImage(synth.gradient_noise(64, 48, 3, depth=8), color=abi.COLOR_YUV420, overlap=1)
+    yuv = OutputSpec(storage=abi.STORE_U8, depth=abi.DEPTH_U8, layout=abi.LAYOUT_YUV, output_color=abi.COLOR_YUV420,
+                     scale=abi.SCALE_SIXTEENTH)
+    refused(image_desc(enc, yuv), abi.ERR_UNSUPPORTED_OUTPUT)
+    assert O.decode_status(image_desc(enc, yuv))[0] == abi.ERR_UNSUPPORTED_OUTPUT
+    yuv.scale = abi.SCALE_QUARTER
+    assert HipDecodePlan.from_encoded(enc, yuv).output_bytes == 16 * 12 + 2 * 8 * 6
+    # native_output_region: origin divided, far edges rounded up
+    assert reduced_crop((37, 21, 77, 58), abi.SCALE_QUARTER) == (9, 5, 20, 15)
+    assert reduced_crop((37, 21, 77, 58), abi.SCALE_SIXTEENTH) == (2, 1, 6, 4)
+
+
+def surface_for(w, h):
+    from jxr_amd import surface_layout
+    return surface_layout(w, h, LUMA8)
