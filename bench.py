@@ -9,6 +9,11 @@ the plain per-image path (one launch per image, four streams) for the
 batched/per-image ratio. Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|c3|c4]
+                  [--scale full|quarter|sixteenth]
+
+--scale decodes the same arenas at a reduced resolution (the reference's
+DecodeScale): pixels are the reduced output's, and the algorithmic bytes the
+coefficients such a decode reads.
 """
 import argparse
 import json
@@ -216,6 +221,8 @@ def main():
     ap.add_argument("--serve-loaded-rate", type=float, default=40000.0,
                     help="arrivals per second of a second, loaded serving run (0: none)")
     ap.add_argument("--serve-budget-gb", type=float, default=0.0, help="admission budget (0: the server's default, 8 GiB or 3/4 of free HBM if less)")
+    ap.add_argument("--scale", default="full", choices=["full", "quarter", "sixteenth"],
+                    help="decode resolution (DecodeScale)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also copy 256 MiB device-to-device (HBM counter calibration)")
     args = ap.parse_args()
@@ -232,6 +239,11 @@ def main():
     t0 = time.perf_counter()
     items = make()
     log(f"[bench] built {len(items)} arenas for {args.workload} in {time.perf_counter() - t0:.1f} s")
+    scale = {"full": abi.SCALE_FULL, "quarter": abi.SCALE_QUARTER, "sixteenth": abi.SCALE_SIXTEENTH}[args.scale]
+    for _, spec in items:
+        spec.scale = scale
+    # counter summaries are of full-resolution decodes
+    figures = args.workload if scale == abi.SCALE_FULL else None
     session = HipDecoderSession(0)
     if args.tile:
         tx, ty = (int(v) for v in args.tile.lower().split("x"))
@@ -318,7 +330,7 @@ def main():
         "dtype": "int32",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.workload}: {desc}",
+            "workload": f"{args.workload}: {desc}" + ("" if scale == abi.SCALE_FULL else f", decoded at a {args.scale}"),
             "images": len(items),
             "megapixels_per_step": round(pixels / 1e6, 3),
             "algorithmic_gb_per_step": round(alg / 1e9, 4),
@@ -332,10 +344,10 @@ def main():
                        "p99": round(percentile(batched["wall_ms"], 99), 4)},
         "device_ms_per_step": round(dev_ms, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(figures),
                      "kernel": "jxr_reconstruct_tiles", "avg_launch_ms": round(per_launch_ms, 4)},
     }
-    valu = profile_figure("valu_instructions_per_launch", args.workload)
+    valu = profile_figure("valu_instructions_per_launch", figures)
     if valu is not None:
         # the kernel's VALU instruction rate against the rate a plain chain
         # of independent integer VALU instructions reaches on this device

@@ -313,8 +313,10 @@ uint32_t quant_map(uint32_t qp, bool scaled, uint32_t shift) {
   return m << e;
 }
 
-inline i32 quantize(i32 c, uint32_t step) {
-  uint32_t offset = step / 2;
+// Quantizer::quantize (quantization.rs:27-48): the magnitude plus `offset`,
+// divided by the step toward zero; an offset of a step or more counts as
+// one less than the step.
+inline i32 quantize(i32 c, uint32_t step, uint32_t offset) {
   if (offset >= step) offset = step - 1;
   uint32_t mag = c < 0 ? 0u - (uint32_t)c : (uint32_t)c;
   uint32_t carry = (mag % step) >= (step - offset) ? 1u : 0u;
@@ -446,7 +448,7 @@ void analyze_plane(const PlaneIn &p, uint32_t component, const jxrhip_encode_par
           for (uint32_t r = 0; r < br; r++)
             for (uint32_t c = 0; c < bc; c++) {
               i32 v = low[(size_t)(my * br + r) * lw + mx * bc + c];
-              lp[r * bc + c] = scale_after_first ? (v >> 1) : v;
+              lp[r * bc + c] = v;
             }
           if (bc == 4)
             forward_core(lp);
@@ -459,8 +461,15 @@ void analyze_plane(const PlaneIn &p, uint32_t component, const jxrhip_encode_par
           for (uint32_t k = 0; k < blocks; k++) {
             uint32_t bx = mx * bc + k % bc, by = my * br + k / bc;
             const i32 *h = &high[((size_t)by * lw + bx) * 16];
-            q[k * 16] = quantize(lp[k], k == 0 ? sdc : slp);
-            for (int j = 1; j < 16; j++) q[k * 16 + j] = quantize(h[j], shp);
+            // reconstruction doubles a scaled component's DC and low-pass
+            // band after its first level: they are divided by twice their
+            // step, rounding to nearest (analyze.rs:500-530)
+            const uint32_t step = k == 0 ? sdc : slp;
+            const uint32_t divisor = scale_after_first ? 2 * step : step;
+            q[k * 16] = quantize(lp[k], divisor, divisor / 2);
+            // the high-pass band's zero bin is wider: an offset of 3/8 step
+            // (analyze.rs:532-536)
+            for (int j = 1; j < 16; j++) q[k * 16 + j] = quantize(h[j], shp, shp / 8 * 3);
           }
           uint32_t bands = prm.bands;
           // High-pass direction from the low-pass coefficients' orientation.
