@@ -127,7 +127,9 @@ def cpu_baseline(items, plans, dense, offsets, budget_s=15.0):
             checked += 1
     return dict(value=round(px / dt / 1e6, 3), unit="Mpixel/s", cores=threads, kind="port",
                 sample=f"{n} of {len(descs)} images ({px / 1e6:.1f} Mpx) per pass, median of {len(runs)} passes "
-                       f"{dt:.2f} s (p95 {slow:.2f} s), oracle on {threads} threads",
+                       f"{dt:.2f} s (p95 {slow:.2f} s), oracle on {threads} threads: a scalar C restatement "
+                       f"of the reconstruction, one image per thread, capped at the sixteen host cores of a "
+                       f"one-GPU share; the reference's CPU decoder also runs SIMD lanes, so this understates it",
                 p95_value=round(px / slow / 1e6, 3), passes=len(runs), parity_checked_images=checked)
 
 
