@@ -73,11 +73,30 @@ uint32_t jxrhip_quant_step(uint32_t qp, uint32_t scaled, uint32_t scaled_shift);
  * and HP parameters of component 0, `chroma` those of components 1 and 2,
  * which are colour differences at color_format's chroma sampling when
  * `rgb_source` is nonzero and take the luma parameters otherwise; *overlap
- * receives the mode the quality picks when a request leaves it open. Float
- * sources are not covered. */
+ * receives the mode the quality picks when a request leaves it open. */
 jxrhip_status jxrhip_quality_parameters(uint32_t quality, uint32_t color_format, uint32_t rgb_source,
                                         uint32_t range, uint32_t luma[3], uint32_t chroma[3],
                                         uint32_t *overlap);
+/* As jxrhip_quality_parameters for a source of any sample scale: `range`,
+ * `is_float` and `lowest_quality` as jxrhip_sample_scale gives them. Float
+ * sources keep their differences at the primary's whole levels, and every
+ * quality below `lowest_quality` codes as it does (quality.rs:295-329). */
+jxrhip_status jxrhip_quality_parameters_scaled(uint32_t quality, uint32_t color_format, uint32_t rgb_source,
+                                               uint32_t range, uint32_t is_float, uint32_t lowest_quality,
+                                               uint32_t luma[3], uint32_t chroma[3], uint32_t *overlap);
+/* SampleScale (quality.rs:194-292): how the coded samples of a source
+ * stored at `depth` (JXRHIP_DEPTH_*; RGBE is JXRHIP_DEPTH_U8 with
+ * output_color JXRHIP_COLOR_RGBE) compare with 8-bit samples, for its colour
+ * planes (alpha = 0) or its alpha plane (alpha = 1). *range is the coded
+ * distance standing for the 8-bit range: an integer's maximum less the bits
+ * shift_bits drops, a fixed-point sample's one, and for floats eight
+ * octaves of bit patterns (two for alpha, at most the patterns up to one),
+ * fewer for a 32-bit float whose exponent bias is below six; *is_float says
+ * the samples are float patterns; *lowest_quality is the quality every
+ * lower one codes as (10 for RGBE, else 0). */
+jxrhip_status jxrhip_sample_scale(uint32_t depth, uint32_t output_color, uint32_t shift_bits,
+                                  uint32_t mantissa_length, int32_t exponent_bias, uint32_t alpha,
+                                  uint32_t *range, uint32_t *is_float, uint32_t *lowest_quality);
 /* The reconstructed sample whose f32 output is nearest `bits`
  * (crates/jxr-native/src/output_format/scaling.rs:177-229). */
 jxrhip_status jxrhip_float32_sample(uint32_t bits, uint32_t scaled, uint32_t mantissa_length,

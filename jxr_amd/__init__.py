@@ -19,6 +19,7 @@ from .decoder import (
     plan_hip_route,
     quality_parameters,
     reduced_crop,
+    sample_scale,
     surface_layout,
 )
 from .serving import HipServer, ServerResult
@@ -26,5 +27,5 @@ from .serving import HipServer, ServerResult
 __all__ = [
     "abi", "JxrHipError", "load", "EncodedImage", "HipBatch", "HipDecodePlan", "HipDecoderSession",
     "HipSubmission", "OutputSpec", "ResidentHipImage", "image_desc", "plan_hip_route", "quality_parameters",
-    "surface_layout", "reduced_crop", "HipServer", "ServerResult",
+    "surface_layout", "reduced_crop", "sample_scale", "HipServer", "ServerResult",
 ]

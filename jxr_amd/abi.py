@@ -372,6 +372,10 @@ SIGNATURES = {
     "jxrhip_quality_parameters": (st, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p, u32p,
                                       C.POINTER(C.c_uint32)]),
     "jxrhip_float32_sample": (st, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_int32)]),
+    "jxrhip_quality_parameters_scaled": (st, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                              C.c_uint32, u32p, u32p, u32p]),
+    "jxrhip_sample_scale": (st, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32, u32p, u32p,
+                                 u32p]),
 }
 
 _lib = None

@@ -718,6 +718,13 @@ uint32_t quality_overlap(uint32_t q) { return q >= 99 ? 0u : q >= 7 ? 1u : 2u; }
 extern "C" jxrhip_status jxrhip_quality_parameters(uint32_t quality, uint32_t color_format, uint32_t rgb_source,
                                                    uint32_t range, uint32_t luma[3], uint32_t chroma[3],
                                                    uint32_t *overlap) {
+  return jxrhip_quality_parameters_scaled(quality, color_format, rgb_source, range, 0, 0, luma, chroma, overlap);
+}
+
+extern "C" jxrhip_status jxrhip_quality_parameters_scaled(uint32_t quality, uint32_t color_format,
+                                                          uint32_t rgb_source, uint32_t range, uint32_t is_float,
+                                                          uint32_t lowest_quality, uint32_t luma[3],
+                                                          uint32_t chroma[3], uint32_t *overlap) {
   if (!luma || !chroma || !overlap) {
     jxrhip_set_error("null argument");
     return JXRHIP_ERR_INVALID_ARGUMENT;
@@ -732,7 +739,9 @@ extern "C" jxrhip_status jxrhip_quality_parameters(uint32_t quality, uint32_t co
     *overlap = 0;
     return JXRHIP_OK;
   }
-  const uint32_t level = quality_level(quality);
+  // every quality below the scale's lowest codes as it does (quality.rs:303)
+  const uint32_t q = std::max(quality, std::min<uint32_t>(lowest_quality, 99));
+  const uint32_t level = quality_level(q);
   const uint32_t primary = (level + 1) / 2;
   if (range < kReferenceRange) {
     const uint32_t p = reduced_parameter(primary, range);
@@ -741,10 +750,12 @@ extern "C" jxrhip_status jxrhip_quality_parameters(uint32_t quality, uint32_t co
       chroma[b] = p && difference ? difference_parameter(p, color_format) : p;
     }
     // lossless coding takes no overlap (quality.rs:126-128)
-    *overlap = p == 0 ? 0u : quality_overlap(quality);
+    *overlap = p == 0 ? 0u : quality_overlap(q);
     return JXRHIP_OK;
   }
-  const uint32_t diff = level / 2;
+  // float patterns keep the differences at the primary's whole levels
+  // (quality.rs:312-315)
+  const uint32_t diff = is_float ? primary : level / 2;
   band_parameters(primary, primary, luma);
   if (difference)
     band_parameters(diff, difference_parameter(diff, color_format), chroma);
@@ -754,7 +765,82 @@ extern "C" jxrhip_status jxrhip_quality_parameters(uint32_t quality, uint32_t co
     luma[b] = scaled_parameter(luma[b], range);
     chroma[b] = scaled_parameter(chroma[b], range);
   }
-  *overlap = quality_overlap(quality);
+  *overlap = quality_overlap(q);
+  return JXRHIP_OK;
+}
+
+namespace {
+constexpr int32_t kFloatOctavesLog2 = 3;     // eight octaves for the 8-bit range (quality.rs:41-51)
+constexpr int32_t kLogarithmicFromBias = 6;  // quality.rs:53-61
+constexpr int32_t kAlphaOctavesLog2 = 1;     // two octaves for alpha (quality.rs:63-72)
+constexpr uint32_t kRgbeLowestQuality = 10;  // quality.rs:74-81
+
+// The coded distance of 2^octaves_log2 octaves of float patterns with
+// `mantissa_bits` mantissa bits; a shift of 32 or more gives none, and the
+// range is at least one (quality.rs:259-274).
+uint32_t pattern_range(uint32_t mantissa_bits, int32_t octaves_log2) {
+  const uint32_t magnitude = (uint32_t)(octaves_log2 < 0 ? -(int64_t)octaves_log2 : octaves_log2);
+  const uint32_t octave = 1u << std::min<uint32_t>(mantissa_bits, 23);
+  uint32_t range = 0;
+  if (magnitude < 32) range = octaves_log2 >= 0 ? octave << magnitude : octave >> magnitude;
+  return std::max<uint32_t>(range, 1);
+}
+}  // namespace
+
+extern "C" jxrhip_status jxrhip_sample_scale(uint32_t depth, uint32_t output_color, uint32_t shift_bits,
+                                             uint32_t mantissa_length, int32_t exponent_bias, uint32_t alpha,
+                                             uint32_t *range, uint32_t *is_float, uint32_t *lowest_quality) {
+  if (!range || !is_float || !lowest_quality) {
+    jxrhip_set_error("null argument");
+    return JXRHIP_ERR_INVALID_ARGUMENT;
+  }
+  *is_float = 0;
+  *lowest_quality = 0;
+  const bool f16 = depth == JXRHIP_DEPTH_F16, f32 = depth == JXRHIP_DEPTH_F32;
+  if (alpha && (f16 || f32)) {
+    // the patterns of two octaves, or those up to one if fewer
+    // (quality.rs:238-257)
+    uint32_t bits = f16 ? 10u : mantissa_length, one = 0x3c00;
+    if (f32) {
+      int32_t sample = 0;
+      if (jxrhip_float32_sample(0x3f800000u, 0, mantissa_length, exponent_bias, &sample) != JXRHIP_OK)
+        return JXRHIP_ERR_INVALID_ARGUMENT;
+      one = sample > 0 ? (uint32_t)sample : 0u;
+    }
+    *range = std::max<uint32_t>(std::min<uint32_t>(pattern_range(bits, kAlphaOctavesLog2), one), 1);
+    *is_float = 1;
+    return JXRHIP_OK;
+  }
+  if (!alpha && depth == JXRHIP_DEPTH_U8 && output_color == JXRHIP_COLOR_RGBE) {
+    // an exponent above a 7-bit mantissa (quality.rs:216-220)
+    *range = pattern_range(7, kFloatOctavesLog2);
+    *is_float = 1;
+    *lowest_quality = kRgbeLowestQuality;
+    return JXRHIP_OK;
+  }
+  if (f16 || f32) {
+    // a 32-bit float coded linearly below 2^(1 - bias) halves the range
+    // with each step of bias below six (quality.rs:221-230)
+    const int32_t linear = f32 ? std::max<int32_t>(kLogarithmicFromBias - exponent_bias, 0) : 0;
+    *range = pattern_range(f16 ? 10u : mantissa_length, kFloatOctavesLog2 - linear);
+    *is_float = 1;
+    return JXRHIP_OK;
+  }
+  // integers span their range, fixed point zero to one, less the dropped
+  // bits (quality.rs:276-291, output_policy.rs:99-108)
+  switch (depth) {
+    case JXRHIP_DEPTH_U16: *range = 65535u >> std::min<uint32_t>(shift_bits, 31); break;
+    case JXRHIP_DEPTH_I16: *range = (1u << 13) >> std::min<uint32_t>(shift_bits, 31); break;
+    case JXRHIP_DEPTH_I32: *range = (1u << 24) >> std::min<uint32_t>(shift_bits, 31); break;
+    case JXRHIP_DEPTH_BIT1_WHITE:
+    case JXRHIP_DEPTH_BIT1_BLACK: *range = 1; break;
+    case JXRHIP_DEPTH_U10:
+    case JXRHIP_DEPTH_RGB101010: *range = 1023; break;
+    case JXRHIP_DEPTH_RGB555: *range = 31; break;
+    case JXRHIP_DEPTH_RGB565: *range = 63; break;  // red and blue coded at twice their five bits
+    default: *range = 255; break;
+  }
+  *range = std::max<uint32_t>(*range, 1);
   return JXRHIP_OK;
 }
 

@@ -23,17 +23,32 @@ from .abi import JxrHipError, check, load
 # ---------------------------------------------------------------------------
 # encoded images (forward analysis into arenas)
 # ---------------------------------------------------------------------------
-def quality_parameters(quality: int, color: int, rgb_source: bool, sample_range: int):
+def quality_parameters(quality: int, color: int, rgb_source: bool, sample_range: int, is_float: bool = False,
+                       lowest_quality: int = 0):
     """The reference encoder's quality scale (crates/jxr-native/src/encode/
     quality.rs:101-499): ((DC, LP, HP) QPs of luma, (DC, LP, HP) QPs of
-    components 1 and 2, overlap mode the quality picks) for an integer
-    source whose samples span `sample_range` values (255 for 8-bit, 65535
-    for 16-bit). Components 1 and 2 of an RGB source are colour differences
-    at the chroma sampling of `color`. QP 0 is lossless."""
+    components 1 and 2, overlap mode the quality picks) for a source whose
+    coded samples span `sample_range` values for the 8-bit range (255 for
+    8-bit, 65535 for 16-bit; sample_scale() gives it for any format).
+    Components 1 and 2 of an RGB source are colour differences at the chroma
+    sampling of `color`. QP 0 is lossless."""
     luma, chroma, ov = (C.c_uint32 * 3)(), (C.c_uint32 * 3)(), C.c_uint32()
-    check(load().jxrhip_quality_parameters(int(quality), int(color), 1 if rgb_source else 0, int(sample_range),
-                                           luma, chroma, C.byref(ov)), "quality parameters")
+    check(load().jxrhip_quality_parameters_scaled(int(quality), int(color), 1 if rgb_source else 0,
+                                                  int(sample_range), 1 if is_float else 0, int(lowest_quality),
+                                                  luma, chroma, C.byref(ov)), "quality parameters")
     return tuple(luma), tuple(chroma), int(ov.value)
+
+
+def sample_scale(depth: int, output_color: int = abi.COLOR_RGB, shift_bits: int = 0, mantissa_length: int = 13,
+                 exponent_bias: int = 4, alpha: bool = False):
+    """SampleScale (crates/jxr-native/src/encode/quality.rs:194-292) of a
+    source stored at `depth`: (range, is_float, lowest_quality) of its colour
+    planes, or of its alpha plane."""
+    r, f, lq = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    check(load().jxrhip_sample_scale(int(depth), int(output_color), int(shift_bits), int(mantissa_length),
+                                     int(exponent_bias), 1 if alpha else 0, C.byref(r), C.byref(f), C.byref(lq)),
+          "sample scale")
+    return int(r.value), bool(f.value), int(lq.value)
 
 
 class EncodedImage:
@@ -89,9 +104,16 @@ class EncodedImage:
         parameters for the source's sample range, scaled arithmetic unless
         every component is lossless, and the quality's overlap mode unless
         `overlap` is given (crates/jxr-native/src/encode/quality.rs:101-150)."""
-        sample_range = {np.dtype(np.uint8): 255, np.dtype(np.uint16): 65535}[np.asarray(pixels).dtype]
+        dtype = np.asarray(pixels).dtype
+        if dtype == np.float32:
+            # float sources: the coded bit patterns' scale at the request's
+            # precision (quality.rs:221-230)
+            scale = sample_scale(abi.DEPTH_F32, mantissa_length=kw.get("mantissa_length", 13),
+                                 exponent_bias=kw.get("exponent_bias", 4))
+        else:
+            scale = ({np.dtype(np.uint8): 255, np.dtype(np.uint16): 65535}[dtype], False, 0)
         rgb = pixels.ndim == 3 and pixels.shape[2] >= 3
-        luma, chroma, ov = quality_parameters(quality, color, rgb, sample_range)
+        luma, chroma, ov = quality_parameters(quality, color, rgb, *scale)
         return cls(pixels, color=color, overlap=ov if overlap is None else overlap,
                    scaled=any(luma) or any(chroma), qp=luma, chroma_qp=chroma, **kw)
 

@@ -115,3 +115,45 @@ def test_invalid_requests_are_refused():
         Q(101, RGB444, True, 255)
     with pytest.raises(abi.JxrHipError):
         Q(50, RGB444, True, 0)
+
+
+def test_sample_ranges_follow_the_coded_samples():
+    """quality.rs:794-853: integer ranges less the dropped bits, fixed point
+    from zero to one, and floats as octaves of bit patterns (eight for
+    colour, two for alpha), a 32-bit float's range halving with each step
+    of exponent bias below six, RGBE coded no lower than quality 10."""
+    from jxr_amd import sample_scale as S
+    for depth, rng in [(abi.DEPTH_BIT1_WHITE, 1), (abi.DEPTH_RGB555, 31), (abi.DEPTH_RGB565, 63),
+                       (abi.DEPTH_U8, 255), (abi.DEPTH_RGB101010, 1023), (abi.DEPTH_U16, 65535),
+                       (abi.DEPTH_I16, 8192), (abi.DEPTH_I32, 1 << 24)]:
+        assert S(depth) == (rng, False, 0), depth
+        assert S(depth, alpha=True)[0] == rng, depth
+    assert S(abi.DEPTH_U16, shift_bits=4)[0] == 4095
+    assert S(abi.DEPTH_I16, shift_bits=4)[0] == 512
+    assert S(abi.DEPTH_I32, shift_bits=4)[0] == 1 << 20
+    assert S(abi.DEPTH_F16) == (8 << 10, True, 0)
+    assert S(abi.DEPTH_F16, alpha=True)[0] == 2 << 10
+    assert S(abi.DEPTH_U8, output_color=abi.COLOR_RGBE) == (8 << 7, True, 10)
+    for bias, rng, alpha_rng in [(15, 8 << 13, 2 << 13), (6, 8 << 13, 2 << 13), (5, 4 << 13, 2 << 13),
+                                 (4, 2 << 13, 2 << 13), (1, 1 << 11, 1 << 13), (0, 1 << 10, 1 << 12),
+                                 (-40, 1, 1)]:
+        assert S(abi.DEPTH_F32, mantissa_length=13, exponent_bias=bias) == (rng, True, 0), bias
+        assert S(abi.DEPTH_F32, mantissa_length=13, exponent_bias=bias, alpha=True)[0] == alpha_rng, bias
+    assert S(abi.DEPTH_F32, mantissa_length=16, exponent_bias=4)[0] == 2 << 16
+
+
+def test_float_differences_keep_to_the_primary_whole_levels():
+    """quality.rs:312-315: at every lossy quality a float source's colour
+    differences take the primary's level, where an integer source's round
+    its half level down; RGBE codes every quality below 10 as 10."""
+    for q in range(0, 100):
+        luma, chroma, _ = Q(q, RGB444, True, 8 << 13, True, 0)
+        ref_l, ref_c, _ = Q(q, RGB444, True, 8 << 13, False, 0)
+        assert luma == ref_l
+        # the float's difference parameter is never finer than the integer's
+        assert all(c >= r for c, r in zip(chroma, ref_c)), q
+        if q < 10:
+            assert Q(q, RGB444, True, 8 << 7, True, 10) == Q(10, RGB444, True, 8 << 7, True, 10)
+    # at a level whose half rounds down, the two differ
+    assert any(Q(q, RGB444, True, 8 << 13, True, 0)[1] != Q(q, RGB444, True, 8 << 13, False, 0)[1]
+               for q in range(0, 99))
