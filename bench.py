@@ -384,6 +384,11 @@ def main():
             # with the device busy (waves of several images)
             out["serving_loaded"] = serve(session, min(args.serve_seconds, 5.0), args.serve_uploads == "resident",
                                           args.serve_budget_gb, args.serve_loaded_rate)
+        if args.serve_uploads == "resident":
+            # the same stream with every request uploading its own
+            # coefficients from host memory: PCIe inside the latency
+            out["serving_uploads"] = serve(session, min(args.serve_seconds, 3.0), False, args.serve_budget_gb,
+                                           args.serve_rate)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(items, plans, dense, offsets)
     print(json.dumps(out), flush=True)
