@@ -132,11 +132,14 @@ struct ProbeSet {
   uint32_t arena, bc, br, overlap;
   bool scale_first, alpha;
   uint32_t mbx, mby;                     // probe macroblocks
+  std::vector<uint32_t> bounds_x, bounds_y;  // probe regions (macroblocks), 0 .. extent
   std::vector<uint32_t> colmask, rowmask;
   std::vector<MbRec> recs;               // shared by every probe image
-  std::vector<int32_t> coef[2 * kProbeLevels];  // +L1, -L1, +L2, ...: DC and low-pass
   uint32_t stride, length;               // probe picture: samples across, samples
   int64_t span, gain;
+  // what the terms depend on: block shape, doubling, overlap, decode scale
+  // and the probe's regions; a session computes them once per key
+  std::vector<uint32_t> key;
 };
 
 static std::atomic<uint64_t> g_plan_ids{1};
@@ -399,17 +402,17 @@ static jxrhip_status build_reduction(jxrhip_plan *plan) {
       q.mby = by.back();
       probe_mask(bx, q.colmask);
       probe_mask(by, q.rowmask);
+      q.bounds_x = bx;
+      q.bounds_y = by;
       const uint32_t blocks = pp.bc * pp.br;
       q.recs.resize((size_t)q.mbx * q.mby);
       for (uint32_t m = 0; m < q.mbx * q.mby; m++)
         q.recs[m] = MbRec{m * blocks, 1, 1, 1, JXRHIP_BANDS_NO_HIGH_PASS, JXRHIP_PRED_NONE, 0, 0};
       q.span = 0;
-      for (uint32_t k = 0; k < kProbeLevels; k++) {
-        const int32_t level = 0x10001 + (int32_t)k;
-        flat_probe_coefficients(level, pp.bc, pp.br, pp.scale_first, pp.overlap, bx, by, q.coef[2 * k]);
-        flat_probe_coefficients(-level, pp.bc, pp.br, pp.scale_first, pp.overlap, bx, by, q.coef[2 * k + 1]);
-        q.span += 2 * (int64_t)level;
-      }
+      for (uint32_t k = 0; k < kProbeLevels; k++) q.span += 2 * (int64_t)(0x10001 + (int32_t)k);
+      q.key = {pp.bc, pp.br, pp.scale_first ? 1u : 0u, pp.overlap, scale, (uint32_t)bx.size()};
+      q.key.insert(q.key.end(), bx.begin(), bx.end());
+      q.key.insert(q.key.end(), by.begin(), by.end());
       q.gain = 2 * (int64_t)kProbeLevels * q.span;
       q.stride = q.mbx * pp.ux;
       q.length = q.stride * q.mby * pp.uy;
@@ -737,7 +740,7 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
     plan->descriptor_bytes += p.recs.size() * sizeof(MbRec) + (p.colmask.size() + p.rowmask.size()) * 4 +
                               (p.probe_cols.size() + p.probe_rows.size()) * 4;
   }
-  for (const ProbeSet &q : plan->probes) plan->descriptor_bytes += (uint64_t)q.length * 16;
+
   for (uint32_t a = 0; a < plan->arena_count; a++) plan->coefficient_bytes += plan->arenas[a]->coefficient_count * 4;
   {
     // algorithmic bytes: 4 B per stored coefficient read, 32 B per
@@ -941,6 +944,10 @@ struct jxrhip_session {
   uint8_t *scratch = nullptr;
   std::vector<uint32_t> free_status;
   std::vector<hipEvent_t> free_events;
+  // the sample-scale terms of reduced decodes, by probe key: computed on the
+  // device once per key and kept until the session ends (descriptors point
+  // at them; each is a few kilobytes)
+  std::map<std::vector<uint32_t>, DeviceBuffer> probe_terms;
 };
 
 static constexpr uint32_t kStatusSlots = 16384;
@@ -1060,6 +1067,7 @@ static void session_free(jxrhip_session *s) {
     if (s->streams[i]) (void)hipStreamSynchronize(s->streams[i]);
   for (auto &kv : s->coef_cache) drop_entry(s, kv.second);
   for (auto &kv : s->desc_cache) drop_entry(s, kv.second);
+  for (auto &kv : s->probe_terms) s->pool.put(kv.second);
   if (s->status_dev) (void)hipFree(s->status_dev);
   if (s->status_host) (void)hipHostFree(s->status_host);
   for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
@@ -1187,22 +1195,36 @@ static jxrhip_status device_coefficients(jxrhip_session *s, const jxrhip_arena *
   return JXRHIP_OK;
 }
 
-// The terms of each probe set's sample scale (sample_scale.rs:157-246),
-// computed on the device into `terms`: the reduced kernel decodes the set's
-// flat probe images (+-level, four levels) into raw reduced pictures, and
-// jxr_scale_terms fits the line through them for every probe sample. Runs
-// once per descriptor upload and waits for the device; an overflow in a
-// probe, or a term whose divisor is not positive, fails the upload.
-static jxrhip_status run_probes(jxrhip_session *s, const jxrhip_plan *p, const std::vector<int64_t *> &terms,
-                                hipStream_t stream) {
+// The terms of probe sets `which` of a plan's sample scale
+// (sample_scale.rs:157-246), computed on the device into `terms[k]`: the
+// forward analysis gives each set's eight flat probe images (+-level, four
+// levels) on the host, the reduced kernel decodes them into raw reduced
+// pictures, and jxr_scale_terms fits the line through them for every probe
+// sample. Waits for the device; an overflow in a probe, or a term whose
+// divisor is not positive, fails.
+static jxrhip_status run_probes(jxrhip_session *s, const jxrhip_plan *p, const std::vector<size_t> &which,
+                                const std::vector<int64_t *> &terms, hipStream_t stream) {
+  constexpr uint32_t kImages = 2 * kProbeLevels;
+  // the flat probe images' DC and low-pass coefficients, +L1, -L1, +L2, ...
+  std::vector<std::vector<int32_t>> coef(which.size() * kImages);
+  for (size_t w = 0; w < which.size(); w++) {
+    const ProbeSet &q = p->probes[which[w]];
+    for (uint32_t k = 0; k < kProbeLevels; k++) {
+      const int32_t level = 0x10001 + (int32_t)k;
+      flat_probe_coefficients(level, q.bc, q.br, q.scale_first, q.overlap, q.bounds_x, q.bounds_y,
+                              coef[w * kImages + 2 * k]);
+      flat_probe_coefficients(-level, q.bc, q.br, q.scale_first, q.overlap, q.bounds_x, q.bounds_y,
+                              coef[w * kImages + 2 * k + 1]);
+    }
+  }
   // layout: status word | per set: records, masks, sample scale, pictures,
   // then each image's coefficients and descriptor
-  constexpr uint32_t kImages = 2 * kProbeLevels;
   size_t bytes = 256;
-  for (const ProbeSet &q : p->probes) {
+  for (size_t w = 0; w < which.size(); w++) {
+    const ProbeSet &q = p->probes[which[w]];
     bytes += q.recs.size() * sizeof(MbRec) + (q.colmask.size() + q.rowmask.size()) * 4 + sizeof(DevReduce) +
              (size_t)kImages * q.length * 4 + 5 * 64;
-    for (const auto &c : q.coef) bytes += c.size() * 4 + sizeof(DevImage) + 2 * 64;
+    for (uint32_t j = 0; j < kImages; j++) bytes += coef[w * kImages + j].size() * 4 + sizeof(DevImage) + 2 * 64;
   }
   std::vector<uint8_t> host(bytes, 0);
   DeviceBuffer b;
@@ -1223,7 +1245,8 @@ static jxrhip_status run_probes(jxrhip_session *s, const jxrhip_plan *p, const s
   };
   std::vector<ProbeLaunch> launches;
   std::vector<int32_t *> pictures;
-  for (const ProbeSet &q : p->probes) {
+  for (size_t w = 0; w < which.size(); w++) {
+    const ProbeSet &q = p->probes[which[w]];
     const MbRec *recs = (const MbRec *)place(q.recs.data(), q.recs.size() * sizeof(MbRec));
     const uint32_t *cm = (const uint32_t *)place(q.colmask.data(), q.colmask.size() * 4);
     const uint32_t *rm = (const uint32_t *)place(q.rowmask.data(), q.rowmask.size() * 4);
@@ -1234,11 +1257,12 @@ static jxrhip_status run_probes(jxrhip_session *s, const jxrhip_plan *p, const s
     int32_t *pic = (int32_t *)place(nullptr, (size_t)kImages * q.length * 4);
     pictures.push_back(pic);
     for (uint32_t j = 0; j < kImages; j++) {
-      const int32_t *coef = (const int32_t *)place(q.coef[j].data(), q.coef[j].size() * 4);
+      const std::vector<int32_t> &c = coef[w * kImages + j];
+      const int32_t *cd = (const int32_t *)place(c.data(), c.size() * 4);
       DevImage im;
       std::memset(&im, 0, sizeof im);
       DevPlane &pl = im.planes[0];
-      pl.coef = coef;
+      pl.coef = cd;
       pl.mbs = recs;
       pl.colmask = cm;
       pl.rowmask = rm;
@@ -1292,10 +1316,10 @@ static jxrhip_status run_probes(jxrhip_session *s, const jxrhip_plan *p, const s
     if ((e = launch_tiles(args, (tiles + 7u) & ~7u, stream)) != hipSuccess) return undo(hip_fail(e, "probe launch"));
     s->diag.kernel_launches++;
   }
-  for (size_t k = 0; k < p->probes.size(); k++) {
-    const ProbeSet &q = p->probes[k];
-    if ((e = launch_scale_terms(pictures[k], q.length, kProbeLevels, q.span, terms[k], (uint32_t *)dev, stream)) !=
-        hipSuccess)
+  for (size_t w = 0; w < which.size(); w++) {
+    const ProbeSet &q = p->probes[which[w]];
+    if ((e = launch_scale_terms(pictures[w], q.length, kProbeLevels, q.span, terms[which[w]], (uint32_t *)dev,
+                                stream)) != hipSuccess)
       return undo(hip_fail(e, "sample scale terms"));
     s->diag.kernel_launches++;
   }
@@ -1305,6 +1329,47 @@ static jxrhip_status run_probes(jxrhip_session *s, const jxrhip_plan *p, const s
   if (e != hipSuccess) return undo(hip_fail(e, "probe status"));
   s->pool.put(b);
   if (status) return fail(JXRHIP_ERR_KERNEL_ARITHMETIC, "reduced-resolution sample scale");
+  return JXRHIP_OK;
+}
+
+// The device terms of every probe set of a plan, from the session's table,
+// computing the sets it does not hold yet.
+static jxrhip_status sample_scale_terms(jxrhip_session *s, const jxrhip_plan *p, hipStream_t stream,
+                                       std::vector<int64_t *> &terms) {
+  terms.assign(p->probes.size(), nullptr);
+  std::vector<size_t> missing;
+  std::vector<DeviceBuffer> fresh;
+  for (size_t k = 0; k < p->probes.size(); k++) {
+    auto it = s->probe_terms.find(p->probes[k].key);
+    if (it != s->probe_terms.end()) {
+      terms[k] = (int64_t *)it->second.ptr;
+      continue;
+    }
+    // two sets of one plan may share a key: compute it once
+    bool dup = false;
+    for (size_t m : missing)
+      if (p->probes[m].key == p->probes[k].key) {
+        terms[k] = terms[m];
+        dup = true;
+      }
+    if (dup) continue;
+    DeviceBuffer b;
+    const hipError_t e = s->pool.get((size_t)p->probes[k].length * 16, b, s->diag.scratch_hits, s->diag.scratch_misses);
+    if (e != hipSuccess) {
+      for (auto &f : fresh) s->pool.put(f);
+      return hip_fail(e, "probe terms");
+    }
+    terms[k] = (int64_t *)b.ptr;
+    missing.push_back(k);
+    fresh.push_back(b);
+  }
+  if (missing.empty()) return JXRHIP_OK;
+  const jxrhip_status st = run_probes(s, p, missing, terms, stream);
+  if (st) {
+    for (auto &f : fresh) s->pool.put(f);
+    return st;
+  }
+  for (size_t i = 0; i < missing.size(); i++) s->probe_terms[p->probes[missing[i]].key] = fresh[i];
   return JXRHIP_OK;
 }
 
@@ -1336,9 +1401,13 @@ static jxrhip_status device_image(jxrhip_session *s, const jxrhip_plan *p, hipSt
   for (const PlanPlane &pp : p->planes)
     bytes += pp.recs.size() * sizeof(MbRec) + (pp.colmask.size() + pp.rowmask.size()) * 4 + 64 +
              (pp.probe_cols.size() + pp.probe_rows.size()) * 4;
-  if (p->scale != JXRHIP_SCALE_FULL) {
+  if (p->scale != JXRHIP_SCALE_FULL)
     for (const auto &gp : p->group_planes) bytes += gp.size() * sizeof(DevReduce) + 64;
-    for (const ProbeSet &q : p->probes) bytes += (size_t)q.length * 16 + 64;
+  // the sample-scale terms come first: the descriptors point at them
+  std::vector<int64_t *> terms;
+  if (!p->probes.empty()) {
+    const jxrhip_status pst = sample_scale_terms(s, p, stream, terms);
+    if (pst) return pst;
   }
   std::vector<uint8_t> host(bytes, 0);
   DeviceBuffer b;
@@ -1370,15 +1439,9 @@ static jxrhip_status device_image(jxrhip_session *s, const jxrhip_plan *p, hipSt
     d.overlap = pp.overlap;
     d.alpha = pp.alpha ? 1 : 0;
   }
-  // reduced decode: probe terms, then each plane's sample scale
+  // reduced decode: each plane's sample scale
   std::vector<DevReduce> dr(p->planes.size());
-  std::vector<int64_t *> terms(p->probes.size(), nullptr);
   if (p->scale != JXRHIP_SCALE_FULL) {
-    for (size_t k = 0; k < p->probes.size(); k++) {
-      off = (off + 63) & ~size_t(63);
-      terms[k] = (int64_t *)((uint8_t *)b.ptr + off);
-      off += (size_t)p->probes[k].length * 16;
-    }
     for (size_t i = 0; i < p->planes.size(); i++) {
       const PlanPlane &pp = p->planes[i];
       DevReduce &r = dr[i];
@@ -1415,13 +1478,6 @@ static jxrhip_status device_image(jxrhip_session *s, const jxrhip_plan *p, hipSt
   HIP_TRY(hipMemcpyAsync(b.ptr, host.data(), bytes, hipMemcpyHostToDevice, stream), "descriptor upload");
   // pageable copies are staged before the call returns, so `host` may go
   HIP_TRY(hipStreamSynchronize(stream), "descriptor upload");
-  if (!p->probes.empty()) {
-    const jxrhip_status pst = run_probes(s, p, terms, stream);
-    if (pst) {
-      s->pool.put(b);
-      return pst;
-    }
-  }
   if (cacheable) {
     // the eviction above may have removed the stale entry already
     it = s->desc_cache.find(p->id);

@@ -80,6 +80,21 @@ def main():
                     print(f"MISMATCH {name} scale {scale} tile {tile}", flush=True)
                     failures += 1
     session.set_tile(0, 0)
+    # the session keeps sample-scale terms per configuration: a second image
+    # of the same configuration launches no probe decode
+    counts = []
+    for seed in (11, 12):
+        enc, spec, _ = make(dict(CASES["rgb8_420_ov1"], seed=seed, scale=abi.SCALE_QUARTER))
+        plan = HipDecodePlan.from_encoded(enc, spec)
+        before = session.diagnostics().kernel_launches
+        session.upload(plan)
+        counts.append(session.diagnostics().kernel_launches - before)
+        if not np.array_equal(session.decode_to_host(plan), O.decode(image_desc(enc, spec))):
+            print(f"MISMATCH probe-cache seed {seed}", flush=True)
+            failures += 1
+    if counts[1] != 0:
+        print(f"PROBE TERMS recomputed: {counts}", flush=True)
+        failures += 1
     # persistent batch over every case at once, and every third case at a
     # reduced scale besides
     encs, plans, wants = [], [], []

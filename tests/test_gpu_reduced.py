@@ -100,3 +100,23 @@ def test_persistent_batch_mixes_reduced_and_full(session):
     assert not stages.any()
     for label, b, want in zip(labels, bufs, wants):
         assert np.array_equal(b.to_host(0, want.nbytes), want), label
+
+
+def test_sample_scale_terms_are_computed_once_per_configuration():
+    """Two images of one configuration (block shapes, doubling, overlap,
+    scale, probe regions) share the session's probe terms: the second
+    upload launches no probe decode, and both decodes match the oracle."""
+    s = HipDecoderSession(0)
+    try:
+        launches = []
+        for seed in (11, 12):
+            enc, spec, _ = make(dict(CASES["rgb8_420_ov1"], seed=seed, scale=abi.SCALE_QUARTER))
+            plan = HipDecodePlan.from_encoded(enc, spec)
+            before = s.diagnostics().kernel_launches
+            s.upload(plan)
+            launches.append(s.diagnostics().kernel_launches - before)
+            assert np.array_equal(s.decode_to_host(plan), O.decode(image_desc(enc, spec)))
+        # luma and chroma probe sets: eight probe decodes and one fit each
+        assert launches == [18, 0], launches
+    finally:
+        s.close()
