@@ -80,3 +80,29 @@ def test_random_sweep_in_one_batch(seed):
             assert np.array_equal(b.to_host(), want), (i, e.width, e.height, e.color_format, e.overlap, sp)
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("seed", [7, 8])
+def test_random_reduced_sweep_in_one_batch(seed):
+    """The same random images decoded at a quarter or a sixteenth, mixed
+    with full-resolution ones, in one persistent batch."""
+    rng = np.random.default_rng(seed)
+    items = []
+    for _ in range(120):
+        enc, spec = random_case(rng)
+        spec.scale = int(rng.choice([abi.SCALE_FULL, abi.SCALE_QUARTER, abi.SCALE_SIXTEENTH]))
+        items.append((enc, spec))
+    s = HipDecoderSession(0)
+    try:
+        plans = [HipDecodePlan.from_encoded(e, sp) for e, sp in items]
+        bufs = [s.allocate_destination(p.output_bytes) for p in plans]
+        for b in bufs:
+            b.fill(0xA5)
+        stages = s.submit_batch(plans, [b.ptr for b in bufs], abi.BATCH_PERSISTENT).wait()
+        assert not np.any(stages)
+        wants = O.decode_many([image_desc(e, sp) for e, sp in items], 16)
+        for i, (b, want) in enumerate(zip(bufs, wants)):
+            e, sp = items[i]
+            assert np.array_equal(b.to_host(), want), (i, e.width, e.height, e.color_format, e.overlap, sp)
+    finally:
+        s.close()
