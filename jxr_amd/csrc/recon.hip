@@ -1670,6 +1670,188 @@ __device__ __forceinline__ void recipe_tile(const OutCtx &o, JXR_GLOBAL uint8_t 
   }
 }
 
+// ---- float output recipe ----------------------------------------------------
+// One BD32F sample format (scaling.rs:254-296): the rounding shift of a
+// scaled sample, the mantissa length and k = 127 - exponent bias.
+struct FloatFmt {
+  uint32_t scaled, length;
+  int k;
+};
+
+// f32_bits in 32-bit arithmetic for |exponent bias| <= 2^23: an exponent
+// field of 2^24 or more overflows the f32 exponent at any such bias, so it is
+// clamped there; the result is f32_bits' bit for bit (mantissas then always
+// fit the f32 field). The biased exponent e = field + k is at most
+// 2^24 + 2^23 + 127, and 1 + k for a coded subnormal.
+__device__ __forceinline__ uint32_t f32_recipe_bits(int v, const FloatFmt &f) {
+  const int q = f.scaled ? (v + 3) >> 3 : v;
+  const uint32_t sign = q < 0 ? 0x80000000u : 0u;
+  const uint32_t mag = q < 0 ? 0u - (uint32_t)q : (uint32_t)q;
+  const uint32_t implicit = 1u << f.length;
+  const uint32_t field = mag >> f.length;
+  uint32_t m = mag & (implicit - 1u);
+  int e;
+  if (field != 0) {
+    e = (int)min(field, 1u << 24) + f.k;
+  } else {
+    // a coded subnormal: normalised as far as the exponent allows
+    e = 1 + f.k;
+    if (m != 0 && e > 1) {
+      const int need = (int)f.length - (31 - __builtin_clz(m));
+      const int steps = min(need, e - 1);
+      m <<= steps;
+      e -= steps;
+    }
+    if (m < implicit)
+      e = 0;
+    else
+      m ^= implicit;
+  }
+  if (e < 0 || e > 255) return sign | 0x7f7fffffu;
+  return sign | ((uint32_t)e << 23) | (m << (23 - f.length));
+}
+
+// What the float recipe reads of the image for every group of a tile, read
+// once per tile: output plane rows and strides (plane 0 when interleaved), the
+// luma and alpha sample regions and the two sample formats.
+struct F32Ctx {
+  JXR_GLOBAL uint8_t *row0[4];
+  uint64_t stride[4];
+  const int *luma;   // sample (0, 0) of the tile's coordinates: luma + y * lw + x
+  const int *alpha;  // likewise for the alpha plane
+  int lw, aw;
+  int crop_x, crop_y;
+  FloatFmt color, alpha_fmt;
+  uint32_t conv, channels;
+  bool has_alpha, planar, reverse, upsample;
+};
+
+// YUV -> RGB/BGR (output_format/color.rs:100-110), luma -> RGB or luma ->
+// luma, then alpha, each channel formatted as a BD32F sample; stored planar
+// or interleaved.
+__device__ __forceinline__ void f32_group(const OutCtx &o, const F32Ctx &c4, int gx, int gy, int x0, int x1, int y0,
+                                          int y1) {
+  const int xb = 4 * gx, yb = 2 * gy;
+  int chroma[2][2][4];
+  if (c4.upsample) group_chroma<Fast>(o, gx, gy, chroma);
+  uint32_t valid = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) valid |= (xb + i >= x0 && xb + i < x1) ? 1u << i : 0u;
+  const int ox0 = xb - c4.crop_x;
+#pragma unroll 1
+  for (int j = 0; j < 2; j++) {
+    const int y = yb + j;
+    if (y < y0 || y >= y1) continue;
+    const v4i l = *reinterpret_cast<const v4i *>(c4.luma + y * c4.lw + xb);
+    uint32_t c[4][4];
+    if (c4.conv == kConvYuvRgb) {
+      v4i u, v;
+      if (c4.upsample) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          u[i] = j ? chroma[0][1][i] : chroma[0][0][i];
+          v[i] = j ? chroma[1][1][i] : chroma[1][0][i];
+        }
+      } else {
+        u = sample_quad(o, 1, xb, y);
+        v = sample_quad(o, 2, xb, y);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int t0 = -u[i];
+        const int g = l[i] - (t0 >> 1);
+        const int r = t0 + g - ((v[i] >> 1) + (v[i] & 1));
+        const int b = v[i] + r;
+        c[0][i] = f32_recipe_bits(c4.reverse ? b : r, c4.color);
+        c[1][i] = f32_recipe_bits(g, c4.color);
+        c[2][i] = f32_recipe_bits(c4.reverse ? r : b, c4.color);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; i++) c[0][i] = c[1][i] = c[2][i] = f32_recipe_bits(l[i], c4.color);
+    }
+    const uint32_t pa = c4.conv == kConvLuma ? 1u : 3u;  // alpha's channel
+    if (c4.has_alpha) {
+      const v4i al = *reinterpret_cast<const v4i *>(c4.alpha + y * c4.aw + xb);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const uint32_t a = f32_recipe_bits(al[i], c4.alpha_fmt);
+        if (pa == 1)
+          c[1][i] = a;
+        else
+          c[3][i] = a;
+      }
+    }
+    const uint64_t oy = (uint32_t)(y - c4.crop_y);
+    if (c4.planar) {
+#pragma unroll
+      for (uint32_t k = 0; k < 4; k++)
+        if (k < c4.channels) store_run<4>(c4.row0[k] + oy * c4.stride[k] + (int64_t)ox0 * 4, c[k], valid);
+    } else {
+      JXR_GLOBAL uint8_t *px0 = c4.row0[0] + oy * c4.stride[0] + (int64_t)ox0 * 4 * c4.channels;
+      if (valid == 0xFu && c4.channels == 4) {
+        uint32_t w[16];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int k = 0; k < 4; k++) w[4 * i + k] = c[k][i];
+        store_dwords<16>(px0, w);
+      } else if (valid == 0xFu && c4.channels == 3) {
+        uint32_t w[12];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int k = 0; k < 3; k++) w[3 * i + k] = c[k][i];
+        store_dwords<12>(px0, w);
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          if (!(valid & (1u << i))) continue;
+#pragma unroll
+          for (uint32_t k = 0; k < 4; k++)
+            if (k < c4.channels) store_sample<4>(px0 + (i * c4.channels + k) * 4, c[k][i]);
+        }
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void f32_tile(const OutCtx &o, JXR_GLOBAL uint8_t *out, int gx0, int gy0, int ngx, int ngy,
+                                         int x0, int x1, int y0, int y1) {
+  const DevImage &im = *o.im;
+  F32Ctx c4{};
+  for (int k = 0; k < 4; k++) {
+    c4.row0[k] = out + im.plane_offset[im.planar ? k : 0];
+    c4.stride[k] = im.row_stride[im.planar ? k : 0];
+  }
+  const PlaneGeo &g = o.G[0];
+  c4.lw = (int)g.srw;
+  c4.luma = o.lds + o.soff[0] - g.soy * c4.lw - g.sox;
+  c4.has_alpha = im.has_alpha != 0;
+  if (c4.has_alpha) {
+    const PlaneGeo &ga = o.G[im.alpha_plane];
+    c4.aw = (int)ga.srw;
+    c4.alpha = o.lds + o.soff[im.alpha_plane] - ga.soy * c4.aw - ga.sox;
+  } else {
+    c4.aw = 0;
+    c4.alpha = c4.luma;
+  }
+  c4.crop_x = (int)im.crop_x;
+  c4.crop_y = (int)im.crop_y;
+  c4.color = FloatFmt{im.f_scaled, im.f_mant, 127 - im.f_bias};
+  c4.alpha_fmt = FloatFmt{im.f_alpha_scaled, im.f_alpha_mant, 127 - im.f_alpha_bias};
+  c4.conv = im.conv;
+  c4.channels = im.channels;
+  c4.planar = im.planar != 0;
+  c4.reverse = im.reverse != 0;
+  c4.upsample = im.upsample != 0;
+  const uint32_t m = div_magic((uint32_t)ngx);
+  for (int t = threadIdx.x; t < ngx * ngy; t += blockDim.x) {
+    const DivMod u = divmod(t, (uint32_t)ngx, m);
+    f32_group(o, c4, gx0 + u.r, gy0 + u.q, x0, x1, y0, y1);
+  }
+}
+
 // ---- one tile, every plane, then output -----------------------------------
 // Everything a tile indexes by plane lives in LDS: a copy of the image
 // descriptor, the per-plane region geometry and the per-stage work counts.
@@ -2242,7 +2424,9 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
       return;
     }
   }
-  if (integer && fast && im.recipe == kRecipeRgb3)
+  if (STORE == JXRHIP_STORE_F32 && im.recipe == kRecipeF32 && ts.out_mag < kOutputSampleBound)
+    f32_tile(o, out, gx0, gy0, ngx, ngy, x0, x1, y0, y1);  // its only arithmetic: a scaled sample's + 3
+  else if (integer && fast && im.recipe == kRecipeRgb3)
     recipe_tile<STORE, kRecipeRgb3>(o, out, gx0, gy0, ngx, ngy, x0, x1, y0, y1);
   else if (integer && fast && im.recipe == kRecipeLuma1)
     recipe_tile<STORE, kRecipeLuma1>(o, out, gx0, gy0, ngx, ngy, x0, x1, y0, y1);

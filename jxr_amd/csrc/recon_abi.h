@@ -70,12 +70,16 @@ struct ChanScale {
 };
 
 // Output paths specialised for common requests; applied to tiles whose
-// output arithmetic cannot overflow (every channel shares one scale with no
-// post-shift, no alpha, integer storage).
+// output arithmetic cannot overflow (integer recipes: every channel shares
+// one scale with no post-shift, no alpha; the float recipe: one sample
+// format for the colour channels and one for alpha).
 enum OutRecipe : uint32_t {
   kRecipeGeneric = 0,
   kRecipeRgb3 = 1,   // YUV -> RGB or BGR, three stored channels
   kRecipeLuma1 = 2,  // Luma -> Luma, one stored channel
+  // F32 storage: YUV -> RGB/BGR, luma -> RGB or luma -> luma, optionally
+  // alpha last, no padding channel, no premultiply, |exponent bias| <= 2^23
+  kRecipeF32 = 3,
 };
 
 // What the output stage stores (output_format/{packing,packed_color,planar}.rs).

@@ -661,6 +661,20 @@ extern "C" jxrhip_status jxrhip_plan_create(const jxrhip_image_desc *d, jxrhip_p
       if (dv.conv == kConvYuvRgb && dv.channels == 3) dv.recipe = kRecipeRgb3;
       else if (dv.conv == kConvLuma && dv.channels == 1) dv.recipe = kRecipeLuma1;
     }
+    if (dv.out_kind == kOutTyped && f.storage == JXRHIP_STORE_F32 && !f.premultiply_alpha) {
+      // every stored channel a BD32F sample, colour first, alpha last (these
+      // conversions have at most three colour planes: one plane group)
+      const uint32_t primaries = dv.conv == kConvLuma ? 1u : 3u;
+      bool ok = dv.conv == kConvYuvRgb || dv.conv == kConvLumaRgb || dv.conv == kConvLuma;
+      ok &= dv.primary_count == primaries && dv.channels == primaries + dv.has_alpha;
+      for (uint32_t c = 0; c < primaries && c < 8; c++) ok &= (dv.color_scale[c].flags & 2u) == 0;
+      if (dv.has_alpha) ok &= (dv.alpha_scale.flags & 2u) == 0;
+      const int64_t limit = 1ll << 23;
+      const int64_t bias = f.exponent_bias;
+      const int64_t abias = f.has_alpha_format ? f.alpha_exponent_bias : f.exponent_bias;
+      ok &= bias >= -limit && bias <= limit && abias >= -limit && abias <= limit;
+      if (ok) dv.recipe = kRecipeF32;
+    }
     dv.recipe_add = c0.shifted_bias + c0.rounding;  // both below 2^29 when out_fast
     dv.recipe_shift = c0.shift;
     switch (f.storage) {

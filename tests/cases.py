@@ -45,6 +45,12 @@ def make(case):
         px = synth.gradient_noise(w, h, seed, channels=1, depth=8)
     elif kind == "rgbaf":
         px = synth.hdr_rgba(w, h, seed)
+    elif kind == "rgbf":
+        px = np.ascontiguousarray(synth.hdr_rgba(w, h, seed)[..., :3] - np.float32(1.5))  # signed
+    elif kind == "grayaf":
+        px = np.ascontiguousarray(synth.hdr_rgba(w, h, seed)[..., 2:])
+    elif kind == "grayf":
+        px = np.ascontiguousarray(synth.hdr_rgba(w, h, seed)[..., 0])
     elif kind == "rgba8":
         px = synth.gradient_noise(w, h, seed, channels=4, depth=8)
     elif kind == "graya8":
@@ -106,6 +112,27 @@ CASES = {
     "rgbaf_mant16_bias6_lossy": dict(w=80, h=48, ov=2, image="rgbaf", alpha=abi.ALPHA_INTEGRATED, quality=90,
                                      mantissa=16, bias=6,
                                      out=dict(RGBAF, planar=False, mantissa_length=16, exponent_bias=6)),
+    # the f32 conversion's edge paths: a coded subnormal normalised only as
+    # far as the exponent allows (bias 126), exponents below the f32 range
+    # (bias 200) and above it (bias -130), both clipped to the largest float;
+    # |bias| above 2^23 is decoded by the general output path
+    "rgbf_bgr_lossy": dict(w=80, h=48, ov=2, image="rgbf", quality=85,
+                           out=dict(RGBAF, layout=abi.LAYOUT_BGR)),
+    "rgbaf_bias126_planar": dict(w=64, h=48, ov=1, image="rgbaf", alpha=abi.ALPHA_SEPARATE,
+                                 out=dict(RGBAF, planar=True, exponent_bias=126)),
+    "rgbaf_bias200_lossy": dict(w=64, h=48, ov=2, image="rgbaf", alpha=abi.ALPHA_INTEGRATED, quality=80,
+                                out=dict(RGBAF, exponent_bias=200)),
+    "rgbaf_bias_m130": dict(w=48, h=48, ov=1, image="rgbaf", alpha=abi.ALPHA_SEPARATE,
+                            out=dict(RGBAF, planar=True, exponent_bias=-130)),
+    "rgbaf_bias_2p23": dict(w=48, h=32, ov=1, image="rgbaf", alpha=abi.ALPHA_SEPARATE,
+                            out=dict(RGBAF, exponent_bias=1 << 23)),
+    "rgbaf_bias_over_2p23": dict(w=48, h=32, ov=1, image="rgbaf", alpha=abi.ALPHA_SEPARATE,
+                                 out=dict(RGBAF, exponent_bias=(1 << 23) + 1)),
+    "grayaf_luma_alpha": dict(w=64, h=48, ov=1, image="grayaf", color=abi.COLOR_LUMA, alpha=abi.ALPHA_SEPARATE,
+                              quality=90,
+                              out=dict(RGBAF, layout=abi.LAYOUT_LUMA_ALPHA, output_color=abi.COLOR_LUMA)),
+    "grayf_to_rgb_planar": dict(w=64, h=32, ov=2, image="grayf", color=abi.COLOR_LUMA,
+                                out=dict(RGBAF, layout=abi.LAYOUT_RGB, planar=True)),
     "rgba8_bgra_premultiply": dict(w=80, h=64, ov=1, image="rgba8", alpha=abi.ALPHA_INTEGRATED, quality=80,
                                    out=dict(storage=abi.STORE_U8, depth=abi.DEPTH_U8, layout=abi.LAYOUT_BGRA,
                                             output_color=abi.COLOR_RGB, premultiply=True)),
