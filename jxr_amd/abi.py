@@ -1,4 +1,4 @@
-"""ctypes view of include/jxr_hip.h and include/jxr_hip_encode.h.
+"""ctypes view of include/jxr_hip.h, jxr_hip_encode.h and jxr_hip_codestream.h.
 
 The structures here mirror the C declarations field for field; a CPU test
 compiles the header with gcc and checks every size and offset against them.
@@ -21,6 +21,10 @@ ERR_KERNEL_ARITHMETIC = 5
 ERR_INVALID_ARGUMENT = 6
 ERR_BUFFER_TOO_SMALL = 7
 ERR_BUDGET = 8
+ERR_SYNTAX = 9
+ERR_UNSUPPORTED = 10
+ERR_OVERFLOW = 11
+TRIM_NONE = 0xFFFFFFFF
 
 STAGE_FIRST_TRANSFORM = 1
 STAGE_OVERLAP = 2
@@ -296,6 +300,30 @@ class EncodeParams(C.Structure):
     ]
 
 
+class CodestreamInfo(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "width", "height", "margin_top", "margin_left", "margin_bottom", "margin_right",
+        "macroblocks_x", "macroblocks_y", "tile_column_count", "tile_row_count", "overlap", "hard_tiles",
+        "frequency_mode", "orientation", "output_color", "bit_depth", "color_format", "component_count",
+        "bands", "scaled", "chroma_centering_x", "chroma_centering_y", "shift_bits", "mantissa_length")] + [
+        ("exponent_bias", C.c_int32)] + [(n, C.c_uint32) for n in (
+            "alpha_mode", "alpha_bands", "alpha_scaled", "alpha_shift_bits", "alpha_mantissa_length")] + [
+        ("alpha_exponent_bias", C.c_int32)] + [(n, C.c_uint32) for n in (
+            "alpha_bit_depth", "red_blue_not_swapped", "premultiplied_alpha", "short_header", "long_word",
+            "trim_flexbits_present", "index_table_present", "profile_idc", "level_idc", "annex_a")] + [
+        ("pixel_format_guid", C.c_uint8 * 16),
+        ("codestream_offset", C.c_uint64),
+        ("codestream_bytes", C.c_uint64),
+        ("alpha_offset", C.c_uint64),
+        ("alpha_bytes", C.c_uint64),
+    ]
+
+
+class CodestreamOptions(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "frequency_mode", "trim_flexbits", "annex_a", "short_header", "profile_idc", "level_idc")]
+
+
 vp = C.c_void_p
 pp = C.POINTER(C.c_void_p)
 st = C.c_int32
@@ -376,6 +404,17 @@ SIGNATURES = {
                                               C.c_uint32, u32p, u32p, u32p]),
     "jxrhip_sample_scale": (st, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32, u32p, u32p,
                                  u32p]),
+    # jxr_hip_codestream.h
+    "jxrhip_codestream_parse": (st, [vp, C.c_uint64, pp]),
+    "jxrhip_codestream_destroy": (None, [vp]),
+    "jxrhip_codestream_get_info": (st, [vp, C.POINTER(CodestreamInfo)]),
+    "jxrhip_codestream_tile_grid": (st, [vp, u32p, u32p]),
+    "jxrhip_codestream_decode": (st, [vp, C.c_uint32, vp, C.c_uint32, pp]),
+    "jxrhip_decoded_destroy": (None, [vp]),
+    "jxrhip_decoded_arena": (C.POINTER(Arena), [vp, C.c_uint32]),
+    "jxrhip_encoded_codestream": (st, [vp, C.POINTER(CodestreamOptions), C.POINTER(C.POINTER(C.c_uint8)),
+                                       C.POINTER(C.c_uint64)]),
+    "jxrhip_free_bytes": (None, [vp]),
 }
 
 _lib = None

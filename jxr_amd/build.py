@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libjxr_hip.so")
-SOURCES = ["recon.hip", "runtime.cpp", "analysis.cpp"]
+SOURCES = ["recon.hip", "runtime.cpp", "analysis.cpp", "codestream.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 

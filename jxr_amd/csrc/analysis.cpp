@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/jxr_hip_encode.h"
+#include "encoded.h"
 
 extern "C" void jxrhip_set_error(const char *message);
 
@@ -347,14 +348,7 @@ struct PlaneIn {
   std::vector<i32> s;
 };
 
-struct ArenaOut {
-  std::vector<i32> coefficients;
-  std::vector<uint32_t> offsets, quantizers, coded_x, coded_y;
-  std::vector<uint8_t> bands, preds;
-  std::vector<jxrhip_coefficient_plane> planes;
-  std::vector<uint32_t> tile_w, tile_h;
-  jxrhip_arena arena{};
-};
+using jxr::ArenaOut;
 
 std::atomic<uint64_t> g_identity{1};
 
@@ -377,10 +371,20 @@ void downsample_line(const i32 *in, size_t stride_in, uint32_t length, i32 *out,
   }
 }
 
-void analyze_plane(const PlaneIn &p, uint32_t component, const jxrhip_encode_params &prm,
-                   bool scale_after_first, uint32_t qp_dc, uint32_t qp_lp, uint32_t qp_hp,
-                   const std::vector<uint32_t> &tcol, const std::vector<uint32_t> &trow,
-                   ArenaOut &out) {
+// One component's quantized macroblocks in arena (tile-major) order: block k
+// of a macroblock holds its quantized DC/low-pass value at 16k and its
+// quantized high-pass coefficients at 16k+1..16k+15.
+struct PlaneQ {
+  uint32_t bc = 4, br = 4;
+  std::vector<i32> q;          // [macroblock][blocks * 16]
+  std::vector<uint32_t> steps; // [macroblock][3]
+  std::vector<uint8_t> qps;    // [macroblock][3]
+  std::vector<uint32_t> cx, cy;
+};
+
+PlaneQ quantize_plane(const PlaneIn &p, uint32_t component, const jxrhip_encode_params &prm,
+                      bool scale_after_first, uint32_t qp_dc, uint32_t qp_lp, uint32_t qp_hp,
+                      const std::vector<uint32_t> &tcol, const std::vector<uint32_t> &trow) {
   const uint32_t bc = p.bc, br = p.br, blocks = bc * br;
   const uint32_t mbx = p.w / (bc * 4), mby = p.h / (br * 4);
   std::vector<i32> s = p.s;
@@ -426,24 +430,37 @@ void analyze_plane(const PlaneIn &p, uint32_t component, const jxrhip_encode_par
   std::vector<uint32_t> tx{0}, ty{0};
   for (uint32_t w : tcol) tx.push_back(tx.back() + w);
   for (uint32_t h : trow) ty.push_back(ty.back() + h);
-  jxrhip_coefficient_plane cp{};
-  cp.coefficient_offset = out.coefficients.size();
-  cp.macroblock_offset = out.offsets.size();
-  cp.block_columns = bc;
-  cp.block_rows = br;
+  PlaneQ out;
+  out.bc = bc;
+  out.br = br;
+  const size_t mbs = (size_t)mbx * mby;
+  out.q.resize(mbs * blocks * 16);
+  out.steps.reserve(mbs * 3);
+  out.qps.reserve(mbs * 3);
+  out.cx.reserve(mbs);
+  out.cy.reserve(mbs);
+  size_t m = 0;
   for (size_t tj = 0; tj + 1 < ty.size(); tj++)
     for (size_t ti = 0; ti + 1 < tx.size(); ti++) {
       uint32_t tile_vary = prm.vary_quantizers ? mix(prm.seed * 31u + (uint32_t)(tj * 977 + ti)) % 5 : 0;
       for (uint32_t my = ty[tj]; my < ty[tj + 1]; my++)
-        for (uint32_t mx = tx[ti]; mx < tx[ti + 1]; mx++) {
+        for (uint32_t mx = tx[ti]; mx < tx[ti + 1]; mx++, m++) {
           uint32_t mb_vary =
               prm.vary_quantizers ? mix(prm.seed * 7919u + my * 65599u + mx) % 3 : 0;
           uint32_t qd = qp_dc ? qp_dc + tile_vary : 0;
           uint32_t ql = qp_lp ? qp_lp + tile_vary + mb_vary : 0;
           uint32_t qh = qp_hp ? qp_hp + tile_vary + 2 * mb_vary : 0;
-          uint32_t sdc = quant_map(qd > 255 ? 255 : qd, scaled, lb_shift);
-          uint32_t slp = quant_map(ql > 255 ? 255 : ql, scaled, lb_shift);
-          uint32_t shp = quant_map(qh > 255 ? 255 : qh, scaled, 1);
+          qd = qd > 255 ? 255 : qd;
+          ql = ql > 255 ? 255 : ql;
+          qh = qh > 255 ? 255 : qh;
+          // a band that is not coded has no parameters of its own: it takes
+          // those of the band below, as a decoder does
+          // (tile_decode/quantizer.rs parse_low_pass / parse_high_pass)
+          if (prm.bands == JXRHIP_BANDS_DC_ONLY) ql = qd;
+          if (prm.bands <= JXRHIP_BANDS_NO_HIGH_PASS) qh = ql;
+          uint32_t sdc = quant_map(qd, scaled, lb_shift);
+          uint32_t slp = quant_map(ql, scaled, lb_shift);
+          uint32_t shp = quant_map(qh, scaled, 1);
           i32 lp[16] = {0};
           for (uint32_t r = 0; r < br; r++)
             for (uint32_t c = 0; c < bc; c++) {
@@ -456,8 +473,7 @@ void analyze_plane(const PlaneIn &p, uint32_t component, const jxrhip_encode_par
             forward_chroma422(lp);
           else
             forward_chroma420(lp);
-          i32 q[256];
-          std::memset(q, 0, sizeof q);
+          i32 *q = &out.q[m * blocks * 16];
           for (uint32_t k = 0; k < blocks; k++) {
             uint32_t bx = mx * bc + k % bc, by = my * br + k / bc;
             const i32 *h = &high[((size_t)by * lw + bx) * 16];
@@ -471,54 +487,104 @@ void analyze_plane(const PlaneIn &p, uint32_t component, const jxrhip_encode_par
             // (analyze.rs:532-536)
             for (int j = 1; j < 16; j++) q[k * 16 + j] = quantize(h[j], shp, shp / 8 * 3);
           }
-          uint32_t bands = prm.bands;
-          // High-pass direction from the low-pass coefficients' orientation.
-          i64 hor = 0, ver = 0;
-          if (bc == 4) {
-            hor = std::abs((i64)q[16]) + std::abs((i64)q[32]) + std::abs((i64)q[48]);
-            ver = std::abs((i64)q[64]) + std::abs((i64)q[128]) + std::abs((i64)q[192]);
-          } else {
-            hor = std::abs((i64)q[16]);
-            ver = std::abs((i64)q[bc * 16]);
-          }
-          uint8_t pred = JXRHIP_PRED_NONE;
-          if (bands >= JXRHIP_BANDS_NO_FLEXBITS) {
-            if (ver * 4 < hor)
-              pred = JXRHIP_PRED_FROM_TOP;
-            else if (hor * 4 < ver)
-              pred = JXRHIP_PRED_FROM_LEFT;
-          }
-          // Residuals: the forward of the decoder's running sums.
-          i32 res[256];
-          std::memcpy(res, q, sizeof q);
-          if (pred == JXRHIP_PRED_FROM_LEFT) {
-            for (uint32_t r = 0; r < br; r++)
-              for (uint32_t c = 1; c < bc; c++)
-                for (int pos : {4, 8, 12})
-                  res[(r * bc + c) * 16 + pos] = q[(r * bc + c) * 16 + pos] - q[(r * bc + c - 1) * 16 + pos];
-          } else if (pred == JXRHIP_PRED_FROM_TOP) {
-            for (uint32_t r = 1; r < br; r++)
-              for (uint32_t c = 0; c < bc; c++)
-                for (int pos : {1, 2, 3})
-                  res[(r * bc + c) * 16 + pos] = q[(r * bc + c) * 16 + pos] - q[((r - 1) * bc + c) * 16 + pos];
-          }
-          out.offsets.push_back((uint32_t)out.coefficients.size());
-          out.quantizers.push_back(sdc);
-          out.quantizers.push_back(slp);
-          out.quantizers.push_back(shp);
-          out.bands.push_back((uint8_t)bands);
-          out.preds.push_back(pred);
-          out.coded_x.push_back(mx);
-          out.coded_y.push_back(my);
-          if (bands == JXRHIP_BANDS_DC_ONLY) {
-            out.coefficients.push_back(res[0]);
-          } else if (bands == JXRHIP_BANDS_NO_HIGH_PASS) {
-            for (uint32_t k = 0; k < blocks; k++) out.coefficients.push_back(res[k * 16]);
-          } else {
-            out.coefficients.insert(out.coefficients.end(), res, res + blocks * 16);
-          }
+          out.steps.insert(out.steps.end(), {sdc, slp, shp});
+          out.qps.insert(out.qps.end(), {(uint8_t)qd, (uint8_t)ql, (uint8_t)qh});
+          out.cx.push_back(mx);
+          out.cy.push_back(my);
         }
     }
+  return out;
+}
+
+inline uint32_t sat_add(uint32_t a, uint32_t b) { return a > UINT32_MAX - b ? UINT32_MAX : a + b; }
+inline uint32_t sat_times4(uint32_t a) { return a > UINT32_MAX / 4 ? UINT32_MAX : a * 4; }
+inline uint32_t magnitude(i32 v) { return v < 0 ? 0u - (uint32_t)v : (uint32_t)v; }
+
+// The high-pass direction of every macroblock of one coded plane, from its
+// quantized low-pass coefficients, as a decoder derives it
+// (ComponentSyntax::high_pass_mode, tile_decode/syntax.rs:173-205): the
+// luma's energy across (positions 1, 2, 3) and down (4, 8, 12), plus, for a
+// YUV plane, its two chroma components' at their sampling's positions; four
+// times more energy one way picks the other direction. Every component of
+// the plane shares it; a plane without a high-pass band has none.
+std::vector<uint8_t> hp_directions(const std::vector<const PlaneQ *> &planes, uint32_t color, uint32_t bands) {
+  const size_t mbs = planes[0]->cx.size();
+  std::vector<uint8_t> dirs(mbs, JXRHIP_PRED_NONE);
+  if (bands < JXRHIP_BANDS_NO_FLEXBITS) return dirs;
+  const bool yuv = color == JXRHIP_COLOR_YUV420 || color == JXRHIP_COLOR_YUV422 || color == JXRHIP_COLOR_YUV444;
+  std::vector<int> across, down;
+  if (color == JXRHIP_COLOR_YUV420) {
+    across = {1};
+    down = {2};
+  } else if (color == JXRHIP_COLOR_YUV422) {
+    across = {1, 5};
+    down = {2, 6};
+  } else {
+    across = {1};
+    down = {4};
+  }
+  for (size_t m = 0; m < mbs; m++) {
+    auto lp = [&](size_t c, int position) {
+      const PlaneQ &pq = *planes[c];
+      return magnitude(pq.q[(m * pq.bc * pq.br + position) * 16]);
+    };
+    uint32_t hor = 0, ver = 0;
+    for (int position : {1, 2, 3}) hor = sat_add(hor, lp(0, position));
+    for (int position : {4, 8, 12}) ver = sat_add(ver, lp(0, position));
+    if (yuv && planes.size() >= 3)
+      for (size_t c = 1; c < 3; c++) {
+        for (int position : across) hor = sat_add(hor, lp(c, position));
+        for (int position : down) ver = sat_add(ver, lp(c, position));
+      }
+    if (sat_times4(hor) < ver)
+      dirs[m] = JXRHIP_PRED_FROM_LEFT;
+    else if (sat_times4(ver) < hor)
+      dirs[m] = JXRHIP_PRED_FROM_TOP;
+  }
+  return dirs;
+}
+
+// The arena of one component: its macroblocks' DC/low-pass values and their
+// high-pass residuals after the shared direction's prediction is removed
+// (the forward of the decoder's running sums, prediction.rs:53-97).
+void emit_plane(const PlaneQ &pq, const std::vector<uint8_t> &dirs, uint32_t bands, ArenaOut &out) {
+  const uint32_t bc = pq.bc, br = pq.br, blocks = bc * br;
+  jxrhip_coefficient_plane cp{};
+  cp.coefficient_offset = out.coefficients.size();
+  cp.macroblock_offset = out.offsets.size();
+  cp.block_columns = bc;
+  cp.block_rows = br;
+  for (size_t m = 0; m < pq.cx.size(); m++) {
+    const i32 *q = &pq.q[m * blocks * 16];
+    const uint8_t pred = dirs[m];
+    i32 res[256];
+    std::memcpy(res, q, sizeof(i32) * blocks * 16);
+    if (pred == JXRHIP_PRED_FROM_LEFT) {
+      for (uint32_t r = 0; r < br; r++)
+        for (uint32_t c = 1; c < bc; c++)
+          for (int pos : {4, 8, 12})
+            res[(r * bc + c) * 16 + pos] = q[(r * bc + c) * 16 + pos] - q[(r * bc + c - 1) * 16 + pos];
+    } else if (pred == JXRHIP_PRED_FROM_TOP) {
+      for (uint32_t r = 1; r < br; r++)
+        for (uint32_t c = 0; c < bc; c++)
+          for (int pos : {1, 2, 3})
+            res[(r * bc + c) * 16 + pos] = q[(r * bc + c) * 16 + pos] - q[((r - 1) * bc + c) * 16 + pos];
+    }
+    out.offsets.push_back((uint32_t)out.coefficients.size());
+    out.quantizers.insert(out.quantizers.end(), &pq.steps[m * 3], &pq.steps[m * 3] + 3);
+    out.qps.insert(out.qps.end(), &pq.qps[m * 3], &pq.qps[m * 3] + 3);
+    out.bands.push_back((uint8_t)bands);
+    out.preds.push_back(pred);
+    out.coded_x.push_back(pq.cx[m]);
+    out.coded_y.push_back(pq.cy[m]);
+    if (bands == JXRHIP_BANDS_DC_ONLY) {
+      out.coefficients.push_back(res[0]);
+    } else if (bands == JXRHIP_BANDS_NO_HIGH_PASS) {
+      for (uint32_t k = 0; k < blocks; k++) out.coefficients.push_back(res[k * 16]);
+    } else {
+      out.coefficients.insert(out.coefficients.end(), res, res + blocks * 16);
+    }
+  }
   cp.coefficient_count = out.coefficients.size() - cp.coefficient_offset;
   cp.macroblock_count = out.offsets.size() - cp.macroblock_offset;
   out.planes.push_back(cp);
@@ -617,11 +683,6 @@ void flat_probe_coefficients(int32_t level, uint32_t bc, uint32_t br, bool doubl
 
 }  // namespace jxr
 
-struct jxrhip_encoded {
-  ArenaOut primary;
-  ArenaOut alpha;
-  bool has_alpha = false;
-};
 
 extern "C" uint32_t jxrhip_quant_step(uint32_t qp, uint32_t scaled, uint32_t scaled_shift) {
   return quant_map(qp > 255 ? 255 : qp, scaled != 0, scaled_shift);
@@ -1019,24 +1080,41 @@ extern "C" jxrhip_status jxrhip_encode(const void *pixels, uint64_t pixel_bytes,
   ArenaOut &P = enc->primary;
   P.tile_w = tcol;
   P.tile_h = trow;
+  std::vector<PlaneQ> quantized;
   for (uint32_t c = 0; c < comps.size(); c++) {
     const bool chroma = c != 0 && prm->chroma_qp_set;
-    analyze_plane(comps[c], c, *prm, prm->scaled && c != 0, chroma ? prm->qp_chroma_dc : prm->qp_dc,
-                  chroma ? prm->qp_chroma_lp : prm->qp_lp, chroma ? prm->qp_chroma_hp : prm->qp_hp, tcol, trow, P);
+    quantized.push_back(quantize_plane(comps[c], c, *prm, prm->scaled && c != 0,
+                                       chroma ? prm->qp_chroma_dc : prm->qp_dc,
+                                       chroma ? prm->qp_chroma_lp : prm->qp_lp,
+                                       chroma ? prm->qp_chroma_hp : prm->qp_hp, tcol, trow));
   }
-  if (has_alpha && prm->alpha_mode == JXRHIP_ALPHA_INTEGRATED)
-    analyze_plane(alpha_plane, 0, *prm, false, prm->alpha_qp, prm->alpha_qp, prm->alpha_qp, tcol,
-                  trow, P);
+  {
+    std::vector<const PlaneQ *> group;
+    for (const PlaneQ &pq : quantized) group.push_back(&pq);
+    const std::vector<uint8_t> dirs = hp_directions(group, cf, prm->bands);
+    for (const PlaneQ &pq : quantized) emit_plane(pq, dirs, prm->bands, P);
+  }
+  PlaneQ alpha_q;
+  if (has_alpha) {
+    alpha_q = quantize_plane(alpha_plane, 0, *prm, false, prm->alpha_qp, prm->alpha_qp, prm->alpha_qp, tcol, trow);
+  }
+  const std::vector<uint8_t> alpha_dirs =
+      has_alpha ? hp_directions({&alpha_q}, JXRHIP_COLOR_LUMA, prm->bands) : std::vector<uint8_t>();
+  if (has_alpha && prm->alpha_mode == JXRHIP_ALPHA_INTEGRATED) emit_plane(alpha_q, alpha_dirs, prm->bands, P);
   finish_arena(P, *prm);
   if (has_alpha && prm->alpha_mode == JXRHIP_ALPHA_SEPARATE) {
     ArenaOut &A = enc->alpha;
     A.tile_w = tcol;
     A.tile_h = trow;
-    analyze_plane(alpha_plane, 0, *prm, false, prm->alpha_qp, prm->alpha_qp, prm->alpha_qp, tcol,
-                  trow, A);
+    emit_plane(alpha_q, alpha_dirs, prm->bands, A);
     finish_arena(A, *prm);
     enc->has_alpha = true;
   }
+  enc->params = *prm;
+  enc->params.tile_column_widths = nullptr;
+  enc->params.tile_row_heights = nullptr;
+  enc->params.tile_column_count = (uint32_t)tcol.size();
+  enc->params.tile_row_count = (uint32_t)trow.size();
   *out = enc.release();
   return JXRHIP_OK;
 }

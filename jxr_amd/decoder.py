@@ -315,6 +315,20 @@ class HipDecodePlan:
     def from_encoded(cls, enc: EncodedImage, spec: OutputSpec):
         return cls(image_desc(enc, spec), keepalive=enc)
 
+    @classmethod
+    def from_codestream(cls, codestream, spec: Optional["OutputSpec"] = None, decoded=None, threads: int = 0):
+        """The plan of a parsed codestream (jxr_amd.codestream.Codestream):
+        its tiles entropy-decoded on host threads unless `decoded` already
+        holds them, and the file's own output format unless `spec` names
+        another (PreparedReconstruction::cuda_plan, crates/jxr/src/
+        reconstruction.rs:137-158). A reduced spec.scale reads only the bands
+        that scale keeps; a spec.crop is a region of the output image."""
+        spec = spec if spec is not None else codestream.default_output()
+        if decoded is None:
+            tiles = codestream.region_tiles(spec.crop) if spec.crop else None
+            decoded = codestream.decode(scale=spec.scale, tiles=tiles, threads=threads)
+        return cls(codestream.image_desc(decoded, spec), keepalive=(codestream, decoded))
+
     @property
     def handle(self):
         return self._h

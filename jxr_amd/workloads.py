@@ -39,9 +39,12 @@ def rgb_out(depth, planar):
                       output_color=abi.COLOR_RGB, planar=planar)
 
 
+def c1_pixels():
+    return synth.gradient_noise(512, 512, SEED, depth=8, sigma=8.0)
+
+
 def c1():
-    px = synth.gradient_noise(512, 512, SEED, depth=8, sigma=8.0)
-    enc = EncodedImage(px, color=abi.COLOR_YUV444, overlap=1, scaled=False, qp=(0, 0, 0))
+    enc = EncodedImage(c1_pixels(), color=abi.COLOR_YUV444, overlap=1, scaled=False, qp=(0, 0, 0))
     return [(enc, rgb_out(8, False))]
 
 

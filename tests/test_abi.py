@@ -11,7 +11,7 @@ import pytest
 from jxr_amd import abi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("jxr_hip.h", "jxr_hip_encode.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("jxr_hip.h", "jxr_hip_encode.h", "jxr_hip_codestream.h")]
 
 
 def declared_functions():
@@ -57,11 +57,13 @@ STRUCTS = {
     "jxrhip_valu_rate": abi.ValuRate,
     "jxrhip_stage_report": abi.StageReport,
     "jxrhip_decode_report": abi.DecodeReport,
+    "jxrhip_codestream_info": abi.CodestreamInfo,
+    "jxrhip_codestream_options": abi.CodestreamOptions,
 }
 
 
 def test_ctypes_layouts_match_the_c_headers(tmp_path):
-    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADERS[1]}"', "int main(void){"]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADERS[2]}"', "int main(void){"]
     for cname, py in STRUCTS.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
         for field, _ in py._fields_:

@@ -39,3 +39,15 @@ def test_emulated_allocation_free_batches_under_asan():
                          capture_output=True, text=True, env=env, timeout=600)
     assert res.returncode == 0, res.stdout[-3000:] + res.stderr[-3000:]
     assert "0 failures" in res.stdout
+
+
+def test_emulated_codestream_round_trips_and_damage_under_asan():
+    """The codestream reader and writer on the CPU build: round trips, the
+    reference's T.835 file, and damaged codestreams that must decode or be
+    refused without an out-of-bounds access (tests/emu_codestream.py)."""
+    emu_lib.build(asan=True)
+    env = dict(os.environ, LD_PRELOAD=emu_lib.asan_runtime(), ASAN_OPTIONS="detect_leaks=0")
+    res = subprocess.run([sys.executable, os.path.join(HERE, "emu_codestream.py"), "--asan"],
+                         capture_output=True, text=True, env=env, timeout=600)
+    assert res.returncode == 0, res.stdout[-3000:] + res.stderr[-3000:]
+    assert "0 failures" in res.stdout
