@@ -74,24 +74,26 @@ inline u32 sat_mul(u32 a, u32 b) {
 }
 
 // ---- bits -----------------------------------------------------------------
-// Most-significant-bit-first reading of one packet (or of the headers),
-// bounded by a bit length; a read past it is a truncation.
+// Most-significant-bit-first reading of one packet (or of the headers); a
+// read past its last byte is a truncation. The cache holds `avail_` bits
+// at its top; below them it may hold the stream's next bits, loaded early
+// by an eight-byte refill, which a later refill ORs in again unchanged.
 class BitReader {
  public:
-  BitReader(const u8 *data, size_t bytes) : p_(data), nbytes_(bytes), nbits_(bytes * 8) {}
-  BitReader(const u8 *data, size_t bytes, size_t bits) : p_(data), nbytes_(bytes), nbits_(bits) {}
-  size_t position() const { return pos_; }
-  size_t remaining() const { return nbits_ - pos_; }
+  BitReader(const u8 *data, size_t bytes) : p_(data), nbytes_(bytes) {}
+  size_t position() const { return next_ * 8 - (size_t)avail_; }
+  size_t remaining() const { return (nbytes_ - next_) * 8 + (size_t)avail_; }
   // n <= 32
-  u32 bits(unsigned n) {
+  __attribute__((always_inline)) u32 bits(unsigned n) {
     if (n == 0) return 0;
-    if (n > remaining())
-      syntax_error("truncated: " + std::to_string(n) + " bits wanted at bit " + std::to_string(pos_));
-    if ((int)n > avail_) refill();
+    if ((int)n > avail_) {
+      refill();
+      if ((int)n > avail_)
+        syntax_error("truncated: " + std::to_string(n) + " bits wanted at bit " + std::to_string(position()));
+    }
     const u32 v = (u32)(cache_ >> (64 - n));
     cache_ <<= n;
     avail_ -= (int)n;
-    pos_ += n;
     return v;
   }
   u64 bits64(unsigned n) {  // n <= 64
@@ -99,36 +101,61 @@ class BitReader {
     const u64 hi = bits(n - 32);
     return (hi << 32) | bits(32);
   }
-  bool bit() { return bits(1) != 0; }
+  __attribute__((always_inline)) bool bit() { return bits(1) != 0; }
   // the next eight bits, zeros past the end of the data
-  unsigned peek8() {
+  __attribute__((always_inline)) unsigned peek8() {
     if (avail_ < 8) refill();
     return (unsigned)(cache_ >> 56);
   }
+  // A flexbits value of `n` bits (n <= 15) and, when `may_sign` and the
+  // value is not zero, the sign bit after it; false (nothing read) when
+  // the packet holds too few bits for both, where a caller reads them one
+  // at a time.
+  __attribute__((always_inline)) bool magnitude_sign(unsigned n, bool may_sign, u32 &mag, bool &negative) {
+    if ((int)n + 1 > avail_) {
+      refill();
+      if ((int)n + 1 > avail_) return false;
+    }
+    mag = n ? (u32)(cache_ >> (64 - n)) : 0u;
+    const bool take = may_sign && mag != 0;
+    negative = take && ((cache_ << n) >> 63) != 0;
+    const unsigned used = n + (take ? 1u : 0u);
+    cache_ <<= used;
+    avail_ -= (int)used;
+    return true;
+  }
   // n bits that peek8 has shown and remaining() covers
-  void skip(unsigned n) {
+  __attribute__((always_inline)) void skip(unsigned n) {
     cache_ <<= n;
     avail_ -= (int)n;
-    pos_ += n;
   }
   void align() {
-    const unsigned r = (unsigned)(pos_ & 7);
+    const unsigned r = (unsigned)(position() & 7);
     if (r) bits(8 - r);
   }
   void align_zero(const char *what) {
-    while (pos_ & 7)
+    while (position() & 7)
       if (bit()) syntax_error(std::string("non-zero alignment bit in ") + what);
   }
 
  private:
   void refill() {
-    while (avail_ <= 56 && next_ < nbytes_) {
-      cache_ |= (u64)p_[next_++] << (56 - avail_);
-      avail_ += 8;
+    if (next_ + 8 <= nbytes_) {
+      u64 w;
+      std::memcpy(&w, p_ + next_, 8);
+      cache_ |= __builtin_bswap64(w) >> avail_;
+      const int take = (63 - avail_) >> 3;
+      next_ += (size_t)take;
+      avail_ += take * 8;
+    } else {
+      while (avail_ <= 56 && next_ < nbytes_) {
+        cache_ |= (u64)p_[next_++] << (56 - avail_);
+        avail_ += 8;
+      }
     }
   }
   const u8 *p_;
-  size_t nbytes_, nbits_, pos_ = 0, next_ = 0;
+  size_t nbytes_, next_ = 0;
   u64 cache_ = 0;
   int avail_ = 0;
 };
@@ -195,7 +222,7 @@ struct Vlc {
   }
 };
 
-u32 read_vlc(BitReader &r, const Vlc &t) {
+__attribute__((always_inline)) inline u32 read_vlc(BitReader &r, const Vlc &t) {
   const u16 e = t.lut[r.peek8()];
   const unsigned len = e >> 8;
   if (e == 0 || len > r.remaining()) {
@@ -532,7 +559,52 @@ void write_abs_level(BitWriter &w, Adaptive &a, u32 v) {
   w.put(rest - (1u << fixed), fixed);
 }
 
-u32 read_run(BitReader &r, int max_run) {
+// (run << 3) | bits of every eight-bit prefix, for each maximum run 1..14:
+// a run code and its refinement take at most seven bits, so one peek
+// resolves a run whenever eight bits remain.
+struct RunTable {
+  u8 entry[15][256];
+  RunTable() {
+    std::memset(entry, 0, sizeof entry);
+    for (int max_run = 1; max_run < 15; max_run++)
+      for (unsigned prefix = 0; prefix < 256; prefix++) {
+        unsigned run, len;
+        if (max_run == 1) {
+          run = 1;
+          len = 0;
+        } else if (max_run < 5) {
+          const u16 e = kRunValue[max_run - 2].lut[prefix];
+          run = e & 0xff;
+          len = e >> 8;
+        } else {
+          const u16 e = kRunIndex.lut[prefix];
+          const unsigned clen = e >> 8;
+          const int t = (int)(e & 0xff) + 5 * kRunBin[max_run];
+          const unsigned fixed = kRunFixed[t];
+          const unsigned refinement = fixed ? ((prefix << clen) & 0xff) >> (8 - fixed) : 0;
+          run = kRunRemap[t] + refinement;
+          len = clen + fixed;
+        }
+        entry[max_run][prefix] = (u8)((run << 3) | len);
+      }
+  }
+};
+const RunTable kRunTable;
+
+u32 read_run_slow(BitReader &r, int max_run);
+
+__attribute__((always_inline)) inline u32 read_run(BitReader &r, int max_run) {
+  if (max_run >= 1 && max_run <= 14) {
+    const u8 e = kRunTable.entry[max_run][r.peek8()];
+    if ((size_t)(e & 7) <= r.remaining()) {
+      r.skip(e & 7);
+      return e >> 3;
+    }
+  }
+  return read_run_slow(r, max_run);
+}
+
+u32 read_run_slow(BitReader &r, int max_run) {
   if (max_run <= 0 || max_run > 14) syntax_error("coefficient run with no room");
   if (max_run == 1) return 1;
   if (max_run < 5) return read_vlc(r, kRunValue[max_run - 2]);
@@ -646,7 +718,7 @@ void write_block(BitWriter &w, AcVlc &st, int cls, int start, const int *at, con
 
 // A flexbits refinement of a coefficient whose coded part is `coded`: its
 // low `bits`, with a sign of its own only when the coded part is zero.
-i32 read_flex(BitReader &r, i32 coded, unsigned bits) {
+__attribute__((always_inline)) inline i32 read_flex(BitReader &r, i32 coded, unsigned bits) {
   const i32 value = (i32)r.bits(bits);
   if (coded > 0) return value;
   if (coded < 0) return -value;
@@ -1828,8 +1900,15 @@ struct PlaneCoder {
     }
     for (int k = 1; k < 16; k++) {
       const int p = kTranspose[k];
-      const i32 flex = wrap_shl(read_flex(r, blk[p], left), trim);
-      const i64 v = (i64)wrap_shl(blk[p], bits) + flex;
+      const i32 coded = blk[p];
+      u32 mag;
+      bool negative;
+      i32 flex;
+      if (r.magnitude_sign(left, coded == 0, mag, negative))
+        flex = (coded < 0 || negative) ? -(i32)mag : (i32)mag;
+      else
+        flex = read_flex(r, coded, left);
+      const i64 v = (i64)wrap_shl(coded, bits) + wrap_shl(flex, trim);
       if (v < INT32_MIN || v > INT32_MAX) overflow("HP refinement");
       blk[p] = (i32)v;
     }
@@ -2234,7 +2313,8 @@ void decode_stream(const Stream &s, u32 scale, const u8 *mask, u32 threads, jxr:
     total += mbs * (size_t)cpm.back();
   }
   if (total > UINT32_MAX) unsupported("more than 2^32 coefficients in one arena");
-  out.coefficients.assign(total, 0);
+  out.coefficients.clear();
+  out.coefficients.resize(total);
   const size_t all_mbs = mbs * comps.size();
   out.offsets.assign(all_mbs, 0);
   out.quantizers.assign(all_mbs * 3, 0);
