@@ -133,6 +133,97 @@ def cpu_baseline(items, plans, dense, offsets, budget_s=15.0):
                 p95_value=round(px / slow / 1e6, 3), passes=len(runs), parity_checked_images=checked)
 
 
+def end_to_end(session, items, plans, dense, offsets, cpu_recon, passes=3):
+    """The CPU half of the decode beside the reconstruction: every image of
+    the workload coded into a codestream by the repo's writer (untimed), then
+    (i) the host entropy decoder alone, images in parallel on the host cores;
+    (ii) the CPU baseline end to end, that entropy decode followed by the
+    oracle's reconstruction on the same cores; (iii) the GPU path end to end
+    from codestreams in host memory: entropy decode on host threads
+    overlapped with the uploads and persistent-kernel launches of groups of
+    images already decoded, into the same NCHW destinations, whose bytes are
+    then checked against the resident run's."""
+    from concurrent.futures import ThreadPoolExecutor
+    from jxr_amd import HipDecodePlan, abi
+    from jxr_amd.codestream import Codestream, encode_codestream
+    threads = max(1, min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16))
+    t0 = time.perf_counter()
+    # a reduced decode reads band packets: frequency-mode codestreams
+    frequency = any(spec.scale != abi.SCALE_FULL for _, spec in items)
+    with ThreadPoolExecutor(threads) as ex:
+        streams = list(ex.map(lambda it: Codestream(encode_codestream(
+            it[0], frequency=frequency, annex_a=it[0].alpha_mode == abi.ALPHA_SEPARATE)), items))
+    coded = sum(cs.info.codestream_bytes + cs.info.alpha_bytes for cs in streams)
+    log(f"[e2e] {len(streams)} codestreams, {coded / 1e6:.1f} MB, written in {time.perf_counter() - t0:.1f} s")
+    pixels = sum(p.output_pixels for p in plans)
+    scales = [spec.scale for _, spec in items]
+
+    def decode_all(ex):
+        return list(ex.map(lambda k: streams[k].decode(scale=scales[k], threads=1), range(len(streams))))
+
+    runs = []
+    with ThreadPoolExecutor(threads) as ex:
+        decode_all(ex)
+        for _ in range(passes):
+            t0 = time.perf_counter()
+            decs = decode_all(ex)
+            runs.append(time.perf_counter() - t0)
+            del decs
+    t_ent = float(np.median(runs))
+    ent_rate = pixels / t_ent / 1e6
+    log(f"[e2e] entropy decode: {t_ent * 1e3:.1f} ms per pass, {ent_rate:.1f} Mpx/s on {threads} threads")
+
+    # (iii) pipelined: decoding on the pool, groups of images submitted as
+    # soon as all of them are decoded
+    host_resident = dense.to_host()
+    group = 16
+    walls = []
+    with ThreadPoolExecutor(threads) as ex:
+        for rep in range(passes + 1):
+            dense.fill(0)
+            session.clear_cache()
+            session.synchronize()
+            t0 = time.perf_counter()
+            futs = [ex.submit(streams[k].decode, scales[k], None, 1) for k in range(len(streams))]
+            batches, keep = [], []
+            for g in range(0, len(futs), group):
+                idx = range(g, min(g + group, len(futs)))
+                gp = [HipDecodePlan.from_codestream(streams[k], items[k][1], decoded=futs[k].result()) for k in idx]
+                batches.append(session.submit_batch(gp, [dense.ptr + offsets[k] for k in idx], abi.BATCH_PERSISTENT))
+                keep.append(gp)
+            for b in batches:
+                if b.wait().any():
+                    raise SystemExit("end-to-end decode: an image failed on the device")
+            session.synchronize()
+            if rep:
+                walls.append(time.perf_counter() - t0)
+            del keep, batches, futs
+    if not np.array_equal(dense.to_host(), host_resident):
+        raise SystemExit("end-to-end decode differs from the resident arenas' decode")
+    session.clear_cache()
+    t_e2e = float(np.median(walls))
+    cpu_e2e = None
+    if cpu_recon is not None:
+        cpu_e2e = 1.0 / (1.0 / ent_rate + 1.0 / cpu_recon["value"])
+    log(f"[e2e] GPU from codestreams: {t_e2e * 1e3:.1f} ms per pass, {pixels / t_e2e / 1e6:.1f} Mpx/s")
+    return {
+        "codestream_mb": round(coded / 1e6, 2),
+        "bits_per_pixel": round(coded * 8 / pixels, 3),
+        "entropy_decode": {"value": round(ent_rate, 1), "unit": "Mpixel/s", "cores": threads,
+                           "ms_per_pass": round(t_ent * 1e3, 2),
+                           "what": "host adaptive-VLC decode of every tile packet into arenas, one image per "
+                                   "thread, median of passes"},
+        "gpu_from_codestreams": {"value": round(pixels / t_e2e / 1e6, 1), "unit": "Mpixel/s",
+                                 "ms_per_pass": round(t_e2e * 1e3, 2), "group_images": group,
+                                 "what": "codestreams in host memory -> host entropy decode overlapped with "
+                                         "uploads and persistent launches -> device NCHW, bytes checked against "
+                                         "the resident decode"},
+        "cpu_baseline_end_to_end": None if cpu_e2e is None else {
+            "value": round(cpu_e2e, 1), "unit": "Mpixel/s", "cores": threads, "kind": "port",
+            "what": "entropy decode then the oracle's reconstruction on the same host cores"},
+    }
+
+
 _SERVE_POOLS = {}
 
 
@@ -213,6 +304,8 @@ def main():
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-per-image", action="store_true")
+    ap.add_argument("--no-end-to-end", action="store_true",
+                    help="skip the codestream figures (host entropy decode, decode from codestreams)")
     ap.add_argument("--tile", default=None, help="TXxTY macroblocks per work item")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--no-graph", action="store_true", help="submit the batch eagerly every step")
@@ -391,6 +484,8 @@ def main():
                                            args.serve_rate)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(items, plans, dense, offsets)
+    if not args.no_end_to_end:
+        out["end_to_end"] = end_to_end(session, items, plans, dense, offsets, out.get("cpu_baseline"))
     print(json.dumps(out), flush=True)
 
 
