@@ -426,6 +426,16 @@ jxrhip_status jxrhip_fill_destination(jxrhip_session *session, void *device_ptr,
                                       uint64_t bytes);
 jxrhip_status jxrhip_copy_to_host(jxrhip_session *session, const void *device_ptr, void *host,
                                   uint64_t bytes);
+/* Page-locked host memory (hipHostMalloc) of the session's device, for
+ * staging that uploads or readbacks should reach by DMA without a copy:
+ * for example the coefficient storage of jxrhip_codestream_decode_into.
+ * No counterpart in the reference (its uploads go through cudarc from
+ * pageable memory). A block keeps its session until it is freed.
+ * jxrhip_host_free takes only pointers jxrhip_host_alloc returned; the
+ * caller frees a block once every submission that reads it has completed
+ * (the call also waits for the session's streams). */
+jxrhip_status jxrhip_host_alloc(jxrhip_session *session, uint64_t bytes, void **host_ptr);
+jxrhip_status jxrhip_host_free(jxrhip_session *session, void *host_ptr);
 /* Device-to-device copy of `bytes` between destinations, on the session's
  * first stream (waits for completion). */
 jxrhip_status jxrhip_copy_device(jxrhip_session *session, void *dst, const void *src, uint64_t bytes);

@@ -129,6 +129,23 @@ jxrhip_status jxrhip_codestream_tile_grid(const jxrhip_codestream *codestream, u
  * `threads` host threads (0 = one per host core, at most 16). */
 jxrhip_status jxrhip_codestream_decode(const jxrhip_codestream *codestream, uint32_t scale,
                                        const uint8_t *tile_mask, uint32_t threads, jxrhip_decoded **out);
+/* The coefficient counts of the arenas a decode with these arguments
+ * produces (tile_decode/mod.rs:121-127 coefficient_count): the primary
+ * arena's and the separate alpha arena's (0 without one). */
+jxrhip_status jxrhip_codestream_coefficient_count(const jxrhip_codestream *codestream, uint32_t scale,
+                                                  const uint8_t *tile_mask, uint64_t *primary, uint64_t *alpha);
+/* jxrhip_codestream_decode with the coefficients written into caller-owned
+ * storage (tile_decode/mod.rs:99-119 decode_tiles_into), for example pinned
+ * host memory from jxrhip_host_alloc, from which a session's upload is an
+ * asynchronous DMA. The arenas point into that storage, which must outlive
+ * them and every plan made from them; a submission reads it on its stream,
+ * so pinned storage stays unchanged until that submission has completed.
+ * Storage below the counts
+ * jxrhip_codestream_coefficient_count gives is JXRHIP_ERR_BUFFER_TOO_SMALL. */
+jxrhip_status jxrhip_codestream_decode_into(const jxrhip_codestream *codestream, uint32_t scale,
+                                            const uint8_t *tile_mask, uint32_t threads, int32_t *primary,
+                                            uint64_t primary_capacity, int32_t *alpha, uint64_t alpha_capacity,
+                                            jxrhip_decoded **out);
 void jxrhip_decoded_destroy(jxrhip_decoded *decoded);
 /* which = 0: primary arena; 1: separate alpha arena (NULL if none). Valid as
  * long as `decoded`. */

@@ -411,6 +411,10 @@ SIGNATURES = {
     "jxrhip_codestream_tile_grid": (st, [vp, u32p, u32p]),
     "jxrhip_codestream_decode": (st, [vp, C.c_uint32, vp, C.c_uint32, pp]),
     "jxrhip_decoded_destroy": (None, [vp]),
+    "jxrhip_codestream_coefficient_count": (st, [vp, C.c_uint32, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "jxrhip_codestream_decode_into": (st, [vp, C.c_uint32, vp, C.c_uint32, vp, C.c_uint64, vp, C.c_uint64, pp]),
+    "jxrhip_host_alloc": (st, [vp, C.c_uint64, pp]),
+    "jxrhip_host_free": (st, [vp, vp]),
     "jxrhip_decoded_arena": (C.POINTER(Arena), [vp, C.c_uint32]),
     "jxrhip_encoded_codestream": (st, [vp, C.POINTER(CodestreamOptions), C.POINTER(C.POINTER(C.c_uint8)),
                                        C.POINTER(C.c_uint64)]),

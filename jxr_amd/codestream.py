@@ -112,19 +112,42 @@ class Codestream:
     def tile_count(self) -> int:
         return len(self.tile_columns) * len(self.tile_rows)
 
+    def _mask(self, tiles):
+        if tiles is None:
+            return None
+        if len(tiles) != self.tile_count:
+            raise ValueError("one flag per tile")
+        return (C.c_uint8 * len(tiles))(*[1 if t else 0 for t in tiles])
+
+    def coefficient_count(self, scale: int = abi.SCALE_FULL, tiles: Optional[Sequence[bool]] = None):
+        """(primary, separate alpha) coefficient counts of the arenas a
+        decode with these arguments produces."""
+        p, a = C.c_uint64(), C.c_uint64()
+        check(load().jxrhip_codestream_coefficient_count(self._h, scale, self._mask(tiles), C.byref(p), C.byref(a)),
+              "coefficient count")
+        return p.value, a.value
+
     def decode(self, scale: int = abi.SCALE_FULL, tiles: Optional[Sequence[bool]] = None,
-               threads: int = 0) -> DecodedArenas:
+               threads: int = 0, into=None) -> DecodedArenas:
         """Entropy-decodes every tile, or those `tiles` (one flag per tile in
         raster order) selects, on up to `threads` host threads (0: one per
-        core, at most sixteen)."""
-        mask = None
-        if tiles is not None:
-            if len(tiles) != self.tile_count:
-                raise ValueError("one flag per tile")
-            mask = (C.c_uint8 * len(tiles))(*[1 if t else 0 for t in tiles])
+        core, at most sixteen). `into`, a HostBuffer (or any object with
+        `ptr` and `nbytes`) of at least coefficient_count() values, takes the
+        coefficients: pinned memory makes the upload a DMA."""
+        mask = self._mask(tiles)
         h = C.c_void_p()
-        check(load().jxrhip_codestream_decode(self._h, scale, mask, threads, C.byref(h)), "entropy decode")
-        return DecodedArenas(h, self)
+        if into is None:
+            check(load().jxrhip_codestream_decode(self._h, scale, mask, threads, C.byref(h)), "entropy decode")
+            return DecodedArenas(h, self)
+        np_, na = self.coefficient_count(scale, tiles)
+        if (np_ + na) * 4 > into.nbytes:
+            raise ValueError("coefficient storage below the arenas' size")
+        check(load().jxrhip_codestream_decode_into(self._h, scale, mask, threads, C.c_void_p(into.ptr), np_,
+                                                   C.c_void_p(into.ptr + np_ * 4) if na else None, na, C.byref(h)),
+              "entropy decode")
+        d = DecodedArenas(h, self)
+        d._storage = into
+        return d
 
     def region_tiles(self, region) -> list:
         """Tiles a region (x, y, w, h) of the output needs, with the one

@@ -2254,8 +2254,11 @@ struct TileJob {
 };
 
 // Decodes one stream's tiles into `out` (the primary codestream's planes, or
-// a separate alpha codestream's one plane).
-void decode_stream(const Stream &s, u32 scale, const u8 *mask, u32 threads, jxr::ArenaOut &out) {
+// a separate alpha codestream's one plane): its coefficients into `external`
+// when given (at least `capacity` values), else into out.coefficients. With
+// `count_only`, only the arena's coefficient count is computed.
+void decode_stream(const Stream &s, u32 scale, const u8 *mask, u32 threads, jxr::ArenaOut &out,
+                   int32_t *external = nullptr, uint64_t capacity = 0, uint64_t *count_only = nullptr) {
   const Headers &h = s.h;
   const PlaneHeader *planes[2] = {&h.primary, h.image.alpha_plane ? &h.alpha : nullptr};
   const int nplanes = planes[1] ? 2 : 1;
@@ -2313,8 +2316,19 @@ void decode_stream(const Stream &s, u32 scale, const u8 *mask, u32 threads, jxr:
     total += mbs * (size_t)cpm.back();
   }
   if (total > UINT32_MAX) unsupported("more than 2^32 coefficients in one arena");
+  if (count_only) {
+    *count_only = total;
+    return;
+  }
   out.coefficients.clear();
-  out.coefficients.resize(total);
+  int32_t *base;
+  if (external) {
+    if (capacity < total) throw Failure{JXRHIP_ERR_BUFFER_TOO_SMALL, "coefficient storage below the arena's size"};
+    base = external;
+  } else {
+    out.coefficients.resize(total);
+    base = out.coefficients.data();
+  }
   const size_t all_mbs = mbs * comps.size();
   out.offsets.assign(all_mbs, 0);
   out.quantizers.assign(all_mbs * 3, 0);
@@ -2346,7 +2360,7 @@ void decode_stream(const Stream &s, u32 scale, const u8 *mask, u32 threads, jxr:
     std::vector<std::unique_ptr<PlaneCoder>> coders;
     for (int p = 0; p < nplanes; p++) coders.emplace_back(new PlaneCoder(planes[p], bands[p], w, th));
     // component k's destination for macroblock m of this tile
-    auto dest = [&](size_t k, int m) { return out.coefficients.data() + comp_coef[k] + (job.mb_before + m) * cpm[k]; };
+    auto dest = [&](size_t k, int m) { return base + comp_coef[k] + (job.mb_before + m) * cpm[k]; };
     std::vector<size_t> first_k(nplanes);
     {
       size_t k = 0;
@@ -2478,7 +2492,7 @@ void decode_stream(const Stream &s, u32 scale, const u8 *mask, u32 threads, jxr:
             else
               for (int b = 0; b < blocks; b++) d[b * 16] = rc.low[b];
             const size_t mi = k * mbs + job.mb_before + m;
-            out.offsets[mi] = (u32)(d - out.coefficients.data());
+            out.offsets[mi] = (u32)(d - base);
             const size_t set_lp = (size_t)sl.lp_index * pc.q.comps + c, set_hp = (size_t)sl.hp_index * pc.q.comps + c;
             if (set_lp >= pc.q.lp.size() || set_hp >= pc.q.hp.size()) syntax_error("quantizer index past its table");
             const u8 qd = pc.q.dc[c], ql = pc.q.lp[set_lp], qh = pc.q.hp[set_hp];
@@ -2529,8 +2543,8 @@ void decode_stream(const Stream &s, u32 scale, const u8 *mask, u32 threads, jxr:
 
   jxrhip_arena &a = out.arena;
   std::memset(&a, 0, sizeof a);
-  a.coefficients = out.coefficients.data();
-  a.coefficient_count = out.coefficients.size();
+  a.coefficients = base;
+  a.coefficient_count = total;
   a.macroblock_count = out.offsets.size();
   a.coefficient_offsets = out.offsets.data();
   a.quantizers = out.quantizers.data();
@@ -3168,6 +3182,63 @@ extern "C" jxrhip_status jxrhip_codestream_decode(const jxrhip_codestream *cs, u
       if (cs->alpha.cols != cs->primary.cols || cs->alpha.rows != cs->primary.rows)
         unsupported("separate alpha tile grid differs from the primary's");
       decode_stream(cs->alpha, scale, tile_mask, threads, d->alpha);
+      d->has_alpha = true;
+    }
+    *out = d.release();
+    return JXRHIP_OK;
+  } catch (const Failure &f) {
+    return fail_with(f);
+  } catch (const std::bad_alloc &) {
+    jxrhip_set_error("out of host memory");
+    return JXRHIP_ERR_INVALID_ARGUMENT;
+  }
+}
+
+extern "C" jxrhip_status jxrhip_codestream_coefficient_count(const jxrhip_codestream *cs, uint32_t scale,
+                                                             const uint8_t *tile_mask, uint64_t *primary,
+                                                             uint64_t *alpha) {
+  if (!cs || !primary || !alpha) {
+    jxrhip_set_error("null argument");
+    return JXRHIP_ERR_INVALID_ARGUMENT;
+  }
+  *primary = *alpha = 0;
+  if (scale > JXRHIP_SCALE_SIXTEENTH) {
+    jxrhip_set_error("decode scale");
+    return JXRHIP_ERR_INVALID_ARGUMENT;
+  }
+  try {
+    jxr::ArenaOut scratch;
+    decode_stream(cs->primary, scale, tile_mask, 1, scratch, nullptr, 0, primary);
+    if (cs->has_separate) decode_stream(cs->alpha, scale, tile_mask, 1, scratch, nullptr, 0, alpha);
+    return JXRHIP_OK;
+  } catch (const Failure &f) {
+    return fail_with(f);
+  } catch (const std::bad_alloc &) {
+    jxrhip_set_error("out of host memory");
+    return JXRHIP_ERR_INVALID_ARGUMENT;
+  }
+}
+
+extern "C" jxrhip_status jxrhip_codestream_decode_into(const jxrhip_codestream *cs, uint32_t scale,
+                                                       const uint8_t *tile_mask, uint32_t threads, int32_t *primary,
+                                                       uint64_t primary_capacity, int32_t *alpha,
+                                                       uint64_t alpha_capacity, jxrhip_decoded **out) {
+  if (!cs || !out || !primary || (cs->has_separate && !alpha)) {
+    jxrhip_set_error("null argument");
+    return JXRHIP_ERR_INVALID_ARGUMENT;
+  }
+  *out = nullptr;
+  if (scale > JXRHIP_SCALE_SIXTEENTH) {
+    jxrhip_set_error("decode scale");
+    return JXRHIP_ERR_INVALID_ARGUMENT;
+  }
+  try {
+    std::unique_ptr<jxrhip_decoded> d(new jxrhip_decoded());
+    decode_stream(cs->primary, scale, tile_mask, threads, d->primary, primary, primary_capacity);
+    if (cs->has_separate) {
+      if (cs->alpha.cols != cs->primary.cols || cs->alpha.rows != cs->primary.rows)
+        unsupported("separate alpha tile grid differs from the primary's");
+      decode_stream(cs->alpha, scale, tile_mask, threads, d->alpha, alpha, alpha_capacity);
       d->has_alpha = true;
     }
     *out = d.release();

@@ -30,6 +30,7 @@
 #include <string>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/jxr_hip.h"
@@ -947,6 +948,7 @@ struct jxrhip_session {
   // device destinations from jxrhip_allocate_destination; each one is an
   // owner too, until jxrhip_free_destination
   std::unordered_map<void *, uint64_t> destinations;
+  std::unordered_set<void *> host_blocks;  // jxrhip_host_alloc blocks
   double last_ms = 0;
   uint32_t last_launches = 0;
   jxrhip_diagnostics diag{};
@@ -1807,6 +1809,40 @@ extern "C" jxrhip_status jxrhip_plan_decode_report(const jxrhip_plan *p, uint32_
 }
 
 extern "C" jxrhip_status jxrhip_batch_enqueue_wait(const jxrhip_batch *b, void *stream);
+
+extern "C" jxrhip_status jxrhip_host_alloc(jxrhip_session *s, uint64_t bytes, void **ptr) {
+  if (!s || !ptr) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");
+  *ptr = nullptr;
+  make_current(s);
+  HIP_TRY(hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault), "hipHostMalloc");
+  {
+    std::lock_guard<std::mutex> g(s->lock);
+    s->host_blocks.insert(*ptr);
+  }
+  session_ref(s);  // a block keeps the session, as a destination does
+  return JXRHIP_OK;
+}
+extern "C" jxrhip_status jxrhip_host_free(jxrhip_session *s, void *ptr) {
+  if (!s) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null session");
+  if (!ptr) return JXRHIP_OK;
+  {
+    std::lock_guard<std::mutex> g(s->lock);
+    auto it = s->host_blocks.find(ptr);
+    if (it == s->host_blocks.end()) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "not a host block of this session");
+    s->host_blocks.erase(it);
+  }
+  make_current(s);
+  // uploads from the block may still be in flight on the session's streams
+  jxrhip_status st = JXRHIP_OK;
+  for (uint32_t i = 0; i < kStreams && !st; i++) {
+    const hipError_t e = hipStreamSynchronize(s->streams[i]);
+    if (e != hipSuccess) st = hip_fail(e, "hipStreamSynchronize");
+  }
+  const hipError_t e = hipHostFree(ptr);
+  if (e != hipSuccess && !st) st = hip_fail(e, "hipHostFree");
+  session_unref(s);
+  return st;
+}
 
 extern "C" jxrhip_status jxrhip_allocate_destination(jxrhip_session *s, uint64_t bytes, void **ptr) {
   if (!s || !ptr) return fail(JXRHIP_ERR_INVALID_ARGUMENT, "null argument");

@@ -538,6 +538,37 @@ class HipBatchGraph:
             pass
 
 
+class HostBuffer:
+    """Page-locked host memory of a session (jxrhip_host_alloc)."""
+
+    def __init__(self, session, nbytes: int):
+        self._s = session
+        self._sh = session.handle
+        p = C.c_void_p()
+        check(load().jxrhip_host_alloc(session.handle, nbytes, C.byref(p)), "host_alloc")
+        self._p = p
+        self.nbytes = nbytes
+
+    @property
+    def ptr(self) -> int:
+        return self._p.value or 0
+
+    def view(self, dtype=np.uint8) -> np.ndarray:
+        buf = (C.c_uint8 * self.nbytes).from_address(self.ptr)
+        return np.frombuffer(buf, dtype=dtype)
+
+    def close(self):
+        if getattr(self, "_p", None):
+            load().jxrhip_host_free(self._sh, self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceBuffer:
     """Device memory of a session (CudaDecoderSession::allocate_destination)."""
 
@@ -692,6 +723,11 @@ class HipDecoderSession:
 
     def allocate_destination(self, nbytes: int) -> "DeviceBuffer":
         return DeviceBuffer(self, nbytes)
+
+    def allocate_host(self, nbytes: int) -> "HostBuffer":
+        """Page-locked host memory (jxrhip_host_alloc): staging the session
+        uploads from by DMA."""
+        return HostBuffer(self, nbytes)
 
     def synchronize(self):
         check(load().jxrhip_session_synchronize(self._h), "synchronize")

@@ -6,6 +6,8 @@ and the failures a malformed or unsupported codestream gives.
 
 Pixels come from the oracle (the checker) decoding what the reader made; the
 GPU half of the same comparisons is tests/test_gpu_codestream.py."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -13,6 +15,7 @@ import oracle_lib as O
 from codestream_util import (GOLDEN, arena_arrays, arena_differences, bits_to_bytes, codestream,
                              dc_only_luma_codestream, minimal_header)
 from jxr_amd import EncodedImage, JxrHipError, abi, synth
+from jxr_amd.abi import load
 from jxr_amd.codestream import Codestream, encode_codestream
 
 
@@ -260,6 +263,66 @@ def test_threads_do_not_change_the_arena():
     one = arena_arrays(cs.decode(threads=1).primary)
     many = arena_arrays(cs.decode(threads=8).primary)
     assert not arena_differences(one, many)
+
+
+class NumpyStorage:
+    """Caller storage for Codestream.decode(into=...): pageable numpy memory
+    (the ABI takes any host memory; pinned blocks only make uploads DMA)."""
+
+    def __init__(self, values):
+        self.array = np.full(values, 0x5A5A5A5A, np.int32)
+        self.ptr = self.array.ctypes.data
+        self.nbytes = self.array.nbytes
+
+
+@pytest.mark.parametrize("case", [CASES[1], CASES[4], CASES[8], CASES[10]], ids=lambda c: c[0])
+def test_decode_into_caller_storage_equals_the_owned_arena(case):
+    px, enc = make(case)
+    cs = Codestream(encode_codestream(enc, frequency=case[8], annex_a=case[9] == abi.ALPHA_SEPARATE))
+    owned = cs.decode()
+    n_primary, n_alpha = cs.coefficient_count()
+    assert n_primary == owned.primary.contents.coefficient_count
+    assert n_alpha == (owned.alpha.contents.coefficient_count if owned.alpha else 0)
+    store = NumpyStorage(n_primary + n_alpha + 7)
+    into = cs.decode(threads=3, into=store)
+    got = arena_arrays(into.primary)
+    assert not arena_differences(arena_arrays(owned.primary), got)
+    # the arena's coefficients are the caller's storage, written in place
+    assert C.addressof(into.primary.contents.coefficients.contents) == store.ptr
+    assert np.array_equal(store.array[:n_primary], got["coefficients"])
+    if n_alpha:
+        assert not arena_differences(arena_arrays(owned.alpha), arena_arrays(into.alpha))
+        assert C.addressof(into.alpha.contents.coefficients.contents) == store.ptr + 4 * n_primary
+    assert np.all(store.array[n_primary + n_alpha:] == 0x5A5A5A5A)
+
+
+def test_decode_into_counts_selected_tiles_and_reduced_scales():
+    px, enc = make(CASES[4])
+    cs = Codestream(encode_codestream(enc, frequency=True))
+    mask = [True, False, False, True]
+    for scale in (abi.SCALE_FULL, abi.SCALE_QUARTER, abi.SCALE_SIXTEENTH):
+        owned = cs.decode(scale=scale, tiles=mask)
+        n, na = cs.coefficient_count(scale, mask)
+        assert na == 0 and n == owned.primary.contents.coefficient_count
+        store = NumpyStorage(n)
+        into = cs.decode(scale=scale, tiles=mask, into=store)
+        assert not arena_differences(arena_arrays(owned.primary), arena_arrays(into.primary))
+
+
+def test_decode_into_rejects_storage_below_the_arena():
+    px, enc = make(CASES[0])
+    cs = Codestream(encode_codestream(enc))
+    n, _ = cs.coefficient_count()
+    lib = load()
+    store = NumpyStorage(n)
+    h = C.c_void_p()
+    assert lib.jxrhip_codestream_decode_into(cs._h, abi.SCALE_FULL, None, 1, C.c_void_p(store.ptr), n - 1, None, 0,
+                                             C.byref(h)) == abi.ERR_BUFFER_TOO_SMALL
+    assert not h.value
+    assert lib.jxrhip_codestream_decode_into(cs._h, abi.SCALE_FULL, None, 1, None, n, None, 0,
+                                             C.byref(h)) == abi.ERR_INVALID_ARGUMENT
+    with pytest.raises(ValueError):
+        cs.decode(into=NumpyStorage(n - 1))
 
 
 def test_reduced_decodes_keep_the_dc_and_lp_bands():

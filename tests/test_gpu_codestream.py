@@ -114,3 +114,26 @@ def test_codestream_batch_on_gpu(session):
     assert not stages.any()
     for b, want in zip(bufs, wants):
         assert np.array_equal(b.to_host(0, want.nbytes), want)
+
+
+def test_decode_into_pinned_host_blocks_on_gpu(session):
+    # coefficients entropy-decoded straight into page-locked host blocks of
+    # the session, uploaded from there by DMA; the block is reused for a
+    # second codestream once the first submission completed
+    cases = [T.CASES[3], T.CASES[10]]
+    streams = [Codestream(encode_codestream(T.make(c)[1], frequency=c[8], annex_a=c[9] == abi.ALPHA_SEPARATE))
+               for c in cases]
+    need = max(sum(cs.coefficient_count()) for cs in streams) * 4
+    block = session.allocate_host(need)
+    for cs in streams:
+        dec = cs.decode(into=block)
+        spec = cs.default_output()
+        plan = HipDecodePlan.from_codestream(cs, spec, decoded=dec)
+        got = session.decode_to_host(plan)
+        want = O.decode(cs.image_desc(cs.decode(), spec))
+        assert np.array_equal(got, want)
+        del plan, dec
+    block.close()
+    from jxr_amd.abi import load
+    import ctypes as C
+    assert load().jxrhip_host_free(session.handle, C.c_void_p(16)) == abi.ERR_INVALID_ARGUMENT
