@@ -140,10 +140,10 @@ def end_to_end(session, items, plans, dense, offsets, cpu_recon, passes=3):
     (i) the host entropy decoder alone, images in parallel on the host cores;
     (ii) the CPU baseline end to end, that entropy decode followed by the
     oracle's reconstruction on the same cores; (iii) the GPU path end to end
-    from codestreams in host memory: entropy decode on host threads
-    overlapped with the uploads and persistent-kernel launches of groups of
-    images already decoded, into the same NCHW destinations, whose bytes are
-    then checked against the resident run's."""
+    from codestreams in host memory: entropy decode and planning on host
+    threads overlapped with the uploads and persistent-kernel launches of
+    groups of images already decoded, into the same NCHW destinations, whose
+    bytes are then checked against the resident run's."""
     from concurrent.futures import ThreadPoolExecutor
     from jxr_amd import HipDecodePlan, abi
     from jxr_amd.codestream import Codestream, encode_codestream
@@ -198,21 +198,24 @@ def end_to_end(session, items, plans, dense, offsets, cpu_recon, passes=3):
     host_resident = dense.to_host()
     group = 16
 
+    def decode_and_plan(k, pinned):
+        dec = streams[k].decode(scale=scales[k], threads=1, into=slices[k] if pinned else None)
+        return HipDecodePlan.from_codestream(streams[k], items[k][1], decoded=dec)
+
     def pipelined(pinned):
         walls = []
-        with ThreadPoolExecutor(threads) as ex:
+        # one core stays with the thread that submits the groups
+        with ThreadPoolExecutor(max(1, threads - 1)) as ex:
             for rep in range(passes + 1):
                 dense.fill(0)
                 session.clear_cache()
                 session.synchronize()
                 t0 = time.perf_counter()
-                futs = {k: ex.submit(streams[k].decode, scales[k], None, 1, slices[k] if pinned else None)
-                        for k in order}
+                futs = {k: ex.submit(decode_and_plan, k, pinned) for k in order}
                 batches, keep = [], []
                 for g in range(0, len(order), group):
                     idx = order[g:g + group]
-                    gp = [HipDecodePlan.from_codestream(streams[k], items[k][1], decoded=futs[k].result())
-                          for k in idx]
+                    gp = [futs[k].result() for k in idx]
                     batches.append(session.submit_batch(gp, [dense.ptr + offsets[k] for k in idx],
                                                         abi.BATCH_PERSISTENT))
                     keep.append(gp)
@@ -260,6 +263,142 @@ def end_to_end(session, items, plans, dense, offsets, cpu_recon, passes=3):
 
 
 _SERVE_POOLS = {}
+
+
+def serve_codestreams(session, seconds, budget_gb):
+    """The c5 mix served from codestreams: each request's file is
+    entropy-decoded on a host worker (only the tiles a region needs) into a
+    page-locked block of a per-kind pool, planned, and submitted to the
+    serving loop with its arrival time, so the latency runs from arrival
+    through the entropy decode, the upload and the reconstruction. First a
+    closed-loop pass (every request of a one-second schedule submitted at
+    once) measures the capacity; then Poisson arrivals at half of it give
+    p50/p99. Returns both."""
+    import queue
+    from concurrent.futures import ThreadPoolExecutor
+    from jxr_amd import HipDecodePlan, HipServer, workloads
+    from jxr_amd.codestream import Codestream, encode_codestream
+    from jxr_amd.serving import clock_ns
+    t0 = time.perf_counter()
+    if True not in _SERVE_POOLS and False not in _SERVE_POOLS:
+        pool = workloads.c5_pool(cache=True)
+        _SERVE_POOLS[True] = (pool, {k: [HipDecodePlan.from_encoded(e, sp) for e, sp in pool[k]]
+                                     for k in ("small", "gray", "region")})
+    pool = (_SERVE_POOLS.get(True) or _SERVE_POOLS[False])[0]
+    workers = max(2, min(len(os.sched_getaffinity(0)), 16) - 1)
+    with ThreadPoolExecutor(workers) as ex:
+        small = list(ex.map(lambda es: Codestream(encode_codestream(es[0])), pool["small"]))
+        gray = list(ex.map(lambda es: Codestream(encode_codestream(es[0])), pool["gray"]))
+    source = Codestream(encode_codestream(pool["source"]))
+    # per request kind and pool index: codestream, output, tiles to decode
+    info = {"small": [(cs, sp, None) for cs, (_, sp) in zip(small, pool["small"])],
+            "gray": [(cs, sp, None) for cs, (_, sp) in zip(gray, pool["gray"])],
+            "region": [(source, sp, source.region_tiles(sp.crop)) for _, sp in pool["region"]]}
+    coded = {k: sum(cs.info.codestream_bytes for cs, _, _ in v) / len(v) for k, v in info.items()}
+    log(f"[serve-cs] codestreams written in {time.perf_counter() - t0:.1f} s; mean file bytes "
+        + ", ".join(f"{k} {v / 1e3:.0f} kB" for k, v in coded.items()))
+    # page-locked blocks, one per request in flight, pooled by kind
+    blocks_per_kind = {"small": 48, "gray": 12, "region": 24}
+    slabs, free = [], {}
+    for kind, count in blocks_per_kind.items():
+        need = max(sum(cs.coefficient_count(sp.scale, m)) for cs, sp, m in info[kind]) * 4
+        slab = session.allocate_host(need * count)
+        slabs.append(slab)
+        free[kind] = queue.SimpleQueue()
+        for j in range(count):
+            free[kind].put(SimpleNamespace(ptr=slab.ptr + j * need, nbytes=need))
+
+    # the bytes of the first requests of each kind are compared with the
+    # device decode of the encoder's own arenas for the same image
+    want = {}
+    checked = {"small": 0, "gray": 0, "region": 0}
+    plans_pool = (_SERVE_POOLS.get(True) or _SERVE_POOLS[False])[1]
+
+    def check(kind, k, image):
+        if checked[kind] >= 4:
+            return
+        if (kind, k) not in want:
+            want[(kind, k)] = session.decode_to_host(plans_pool[kind][k])
+        if not np.array_equal(image.readback(), want[(kind, k)]):
+            raise SystemExit(f"serving from codestreams: {kind} image {k} differs from its arena's decode")
+        checked[kind] += 1
+
+    def run(sched, arrivals, label):
+        n = len(sched)
+        server = HipServer(session, hbm_budget_bytes=int(budget_gb * (1 << 30)), max_wave_images=256, max_waves=4)
+        keep, errors = {}, []
+
+        def work(i):
+            try:
+                _, kind, k = sched[i]
+                cs, sp, mask = info[kind][k]
+                blk = free[kind].get()
+                # a 4096x4096 file's 64 tiles decode on four threads of its own
+                dec = cs.decode(scale=sp.scale, tiles=mask, threads=4 if kind == "gray" else 1, into=blk)
+                # each request uploads its own coefficients (no upload cache entry)
+                dec.primary.contents.identity = 0
+                plan = HipDecodePlan.from_codestream(cs, sp, decoded=dec)
+                keep[i] = (plan, blk, kind)
+                server.submit(plan, tag=i, arrival_ns=arrivals[i])
+            except Exception as e:  # noqa: BLE001
+                errors.append(e)
+
+        px = 0
+        done = failed = 0
+        i = 0
+        deadline = time.perf_counter() + max(seconds, 3.0) + 120.0
+        start = clock_ns()
+        with ThreadPoolExecutor(workers) as ex:
+            while done < n:
+                if errors:
+                    server.close()
+                    raise SystemExit(f"serving from codestreams ({label}): {errors[0]}")
+                if time.perf_counter() > deadline:
+                    server.close()
+                    raise SystemExit(f"serving from codestreams ({label}): {n - done} of {n} requests without "
+                                     "a result two minutes after the stream")
+                now = clock_ns()
+                while i < n and arrivals[i] <= now:
+                    ex.submit(work, i)
+                    i += 1
+                wait_us = 2000 if i >= n else int(max(0, min(2000, (arrivals[i] - clock_ns()) // 1000)))
+                for r in server.poll(4096, timeout_us=wait_us):
+                    plan, blk, kind = keep.pop(r.tag)
+                    px += plan.output_pixels
+                    if r.image is not None:
+                        if r.status == 0:
+                            check(kind, sched[r.tag][2], r.image)
+                        r.image.close()
+                    free[kind].put(blk)
+                    failed += r.status != 0
+                    done += 1
+        end = clock_ns()
+        st = server.stats()
+        server.close()
+        wall = (end - start) / 1e9
+        log(f"[serve-cs] {label}: {n} requests in {wall:.2f} s, {px / wall / 1e6:.1f} Mpx/s, "
+            f"p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms")
+        return dict(requests=n, completed=st["completed"], failed=failed, seconds=round(wall, 3),
+                    requests_per_s=round(n / wall, 1), mpx_per_s=round(px / wall / 1e6, 1),
+                    latency_ms={"p50": round(st["p50_ms"], 3), "p99": round(st["p99_ms"], 3)})
+
+    # closed loop: a one-second schedule's requests all arrive now
+    burst = workloads.c5_schedule(1.0)
+    now = clock_ns() + 5_000_000
+    cap = run(burst, [now] * len(burst), "capacity (closed loop)")
+    rate = max(10.0, 0.5 * cap["requests_per_s"])
+    sched = workloads.c5_schedule(seconds, rate=rate)
+    start = clock_ns() + 20_000_000
+    open_loop = run(sched, [start + int(t * 1e9) for t, _, _ in sched], f"Poisson {rate:.0f} req/s")
+    for slab in slabs:
+        slab.close()
+    return {
+        "workload": "c5 mix from codestreams: host entropy decode (tiles a region needs) into page-locked blocks "
+                    f"on {workers} host workers, per-request DMA upload, serving loop; latency from arrival",
+        "capacity": cap,
+        "poisson_half_capacity": dict(open_loop, offered_requests_per_s=round(rate, 1)),
+        "bytes_checked": dict(checked),
+    }
 
 
 def serve(session, seconds, cache, budget_gb, rate=None):
@@ -348,6 +487,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="seconds of the c5 serving stream to run after the batch (0: none)")
     ap.add_argument("--serve-uploads", choices=["resident", "per-request"], default="resident")
+    ap.add_argument("--serve-codestream-seconds", type=float, default=4.0,
+                    help="Poisson run of the c5 mix served from codestreams (0: skip)")
     ap.add_argument("--serve-loaded-rate", type=float, default=40000.0,
                     help="arrivals per second of a second, loaded serving run (0: none)")
     ap.add_argument("--serve-budget-gb", type=float, default=0.0, help="admission budget (0: the server's default, 8 GiB or 3/4 of free HBM if less)")
@@ -517,6 +658,9 @@ def main():
             # coefficients from host memory: PCIe inside the latency
             out["serving_uploads"] = serve(session, min(args.serve_seconds, 3.0), False, args.serve_budget_gb,
                                            args.serve_rate)
+        if args.serve_codestream_seconds > 0:
+            out["serving_codestreams"] = serve_codestreams(session, args.serve_codestream_seconds,
+                                                           args.serve_budget_gb)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(items, plans, dense, offsets)
     if not args.no_end_to_end:

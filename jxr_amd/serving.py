@@ -6,6 +6,7 @@ The nearest caller in the reference is the batch decoder's grouped
 submission (crates/jxr/src/batch/cuda.rs:20-156); admission and continuous
 batching have no counterpart there."""
 import ctypes as C
+import threading
 from dataclasses import dataclass
 from typing import Dict, List, Optional
 
@@ -43,13 +44,19 @@ class HipServer:
         self._h = h
         self._s = session
         self._keep: Dict[int, HipDecodePlan] = {}  # plans of requests without a result yet
+        # submit may run on several threads while another polls: a plan is
+        # recorded under the lock its submission holds, so a result polled
+        # at once still finds (and releases) it
+        self._lock = threading.Lock()
 
     def submit(self, plan: HipDecodePlan, dst: Optional[int] = None, dst_bytes: int = 0, tag: int = 0,
                arrival_ns: int = 0) -> int:
         t = C.c_uint64()
-        check(load().jxrhip_server_submit(self._h, plan.handle, C.c_void_p(dst) if dst else None,
-                                          dst_bytes if dst else 0, tag, arrival_ns, C.byref(t)), "server_submit")
-        self._keep[t.value] = plan
+        with self._lock:
+            check(load().jxrhip_server_submit(self._h, plan.handle, C.c_void_p(dst) if dst else None,
+                                              dst_bytes if dst else 0, tag, arrival_ns, C.byref(t)),
+                  "server_submit")
+            self._keep[t.value] = plan
         return t.value
 
     def poll(self, capacity: int = 256, timeout_us: int = 0) -> List[ServerResult]:
@@ -57,7 +64,8 @@ class HipServer:
         n = load().jxrhip_server_poll(self._h, buf, capacity, timeout_us)
         out = []
         for r in buf[:n]:
-            self._keep.pop(r.ticket, None)
+            with self._lock:
+                self._keep.pop(r.ticket, None)
             img = ResidentHipImage(self._s, C.c_void_p(r.image)) if r.image else None
             out.append(ServerResult(r.ticket, r.tag, r.status, r.stage, r.latency_ms, r.queue_ms, img))
         return out
