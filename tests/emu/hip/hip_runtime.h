@@ -72,6 +72,8 @@ inline unsigned atomicAdd(unsigned *p, unsigned v) {
 }
 // one thread runs a workgroup: it is its own first lane
 inline unsigned __builtin_amdgcn_readfirstlane(unsigned v) { return v; }
+// wave issue priority: one thread per workgroup here, nothing to order
+inline void __builtin_amdgcn_s_setprio(short) {}
 inline float __uint_as_float(unsigned u) {
   float f;
   memcpy(&f, &u, 4);
