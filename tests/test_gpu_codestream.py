@@ -137,3 +137,34 @@ def test_decode_into_pinned_host_blocks_on_gpu(session):
     from jxr_amd.abi import load
     import ctypes as C
     assert load().jxrhip_host_free(session.handle, C.c_void_p(16)) == abi.ERR_INVALID_ARGUMENT
+
+
+def arena_path_and_codestream_path(session, enc, spec, **writer):
+    """Device output of the encoder's own arenas and of the same arenas
+    written to a codestream and entropy-decoded again, into pinned storage."""
+    cs = Codestream(encode_codestream(enc, **writer))
+    block = session.allocate_host(4 * sum(cs.coefficient_count(spec.scale)))
+    dec = cs.decode(scale=spec.scale, into=block)
+    got = session.decode_to_host(HipDecodePlan.from_codestream(cs, spec, decoded=dec))
+    want = session.decode_to_host(HipDecodePlan.from_encoded(enc, spec))
+    block.close()
+    return got, want
+
+
+@pytest.mark.parametrize("frequency", [False, True])
+def test_4096_gray_u16_config_from_codestream_on_gpu(session, frequency):
+    # 4096x4096 gray u16 pink noise, 8x8 soft tiles of 512 px, overlap 2,
+    # quality 75, all bands -> u16: from its codestream (spatial or
+    # frequency packets, 64 tiles on host threads) the device output equals
+    # that of the encoder's arenas, byte for byte
+    (enc, spec), = workloads.c2()
+    got, want = arena_path_and_codestream_path(session, enc, spec, frequency=frequency)
+    assert got.nbytes == 4096 * 4096 * 2 and np.array_equal(got, want)
+
+
+def test_1024_rgba_f32_config_from_codestream_on_gpu(session):
+    # 1024x1024 RGBA f32 lossless, separate alpha (Annex-A container),
+    # overlap 1 -> planar f32 + alpha: two images of the batch of 128
+    for enc, spec in workloads.c4(count=2):
+        got, want = arena_path_and_codestream_path(session, enc, spec, annex_a=True)
+        assert got.nbytes == 1024 * 1024 * 16 and np.array_equal(got, want)
