@@ -149,3 +149,69 @@ def test_held_results_slow_admission_but_every_request_finishes(session, request
         assert st["completed"] == 10 and st["failed"] == 0 and st["rejected"] == 0
     finally:
         server.close()
+
+
+def test_codestream_requests_from_worker_threads_match_alone(session):
+    """Requests arriving as files: four worker threads entropy-decode each
+    one into a page-locked block of the session (tiles a region needs only),
+    plan it and submit it while the main thread polls; every result equals
+    the device decode of the encoder's own arenas for the same image, alone."""
+    import queue
+    from concurrent.futures import ThreadPoolExecutor
+    from jxr_amd.codestream import Codestream, encode_codestream
+    qp = quality_qp(75)
+    items = []  # (codestream, spec, tiles, encoder arenas)
+    for i, (w, h) in enumerate([(320, 256), (512, 384), (208, 528)]):
+        px = synth.gradient_noise(w, h, 120 + i, depth=8)
+        enc = EncodedImage(px, color=abi.COLOR_YUV420 if i % 2 else abi.COLOR_YUV444, overlap=1 + i % 2,
+                           scaled=True, qp=(qp, qp, qp + 4), tiles=([8, w // 16 - 8], [h // 16]) if i == 2 else None)
+        items.append((Codestream(encode_codestream(enc)), workloads.rgb_out(8, i == 1), None, enc))
+    px = synth.gradient_noise(1024, 1024, 130, depth=8)
+    src = EncodedImage(px, color=abi.COLOR_YUV444, overlap=2, tiles=([16] * 4, [16] * 4), scaled=True,
+                       qp=(qp, qp, qp + 4))
+    src_cs = Codestream(encode_codestream(src))
+    for (x, y) in [(256, 256), (520, 128), (0, 768)]:
+        spec = workloads.rgb_out(8, False)
+        spec.crop = (x, y, 256, 256)
+        items.append((src_cs, spec, src_cs.region_tiles(spec.crop), RegionImage(src, spec.crop)))
+    alone = [session.decode_to_host(HipDecodePlan(image_desc(e, sp), keepalive=e)) for _, sp, _, e in items]
+    need = max(sum(cs.coefficient_count(sp.scale, m)) for cs, sp, m, _ in items) * 4
+    blocks = [session.allocate_host(need) for _ in range(6)]
+    free = queue.SimpleQueue()
+    for b in blocks:
+        free.put(b)
+    order = [k % len(items) for k in range(4 * len(items))]
+    server = HipServer(session, hbm_budget_bytes=64 << 20, max_wave_images=4, max_waves=2)
+    keep = {}
+
+    def work(n):
+        cs, sp, mask, _ = items[order[n]]
+        blk = free.get()
+        dec = cs.decode(scale=sp.scale, tiles=mask, threads=1, into=blk)
+        plan = HipDecodePlan.from_codestream(cs, sp, decoded=dec)
+        keep[n] = (plan, blk)
+        server.submit(plan, tag=n)
+
+    try:
+        done = 0
+        with ThreadPoolExecutor(4) as ex:
+            futs = [ex.submit(work, n) for n in range(len(order))]
+            deadline = time.time() + 60
+            while done < len(order) and time.time() < deadline:
+                for r in server.poll(64, timeout_us=2000):
+                    i = order[r.tag]
+                    assert r.status == 0, (r.tag, r.status, r.stage)
+                    got = r.image.readback()
+                    r.image.close()
+                    assert np.array_equal(got[: alone[i].nbytes], alone[i]), i
+                    _, blk = keep.pop(r.tag)
+                    free.put(blk)
+                    done += 1
+            for f in futs:
+                f.result()
+        assert done == len(order)
+        assert server.stats()["completed"] == len(order)
+    finally:
+        server.close()
+        for b in blocks:
+            b.close()
