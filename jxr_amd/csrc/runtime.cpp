@@ -923,6 +923,12 @@ struct CacheEntry {
   hipStream_t stream = nullptr;
 };
 
+struct StageBlock {
+  uint8_t *p = nullptr;
+  size_t bytes = 0;
+  hipEvent_t done = nullptr;  // recorded after the copy that reads the block
+};
+
 struct jxrhip_session {
   int device = 0;
   uint32_t cus = 0;
@@ -964,6 +970,12 @@ struct jxrhip_session {
   // device once per key and kept until the session ends (descriptors point
   // at them; each is a few kilobytes)
   std::map<std::vector<uint32_t>, DeviceBuffer> probe_terms;
+  // page-locked staging of descriptor uploads: a copy from pageable memory
+  // would wait for the DMA already queued (the coefficient uploads of the
+  // same submission), so descriptors are built in pinned blocks, copied
+  // asynchronously, and a block is reused once its copy's event completes
+  std::vector<StageBlock> stage_free, stage_busy;
+  size_t stage_free_bytes = 0;
 };
 
 static constexpr uint32_t kStatusSlots = 16384;
@@ -1006,6 +1018,67 @@ static hipError_t take_event(jxrhip_session *s, hipEvent_t *e) {
     return hipSuccess;
   }
   return hipEventCreateWithFlags(e, hipEventDisableTiming);
+}
+
+// A pinned staging block of at least `bytes`: the smallest free one, after
+// moving blocks whose copies have completed to the free list, or a new one
+// (64 KiB or the next power of two). Blocks beyond 256 MiB of free staging
+// are released.
+static constexpr size_t kStageKeepBytes = size_t(256) << 20;
+static hipError_t stage_get(jxrhip_session *s, size_t bytes, StageBlock &out) {
+  for (size_t i = 0; i < s->stage_busy.size();) {
+    if (hipEventQuery(s->stage_busy[i].done) == hipSuccess) {
+      s->stage_free_bytes += s->stage_busy[i].bytes;
+      s->stage_free.push_back(s->stage_busy[i]);
+      s->stage_busy[i] = s->stage_busy.back();
+      s->stage_busy.pop_back();
+    } else {
+      i++;
+    }
+  }
+  size_t best = SIZE_MAX;
+  for (size_t i = 0; i < s->stage_free.size(); i++)
+    if (s->stage_free[i].bytes >= bytes && (best == SIZE_MAX || s->stage_free[i].bytes < s->stage_free[best].bytes))
+      best = i;
+  if (best != SIZE_MAX) {
+    out = s->stage_free[best];
+    s->stage_free_bytes -= out.bytes;
+    s->stage_free[best] = s->stage_free.back();
+    s->stage_free.pop_back();
+    return hipSuccess;
+  }
+  while (s->stage_free_bytes > kStageKeepBytes && !s->stage_free.empty()) {
+    StageBlock &b = s->stage_free.back();
+    s->stage_free_bytes -= b.bytes;
+    (void)hipHostFree(b.p);
+    (void)hipEventDestroy(b.done);
+    s->stage_free.pop_back();
+  }
+  StageBlock b;
+  b.bytes = size_t(64) << 10;
+  while (b.bytes < bytes) b.bytes <<= 1;
+  hipError_t e = hipHostMalloc((void **)&b.p, b.bytes, hipHostMallocDefault);
+  if (e != hipSuccess) return e;
+  e = hipEventCreateWithFlags(&b.done, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    (void)hipHostFree(b.p);
+    return e;
+  }
+  out = b;
+  return hipSuccess;
+}
+// The block is read by a copy queued on `stream`: busy until that point.
+static hipError_t stage_release(jxrhip_session *s, StageBlock &b, hipStream_t stream) {
+  const hipError_t e = hipEventRecord(b.done, stream);
+  if (e != hipSuccess) {
+    // without its event the copy's end is unknown: wait for the stream
+    (void)hipStreamSynchronize(stream);
+    s->stage_free_bytes += b.bytes;
+    s->stage_free.push_back(b);
+    return e;
+  }
+  s->stage_busy.push_back(b);
+  return hipSuccess;
 }
 
 // Returns a cache entry's memory to the pool (the caller has made sure no
@@ -1087,6 +1160,12 @@ static void session_free(jxrhip_session *s) {
   if (s->status_dev) (void)hipFree(s->status_dev);
   if (s->status_host) (void)hipHostFree(s->status_host);
   for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
+  for (auto *list : {&s->stage_free, &s->stage_busy})
+    for (StageBlock &b : *list) {
+      (void)hipEventSynchronize(b.done);
+      (void)hipHostFree(b.p);
+      (void)hipEventDestroy(b.done);
+    }
   for (uint32_t i = 0; i < kStreams; i++)
     if (s->streams[i]) (void)hipStreamDestroy(s->streams[i]);
   if (s->t0) (void)hipEventDestroy(s->t0);
@@ -1425,10 +1504,20 @@ static jxrhip_status device_image(jxrhip_session *s, const jxrhip_plan *p, hipSt
     const jxrhip_status pst = sample_scale_terms(s, p, stream, terms);
     if (pst) return pst;
   }
-  std::vector<uint8_t> host(bytes, 0);
   DeviceBuffer b;
   if (cacheable) evict_until(s, bytes);
   HIP_TRY(s->pool.get(bytes, b, s->diag.scratch_hits, s->diag.scratch_misses), "descriptor buffer");
+  StageBlock stage;
+  {
+    const hipError_t se = stage_get(s, bytes, stage);
+    if (se != hipSuccess) {
+      s->pool.put(b);
+      return hip_fail(se, "descriptor staging");
+    }
+  }
+  // every byte the kernels read is written below; alignment gaps between
+  // the parts are never read, so the block is not cleared first
+  uint8_t *const host = stage.p;
   std::vector<DevPlane> dp(p->planes.size());
   size_t off = sizeof(DevImage) * ngroups;
   for (size_t i = 0; i < p->planes.size(); i++) {
@@ -1436,14 +1525,14 @@ static jxrhip_status device_image(jxrhip_session *s, const jxrhip_plan *p, hipSt
     DevPlane &d = dp[i];
     d.coef = coef[pp.arena];
     off = (off + 63) & ~size_t(63);
-    std::memcpy(host.data() + off, pp.recs.data(), pp.recs.size() * sizeof(MbRec));
+    std::memcpy(host + off, pp.recs.data(), pp.recs.size() * sizeof(MbRec));
     d.mbs = (const MbRec *)((uint8_t *)b.ptr + off);
     off += pp.recs.size() * sizeof(MbRec);
     if (!pp.colmask.empty()) {
-      std::memcpy(host.data() + off, pp.colmask.data(), pp.colmask.size() * 4);
+      std::memcpy(host + off, pp.colmask.data(), pp.colmask.size() * 4);
       d.colmask = (const uint32_t *)((uint8_t *)b.ptr + off);
       off += pp.colmask.size() * 4;
-      std::memcpy(host.data() + off, pp.rowmask.data(), pp.rowmask.size() * 4);
+      std::memcpy(host + off, pp.rowmask.data(), pp.rowmask.size() * 4);
       d.rowmask = (const uint32_t *)((uint8_t *)b.ptr + off);
       off += pp.rowmask.size() * 4;
     } else {
@@ -1469,10 +1558,10 @@ static jxrhip_status device_image(jxrhip_session *s, const jxrhip_plan *p, hipSt
         r.terms = terms[pp.probe];
         r.gain = q.gain;
         r.stride = q.stride;
-        std::memcpy(host.data() + off, pp.probe_cols.data(), pp.probe_cols.size() * 4);
+        std::memcpy(host + off, pp.probe_cols.data(), pp.probe_cols.size() * 4);
         r.cols = (const uint32_t *)((uint8_t *)b.ptr + off);
         off += pp.probe_cols.size() * 4;
-        std::memcpy(host.data() + off, pp.probe_rows.data(), pp.probe_rows.size() * 4);
+        std::memcpy(host + off, pp.probe_rows.data(), pp.probe_rows.size() * 4);
         r.rows = (const uint32_t *)((uint8_t *)b.ptr + off);
         off += pp.probe_rows.size() * 4;
       }
@@ -1485,15 +1574,23 @@ static jxrhip_status device_image(jxrhip_session *s, const jxrhip_plan *p, hipSt
       off = (off + 63) & ~size_t(63);
       dv.reduce = (const DevReduce *)((uint8_t *)b.ptr + off);
       for (size_t k = 0; k < p->group_planes[g].size(); k++) {
-        std::memcpy(host.data() + off, &dr[p->group_planes[g][k]], sizeof(DevReduce));
+        std::memcpy(host + off, &dr[p->group_planes[g][k]], sizeof(DevReduce));
         off += sizeof(DevReduce);
       }
     }
-    std::memcpy(host.data() + g * sizeof(DevImage), &dv, sizeof dv);
+    std::memcpy(host + g * sizeof(DevImage), &dv, sizeof dv);
   }
-  HIP_TRY(hipMemcpyAsync(b.ptr, host.data(), bytes, hipMemcpyHostToDevice, stream), "descriptor upload");
-  // pageable copies are staged before the call returns, so `host` may go
-  HIP_TRY(hipStreamSynchronize(stream), "descriptor upload");
+  // stream-ordered ahead of the kernels that read it, from pinned memory, so
+  // nothing waits here for the coefficient uploads queued before it
+  {
+    const hipError_t ce = hipMemcpyAsync(b.ptr, host, bytes, hipMemcpyHostToDevice, stream);
+    const hipError_t re = stage_release(s, stage, stream);
+    if (ce != hipSuccess || re != hipSuccess) {
+      (void)hipStreamSynchronize(stream);
+      s->pool.put(b);
+      return hip_fail(ce != hipSuccess ? ce : re, "descriptor upload");
+    }
+  }
   if (cacheable) {
     // the eviction above may have removed the stale entry already
     it = s->desc_cache.find(p->id);
