@@ -1481,8 +1481,10 @@ static jxrhip_status device_image(jxrhip_session *s, const jxrhip_plan *p, hipSt
   const bool cacheable = p->arenas[0]->identity && (p->arena_count < 2 || p->arenas[1]->identity);
   auto it = s->desc_cache.find(p->id);
   if (cacheable && it != s->desc_cache.end()) {
-    // valid while the coefficient buffers it points at are the same
+    // valid while the coefficient buffers it points at are the same; its
+    // copy may still be running on the stream that made it
     if (it->second.deps[0] == coef[0] && it->second.deps[1] == coef[1]) {
+      HIP_TRY(await_upload(s, it->second, stream), "descriptor dependency");
       it->second.last_use = ++s->tick;
       *out = (const DevImage *)it->second.buf.ptr;
       return JXRHIP_OK;
@@ -1599,8 +1601,18 @@ static jxrhip_status device_image(jxrhip_session *s, const jxrhip_plan *p, hipSt
       drop_entry(s, it->second);
       pins = it->second.pins;
     }
-    s->cache_bytes += b.bytes;
     CacheEntry ce{b, ++s->tick, {coef[0], coef[1]}, pins};
+    // the copy is asynchronous: work on other streams that finds this entry
+    // waits for it (await_upload)
+    hipError_t e = take_event(s, &ce.ready);
+    if (e == hipSuccess) e = hipEventRecord(ce.ready, stream);
+    if (e != hipSuccess) {
+      if (ce.ready) s->free_events.push_back(ce.ready);
+      ce.ready = nullptr;
+      (void)hipStreamSynchronize(stream);
+    }
+    ce.stream = stream;
+    s->cache_bytes += b.bytes;
     s->desc_cache[p->id] = ce;
   } else {
     owner->transient.push_back(b);
@@ -2478,6 +2490,7 @@ static jxrhip_status resident_image(jxrhip_session *s, const jxrhip_plan *p, hip
   auto it = s->desc_cache.find(p->id);
   if (it == s->desc_cache.end() || it->second.deps[0] != coef[0] || it->second.deps[1] != coef[1])
     return fail(JXRHIP_ERR_INVALID_ARGUMENT, "descriptors are not resident (jxrhip_session_upload first)");
+  HIP_TRY(await_upload(s, it->second, stream), "descriptor dependency");
   it->second.last_use = ++s->tick;
   *out = (const DevImage *)it->second.buf.ptr;
   return JXRHIP_OK;

@@ -121,3 +121,37 @@ def test_autotuned_shapes_keep_parity(corpus):
         assert s.bucket_tile(key) == (0, 0)
     finally:
         s.close()
+
+
+def test_fresh_uploads_read_from_other_streams(session):
+    """Coefficient and descriptor uploads are asynchronous on the stream of
+    the submission that made them: the same large plans submitted again at
+    once (the per-image path spreads submissions over the session's four
+    streams, and a batch runs on its own) read the cached copies only after
+    those copies have finished. Repeated after emptying the caches, each
+    time with every output compared to the oracle."""
+    from jxr_amd import synth, workloads
+    from jxr_amd.decoder import EncodedImage
+    from cases import quality_qp
+    qp = quality_qp(60)
+    items = []
+    for i, (w, h) in enumerate([(2048, 1536), (1536, 2048), (1024, 1024)]):
+        px = synth.gradient_noise(w, h, 300 + i, depth=8)
+        enc = EncodedImage(px, color=abi.COLOR_YUV420, overlap=1 + i % 2, scaled=True, qp=(qp, qp, qp + 2))
+        spec = workloads.rgb_out(8, True)
+        items.append((enc, HipDecodePlan.from_encoded(enc, spec), O.decode(image_desc(enc, spec))))
+    for rep in range(3):
+        session.clear_cache()
+        subs = []
+        for _ in range(2):
+            for enc, plan, want in items:
+                subs.append((session.submit(plan), want))
+        bufs = [session.allocate_destination(plan.output_bytes) for _, plan, _ in items]
+        batch = session.submit_batch([p for _, p, _ in items], [b.ptr for b in bufs], abi.BATCH_PERSISTENT)
+        for sub, want in subs:
+            img = sub.wait()
+            assert np.array_equal(img.readback(), want), rep
+            img.close()
+        assert not batch.wait().any()
+        for b, (_, _, want) in zip(bufs, items):
+            assert np.array_equal(b.to_host(0, want.nbytes), want), rep
