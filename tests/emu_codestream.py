@@ -2,7 +2,8 @@
 of the library (optionally under AddressSanitizer): writer/reader round
 trips over the encoder's layouts, the reference's T.835 file, and damaged
 codestreams (truncated, bit-flipped, bytes overwritten), each of which must
-either decode or fail with a status, never read or write out of bounds."""
+either decode or fail with a status, never read or write out of bounds, half
+of them decoded into caller storage of exactly the counted size."""
 import os
 import random
 import sys
@@ -60,7 +61,18 @@ def main():
                 b[i] = rng.randrange(256)
             try:
                 cs = Codestream(bytes(b))
-                cs.decode(threads=2)
+                if trial % 2:
+                    cs.decode(threads=2)
+                else:
+                    # into caller storage of exactly the counted size (a
+                    # heap block of its own, so a write past it is caught)
+                    n, na = cs.coefficient_count()
+                    store = np.empty(n + na, np.int32)
+                    keep = (store, store.ctypes.data, store.nbytes)
+
+                    class Storage:
+                        ptr, nbytes = keep[1], keep[2]
+                    cs.decode(threads=2, into=Storage)
                 decoded += 1
             except JxrHipError as e:
                 if e.code not in (abi.ERR_SYNTAX, abi.ERR_UNSUPPORTED, abi.ERR_OVERFLOW):
