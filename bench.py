@@ -549,8 +549,8 @@ def main():
         tuned = session.autotune(plans, ptrs, repeats=3)
         for c in tuned:
             log(f"[bench] autotune bucket {c['bucket_key']:#x}: {c['images']} images, tile "
-                f"{c['tile'][0]}x{c['tile'][1]} {c['best_ms']:.3f} ms (default {c['default_ms']:.3f} ms, "
-                f"{c['candidates']} shapes)")
+                f"{c['tile'][0]}x{c['tile'][1]} at {c['groups_per_cu']} workgroups/CU {c['best_ms']:.3f} ms "
+                f"(default {c['default_ms']:.3f} ms, {c['candidates']} shapes)")
         log(f"[bench] autotune took {time.perf_counter() - t0:.1f} s")
     graph = None
     if args.no_graph:
@@ -609,6 +609,7 @@ def main():
                     + ("" if args.no_graph else ", replayed from a captured HIP graph")
                     + (", autotuned tile shapes" if tuned else ""),
             "tiles": {f"{c['bucket_key']:#x}": f"{c['tile'][0]}x{c['tile'][1]}" for c in tuned},
+            "workgroups_per_cu": {f"{c['bucket_key']:#x}": c["groups_per_cu"] for c in tuned},
             "parallelism": "single-gpu",
         },
         "latency_ms": {"p50": round(percentile(batched["wall_ms"], 50), 4),

@@ -532,20 +532,26 @@ typedef struct jxrhip_tile_choice {
   uint32_t candidates;             /* shapes timed                           */
   double best_ms;                  /* median device time at that shape       */
   double default_ms;               /* median device time at the untuned shape*/
+  uint32_t groups_per_cu;          /* workgroups per CU the persistent launch uses */
+  uint32_t reserved;
 } jxrhip_tile_choice;
 
 /* Autotuner: times every candidate tile shape on each shape bucket of the
  * batch (one persistent launch of the bucket alone, `repeats` timed runs
- * after an untimed one) and keeps the fastest per bucket key for later
- * submissions of this session, per-image and persistent. The destinations
- * receive the images. Writes up to `capacity` choices and the bucket count.
- * A fixed shape from jxrhip_session_set_tile still takes precedence. No
- * counterpart in the reference. */
+ * after an untimed one), then, at the fastest shape, one and two fewer
+ * workgroups per CU than fit; keeps the fastest per bucket key (a smaller
+ * workgroup count only when more than 1% faster) for later persistent
+ * submissions of this session, and the shape also for per-image ones. The
+ * destinations receive the images. Writes up to `capacity` choices and the
+ * bucket count. A fixed shape from jxrhip_session_set_tile, or a count from
+ * jxrhip_session_set_occupancy, still takes precedence. No counterpart in
+ * the reference. */
 jxrhip_status jxrhip_session_autotune(jxrhip_session *session, const jxrhip_plan *const *plans, uint32_t count,
                                       void *const *device_dsts, uint32_t repeats, jxrhip_tile_choice *choices,
                                       uint32_t capacity, uint32_t *bucket_count);
 /* Installs (or, with 0 x 0, removes) a bucket's tile shape, e.g. one saved
- * from an earlier autotune; a shape whose LDS does not fit is ignored. */
+ * from an earlier autotune, with as many workgroups per CU as fit; a shape
+ * whose LDS does not fit is ignored. */
 jxrhip_status jxrhip_session_set_bucket_tile(jxrhip_session *session, uint64_t bucket_key, uint32_t tile_mbs_x,
                                              uint32_t tile_mbs_y);
 /* The bucket's installed shape, 0 x 0 when none. */

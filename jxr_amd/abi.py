@@ -214,6 +214,8 @@ class TileChoice(C.Structure):
         ("candidates", C.c_uint32),
         ("best_ms", C.c_double),
         ("default_ms", C.c_double),
+        ("groups_per_cu", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 

@@ -945,6 +945,9 @@ struct jxrhip_session {
   uint32_t next_stream = 0;
   uint32_t tile_x = 0, tile_y = 0, groups_per_cu = 0;
   std::map<uint64_t, std::pair<uint32_t, uint32_t>> tuned;  // bucket key -> tile shape
+  // bucket key -> workgroups per CU the autotuner chose below the most that
+  // fit (absent: as many as fit)
+  std::map<uint64_t, uint32_t> tuned_groups;
   uint32_t live_graphs = 0, live_servers = 0;  // objects that must be destroyed before the session
   // owners of the session: the caller's handle, plus every submission,
   // result image and batch not yet released. The last one frees it, so an
@@ -2146,7 +2149,7 @@ static void release_prepared(jxrhip_session *s, PreparedBatch &pb) {
 static jxrhip_status prepare_persistent(jxrhip_session *s, const jxrhip_plan *const *plans, uint32_t count,
                                         void *const *dsts, const uint32_t *force, PreparedBatch &pb,
                                         hipStream_t upload_stream = nullptr, Slot *pinned_slots = nullptr,
-                                        size_t pinned_capacity = 0) {
+                                        size_t pinned_capacity = 0, uint32_t force_groups = 0) {
   hipStream_t stream = upload_stream ? upload_stream : s->streams[0];
   pb.plans.assign(plans, plans + count);
   pb.dsts.assign(dsts, dsts + count);
@@ -2198,7 +2201,16 @@ static jxrhip_status prepare_persistent(jxrhip_session *s, const jxrhip_plan *co
     int resident = 0;
     if (!s->groups_per_cu)
       HIP_TRY(kernel_occupancy(L.storage, L.reduced, (size_t)L.words * 4, &resident), "occupancy");
-    const uint32_t per_cu = s->groups_per_cu ? s->groups_per_cu : (uint32_t)std::max(resident, 1);
+    uint32_t per_cu = s->groups_per_cu ? s->groups_per_cu : (uint32_t)std::max(resident, 1);
+    if (!s->groups_per_cu) {
+      // fewer than fit when the autotuner found that faster for the bucket
+      uint32_t want = force_groups;
+      if (!want && !force) {
+        auto tg = s->tuned_groups.find(kv.first);
+        if (tg != s->tuned_groups.end()) want = tg->second;
+      }
+      if (want) per_cu = std::min(per_cu, want);
+    }
     uint32_t grid = std::min<uint32_t>(L.items, s->cus * per_cu);
     L.grid = std::max<uint32_t>((grid + 7u) & ~7u, 8u);
     pb.launches.push_back(L);
@@ -2738,6 +2750,7 @@ extern "C" jxrhip_status jxrhip_session_set_bucket_tile(jxrhip_session *s, uint6
   std::lock_guard<std::mutex> g(s->lock);
   if (tx == 0) s->tuned.erase(key);
   else s->tuned[key] = {tx, ty};
+  s->tuned_groups.erase(key);
   return JXRHIP_OK;
 }
 
@@ -2796,12 +2809,13 @@ extern "C" jxrhip_status jxrhip_session_autotune(jxrhip_session *s, const jxrhip
       if ((size_t)w * 4 > std::min<size_t>(s->max_lds, 64 * 1024)) continue;
       shapes.push_back({c[0], c[1]});
     }
-    double best_ms = 1e30, default_ms = 0;
-    std::pair<uint32_t, uint32_t> best{dx, dy};
-    for (auto &sh : shapes) {
-      const uint32_t force[2] = {sh.first, sh.second};
+    // median device time of the bucket alone at a shape and a number of
+    // workgroups per CU (0: as many as fit)
+    auto time_shape = [&](uint32_t tx, uint32_t ty, uint32_t groups, double &med) -> jxrhip_status {
+      const uint32_t force[2] = {tx, ty};
       PreparedBatch pb;
-      jxrhip_status st = prepare_persistent(s, bp.data(), (uint32_t)bp.size(), bd.data(), force, pb);
+      jxrhip_status st = prepare_persistent(s, bp.data(), (uint32_t)bp.size(), bd.data(), force, pb, nullptr,
+                                            nullptr, 0, groups);
       std::vector<double> ms;
       for (uint32_t r = 0; st == JXRHIP_OK && r <= repeats; r++) {
         if (hipEventRecord(s->t0, s0) != hipSuccess) st = fail(JXRHIP_ERR_DEVICE, "hipEventRecord");
@@ -2816,7 +2830,15 @@ extern "C" jxrhip_status jxrhip_session_autotune(jxrhip_session *s, const jxrhip
       release_prepared(s, pb);
       if (st) return st;
       std::sort(ms.begin(), ms.end());
-      const double med = ms[ms.size() / 2];
+      med = ms[ms.size() / 2];
+      return JXRHIP_OK;
+    };
+    double best_ms = 1e30, default_ms = 0;
+    std::pair<uint32_t, uint32_t> best{dx, dy};
+    for (auto &sh : shapes) {
+      double med = 0;
+      const jxrhip_status st = time_shape(sh.first, sh.second, 0, med);
+      if (st) return st;
       if (sh.first == dx && sh.second == dy) default_ms = med;
       if (med < best_ms) {
         best_ms = med;
@@ -2824,6 +2846,30 @@ extern "C" jxrhip_status jxrhip_session_autotune(jxrhip_session *s, const jxrhip
       }
     }
     s->tuned[kv.first] = best;
+    // then the workgroups per CU at that shape: as many as fit, or one or
+    // two fewer (less L2 and LDS pressure per compute unit) when that is
+    // faster by more than 1%
+    s->tuned_groups.erase(kv.first);
+    uint32_t groups = 0, fit = 0;
+    if (!s->groups_per_cu) {
+      uint32_t words = 0;
+      for (const jxrhip_plan *p : bp) words = std::max(words, tile_words(p, best.first, best.second));
+      int resident = 0;
+      HIP_TRY(kernel_occupancy(p0->dev.storage, p0->scale != JXRHIP_SCALE_FULL ? 1 : 0, (size_t)words * 4,
+                               &resident),
+              "occupancy");
+      fit = (uint32_t)std::max(resident, 1);
+      for (uint32_t g2 = fit > 1 ? fit - 1 : 0; g2 >= 1 && g2 + 2 >= fit; g2--) {
+        double med = 0;
+        const jxrhip_status st = time_shape(best.first, best.second, g2, med);
+        if (st) return st;
+        if (med < best_ms * 0.99) {
+          best_ms = med;
+          groups = g2;
+        }
+      }
+      if (groups) s->tuned_groups[kv.first] = groups;
+    }
     if (choices && k < capacity) {
       jxrhip_tile_choice &c = choices[k];
       c.bucket_key = kv.first;
@@ -2833,6 +2879,8 @@ extern "C" jxrhip_status jxrhip_session_autotune(jxrhip_session *s, const jxrhip
       c.candidates = (uint32_t)shapes.size();
       c.best_ms = best_ms;
       c.default_ms = default_ms;
+      c.groups_per_cu = groups ? groups : (s->groups_per_cu ? s->groups_per_cu : fit);
+      c.reserved = 0;
     }
     k++;
   }

@@ -687,9 +687,9 @@ class HipDecoderSession:
         return HipBatch(self, h, n)
 
     def autotune(self, plans: Sequence[HipDecodePlan], device_ptrs: Sequence[int], repeats: int = 3):
-        """Times the candidate tile shapes of every shape bucket of the batch
-        and keeps the fastest per bucket (jxrhip_session_autotune). Returns
-        one dict per bucket."""
+        """Times the candidate tile shapes of every shape bucket of the batch,
+        then fewer workgroups per CU at the best shape, and keeps the fastest
+        per bucket (jxrhip_session_autotune). Returns one dict per bucket."""
         n = len(plans)
         parr = (C.c_void_p * max(n, 1))(*[p.handle.value for p in plans])
         darr = (C.c_void_p * max(n, 1))(*[int(d) for d in device_ptrs])
@@ -698,7 +698,8 @@ class HipDecoderSession:
         nb = C.c_uint32()
         check(load().jxrhip_session_autotune(self._h, parr, n, darr, repeats, out, cap, C.byref(nb)), "autotune")
         return [{"bucket_key": c.bucket_key, "tile": (c.tile_mbs_x, c.tile_mbs_y), "images": c.images,
-                 "candidates": c.candidates, "best_ms": c.best_ms, "default_ms": c.default_ms}
+                 "candidates": c.candidates, "best_ms": c.best_ms, "default_ms": c.default_ms,
+                 "groups_per_cu": c.groups_per_cu}
                 for c in out[: min(nb.value, cap)]]
 
     def calibrate_valu(self, iterations: int = 20000):
