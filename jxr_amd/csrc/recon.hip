@@ -1882,6 +1882,7 @@ struct TileShared {
   int any_lp, any_sub, any_pot;
   uint32_t out_mag;  // OR of magbits of every sample the tile's output stage can read
   Slot next_rec;     // one-thread workgroups: the next tile's slot record
+  int interior;      // the last tile's setup was of an interior tile (see plane_reposition)
 };
 
 // A stage's per-plane item starts, read once into registers before its loop.
@@ -2081,6 +2082,34 @@ __device__ __forceinline__ void plane_setup(TileShared &ts, int pl, uint32_t til
   w.mD = div_magic(w.wD);
 }
 
+// Setup of a tile that follows an interior tile of the same slot and is
+// interior itself (two macroblocks or more from every window edge, so no
+// range is clipped and the tile is full size): every region size, work
+// count, LDS offset and division multiplier equals the previous tile's, and
+// only the fields that depend on the tile's position are written, as
+// plane_setup computes them.
+__device__ __forceinline__ void plane_reposition(TileShared &ts, int pl, int tmx, int tmy, int nx, int ny) {
+  const DevImage &im = ts.im;
+  const DevPlane &p = im.planes[pl];
+  PlaneGeo &g = ts.g[pl];
+  g.sox = tmx * (int)g.mw - 4;
+  g.soy = tmy * (int)g.mh - kSampleHaloRows;
+  const int h = (int)low_halo(p.bc, p.overlap);
+  g.lox = (tmx - h) * (int)p.bc;
+  g.loy = (tmy - h) * (int)p.br;
+  ts.br[pl] = block_range(p, g, tmx, tmy, nx, ny);
+  ts.cr[pl] = corner_range(im, p, tmx, tmy, nx, ny);
+  ts.colb[pl] = boundary_bits(p.colmask, im.mbx, tmx, nx);
+  ts.rowb[pl] = boundary_bits(p.rowmask, im.mby, tmy, ny);
+  if (pl == 0) {
+    ts.tmx = tmx;
+    ts.tmy = tmy;
+    ts.nx = nx;
+    ts.ny = ny;
+    ts.out_mag = 0;
+  }
+}
+
 // Work assignment. Workgroups are dispatched to the eight XCDs round-robin
 // (b mod 8), and the items of a launch are split into eight contiguous
 // shares, one per XCD, so tiles processed at the same time on one XCD are
@@ -2271,9 +2300,25 @@ __device__ __forceinline__ void process_tile(const Slot &slot, uint32_t tile, co
     for (int i = threadIdx.x; i < (int)(sizeof(DevImage) / sizeof(v4u)); i += blockDim.x) dst[i] = src[i];
     __syncthreads();
   }
-  for (int pl = threadIdx.x; pl < (int)ts.im.plane_count; pl += blockDim.x)
-    plane_setup<R>(ts, pl, tile, slot.tiles_x, (int)args.tile_x, (int)args.tile_y);
+  bool interior = false, fast_setup = false;
+  int itmx = 0, itmy = 0;
+  if constexpr (!R) {
+    const uint32_t tyi = tile / slot.tiles_x;
+    itmx = (int)(tile - tyi * slot.tiles_x) * (int)args.tile_x;
+    itmy = (int)tyi * (int)args.tile_y;
+    interior = itmx >= 2 && itmy >= 2 && itmx + (int)args.tile_x + 2 <= (int)ts.im.mbx &&
+               itmy + (int)args.tile_y + 2 <= (int)ts.im.mby;
+    // ts.interior is the previous tile's, written after this tile's setup
+    fast_setup = !reload && interior && ts.interior;
+  }
+  for (int pl = threadIdx.x; pl < (int)ts.im.plane_count; pl += blockDim.x) {
+    if (fast_setup)
+      plane_reposition(ts, pl, itmx, itmy, (int)args.tile_x, (int)args.tile_y);
+    else
+      plane_setup<R>(ts, pl, tile, slot.tiles_x, (int)args.tile_x, (int)args.tile_y);
+  }
   __syncthreads();
+  if (threadIdx.x == 0) ts.interior = interior ? 1 : 0;  // read again only after the next tile's first barrier
   const DevImage &im = ts.im;
   const int np = (int)im.plane_count;
   const int tmx = ts.tmx, tmy = ts.tmy, nx = ts.nx, ny = ts.ny;
@@ -2471,6 +2516,7 @@ __global__ void __launch_bounds__(kThreads, kMinWavesPerSimd) jxr_reconstruct_ti
   const uint32_t G = gridDim.x, b = blockIdx.x;
   uint32_t word = 0;  // lanes 0-7: word `lane` of the next tile's slot record
   if (threadIdx.x == 0) {
+    ts.interior = 0;
     q.sh = Shares{args.item_count, (G & 7u) == 0 ? 8u : 1u, (G & 7u) == 0 ? G >> 3 : G};
     const uint32_t x = q.sh.count == 8 ? (b & 7u) : 0u;
     q.r = x;
