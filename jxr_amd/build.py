@@ -15,6 +15,11 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libjxr_hip.so")
 SOURCES = ["recon.hip", "runtime.cpp", "analysis.cpp", "codestream.cpp"]
+# The reconstruction kernels are scheduled with LLVM's iterative ILP
+# strategy: at the kernels' fixed four waves per SIMD it orders their long
+# dependency chains better than the default (about 1% less device time per
+# batch for integer and float outputs alike).
+SOURCE_FLAGS = {"recon.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 
@@ -32,7 +37,7 @@ def build(verbose: bool = False) -> str:
     for src in SOURCES:
         obj = os.path.join(tmpdir, src + ".o")
         objs.append(obj)
-        procs.append(subprocess.Popen([*flags, "-c", os.path.join(CSRC, src), "-o", obj],
+        procs.append(subprocess.Popen([*flags, *SOURCE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj],
                                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
     outs = [(p.communicate()[0], p.returncode) for p in procs]
     tmp = LIB + ".tmp"
